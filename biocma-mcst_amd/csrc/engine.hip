@@ -1,0 +1,966 @@
+// engine.hip -- host side of libmcst: runs, flowmap tables, device upload,
+// the per-step launch sequence and the C ABI declared in include/mcst.h.
+//
+// The engine is the MI355X counterpart of the reference's SimulationUnit +
+// CycleFunctors + ParticlesContainer + ReactorDomain + ScalarSimulation for
+// one batch of runs (apps/libs/simulation/public/simulation/simulation.hpp:44-239,
+// apps/libs/simulation/public/simulation/kernels/kernels.hpp:25-224).  All
+// state stays on the device between steps; a step needs no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/mcst.h"
+#include "kernels.hpp"
+#include "models.hpp"
+
+namespace mcst {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) return fail(MCST_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ------------------------------------------------------------ model registry
+struct ModelEntry {
+  const char* name;
+  int id, nv, ns, np;
+  void (*defaults)(double*);
+  void (*init)(const double*, double*, uint64_t, uint32_t);
+  double (*mass)(const double*, const double*);
+};
+
+template <class M>
+double host_mass(const double* p, const double* x) {
+  return M::mass(p, x);
+}
+
+template <class M>
+ModelEntry entry() {
+  return ModelEntry{M::NAME, M::ID, M::NV, M::NS, M::NP, &M::defaults, &M::init, &host_mass<M>};
+}
+
+const std::vector<ModelEntry>& registry() {
+  static const std::vector<ModelEntry> r = {entry<MonodMass>(), entry<DualMonod>(), entry<Monod>(),
+                                            entry<FixedLength>(), entry<SimpleAcetate>()};
+  return r;
+}
+
+const ModelEntry* find_model(const char* name) {
+  if (!name) return nullptr;
+  for (const auto& m : registry())
+    if (std::strcmp(m.name, name) == 0) return &m;
+  return nullptr;
+}
+
+const ModelEntry* find_model(int id) {
+  for (const auto& m : registry())
+    if (m.id == id) return &m;
+  return nullptr;
+}
+
+// ------------------------------------------------------------ flowmap tables
+struct HostMap {
+  std::vector<double> V, inv_v, out_flow, p_move, q_exit, cdf, val;
+  std::vector<uint32_t> nbr, col_ptr, src;
+  uint32_t m = 1;
+};
+
+struct Edge {
+  uint32_t i, j;
+  uint64_t ord;
+  double q;
+};
+
+// Transition data of one flowmap.  Off-diagonal flows F(i,j) (duplicates
+// summed in input order, non-positive totals dropped) give:
+//   out_flow[i] = sum_j F(i,j)                (diag of the transition matrix)
+//   p_move[i]   = dt*out_flow[i] / V[i]        (probability_leaving.hpp:313-325)
+//   cdf[i][k]   = sum_{k'<=k} F(i,nbr_k') / out_flow[i], neighbours ascending,
+//                 rows padded with cdf 1 and the last neighbour
+//   transport matrix by destination column: F(i,j) and -out_flow[j]
+//                 (documentation/pages/03_cma.md:33-64)
+int build_map(uint32_t n, const double* V, uint64_t nnz, const uint32_t* rows, const uint32_t* cols,
+              const double* flows, double dt, HostMap& f) {
+  std::vector<Edge> e;
+  e.reserve(nnz);
+  for (uint64_t t = 0; t < nnz; ++t) {
+    if (rows[t] >= n || cols[t] >= n) return fail(MCST_ERR_ARG, "flowmap entry out of range");
+    if (!std::isfinite(flows[t])) return fail(MCST_ERR_ARG, "flowmap entry is not finite");
+    if (rows[t] == cols[t]) continue;
+    e.push_back(Edge{rows[t], cols[t], t, flows[t]});
+  }
+  std::sort(e.begin(), e.end(), [](const Edge& a, const Edge& b) {
+    if (a.i != b.i) return a.i < b.i;
+    if (a.j != b.j) return a.j < b.j;
+    return a.ord < b.ord;
+  });
+  std::vector<Edge> u;
+  u.reserve(e.size());
+  for (size_t t = 0; t < e.size();) {
+    Edge acc = e[t];
+    size_t k = t + 1;
+    for (; k < e.size() && e[k].i == acc.i && e[k].j == acc.j; ++k) acc.q += e[k].q;
+    if (acc.q > 0.0) u.push_back(acc);
+    t = k;
+  }
+  std::vector<uint64_t> rs(n + 1, 0);
+  for (const auto& x : u) rs[x.i + 1]++;
+  uint32_t m = 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    m = std::max<uint32_t>(m, (uint32_t)rs[i + 1]);
+    rs[i + 1] += rs[i];
+  }
+  f.m = m;
+  f.V.assign(V, V + n);
+  f.inv_v.resize(n);
+  f.out_flow.assign(n, 0.0);
+  f.p_move.resize(n);
+  f.q_exit.assign(n, 2.0);
+  f.cdf.assign((size_t)n * m, 0.0);
+  f.nbr.assign((size_t)n * m, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!(V[i] > 0.0) || !std::isfinite(V[i])) return fail(MCST_ERR_ARG, "compartment volume must be positive");
+    f.inv_v[i] = 1.0 / V[i];
+    double acc = 0.0;
+    for (uint64_t t = rs[i]; t < rs[i + 1]; ++t) acc += u[t].q;
+    f.out_flow[i] = acc;
+    f.p_move[i] = (dt * acc) / V[i];
+    double run = 0.0;
+    uint32_t c = 0;
+    for (uint64_t t = rs[i]; t < rs[i + 1]; ++t, ++c) {
+      run += u[t].q;
+      f.cdf[(size_t)i * m + c] = run / acc;
+      f.nbr[(size_t)i * m + c] = u[t].j;
+    }
+    const bool has = rs[i + 1] > rs[i];
+    const uint32_t last = has ? u[rs[i + 1] - 1].j : i;
+    for (; c < m; ++c) {
+      f.cdf[(size_t)i * m + c] = has ? 1.0 : 0.0;
+      f.nbr[(size_t)i * m + c] = last;
+    }
+  }
+  std::vector<Edge> tr(u);
+  for (uint32_t i = 0; i < n; ++i) tr.push_back(Edge{i, i, 0, -f.out_flow[i]});
+  std::sort(tr.begin(), tr.end(), [](const Edge& a, const Edge& b) {
+    if (a.j != b.j) return a.j < b.j;
+    return a.i < b.i;
+  });
+  if (tr.size() >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "flowmap too large");
+  f.col_ptr.assign(n + 1, 0);
+  f.src.resize(tr.size());
+  f.val.resize(tr.size());
+  for (size_t t = 0; t < tr.size(); ++t) {
+    f.col_ptr[tr[t].j + 1]++;
+    f.src[t] = tr[t].i;
+    f.val[t] = tr[t].q;
+  }
+  for (uint32_t j = 0; j < n; ++j) f.col_ptr[j + 1] += f.col_ptr[j];
+  return MCST_OK;
+}
+
+// ------------------------------------------------------------ runs
+struct HostFeed {
+  int kind;
+  double f0, param;
+  uint32_t in;
+  int64_t out;
+  std::vector<uint32_t> sp;
+  std::vector<double> conc;
+};
+
+struct HostRun {
+  uint32_t n_c = 0, n_s = 0;
+  double dt = 0.0;
+  uint64_t seed = 0;
+  double params[MAX_PARAMS] = {};
+  std::vector<HostMap> maps;
+  uint32_t steps_per_map = 1;
+  std::vector<HostFeed> feeds;
+  std::vector<double> C0;
+  double ratio = 0.01;
+  uint64_t min_removal = 0;
+  uint32_t stage_cap = 64;
+  uint32_t probe_cap = 0;
+  uint32_t step_target = 0xFFFFFFFFu;
+  // initial particles
+  bool has_particles = false;
+  uint64_t n = 0, cap = 0;
+  std::vector<uint32_t> pos;
+  std::vector<uint8_t> status;
+  std::vector<double> state;  // [nv][n]
+  std::vector<double> ah, ad;
+  double weight = 0.0;
+  // device-side placement
+  uint64_t p_off = 0;
+  uint32_t comp_off = 0, tile_cap_off = 0, map_base = 0;
+};
+
+}  // namespace
+}  // namespace mcst
+
+struct mcst_engine {
+  const mcst::ModelEntry* model = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<mcst::HostRun> runs;
+  bool committed = false;
+  // device
+  std::vector<void*> allocs;
+  mcst::BatchDev bd{};
+  mcst::RunDev* d_runs = nullptr;
+  mcst::RunState* d_rs = nullptr;
+  std::vector<mcst::RunDev> h_runs;
+  int grid_particle = 1, grid_transport = 1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint32_t graph_steps = 0;
+  std::vector<uint32_t> host_step;  // steps launched per run (host mirror)
+};
+
+namespace mcst {
+namespace {
+
+int dev_alloc(mcst_engine* e, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t err = hipMalloc(p, bytes);
+  if (err != hipSuccess) return fail(MCST_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(err));
+  e->allocs.push_back(*p);
+  return MCST_OK;
+}
+
+template <class T>
+int dev_upload(mcst_engine* e, T** p, const T* src, size_t count) {
+  int rc = dev_alloc(e, (void**)p, count * sizeof(T));
+  if (rc) return rc;
+  if (count) HIP_TRY(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return MCST_OK;
+}
+
+void release(mcst_engine* e) {
+  if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->graph_exec = nullptr;
+  e->graph = nullptr;
+  for (void* p : e->allocs) (void)hipFree(p);
+  e->allocs.clear();
+}
+
+HostRun* get_run(mcst_engine* e, int32_t run) {
+  if (!e || run < 0 || (size_t)run >= e->runs.size()) return nullptr;
+  return &e->runs[run];
+}
+
+// Feed sources F[s][c], sink[c] (constant feeds) and the per-map leave
+// thresholds exp(-dt*Q/V) of the first leaving flow of each outlet
+// (update_feed, simulation.model.cpp:72-122; find_flow, move_kernel.hpp:105-127).
+void feed_tables(const HostRun& r, std::vector<double>& F, std::vector<double>& sink) {
+  F.assign((size_t)r.n_s * r.n_c, 0.0);
+  sink.assign(r.n_c, 0.0);
+  for (const auto& fd : r.feeds) {
+    const double q = fd.f0;
+    for (size_t v = 0; v < fd.sp.size(); ++v) F[(size_t)fd.sp[v] * r.n_c + fd.in] += q * fd.conc[v];
+    if (fd.out >= 0) sink[(size_t)fd.out] += q;
+  }
+}
+
+void exit_thresholds(const HostRun& r, HostMap& mp) {
+  mp.q_exit.assign(r.n_c, 2.0);
+  std::vector<char> seen(r.n_c, 0);
+  for (const auto& fd : r.feeds) {
+    if (fd.out < 0) continue;
+    const uint32_t o = (uint32_t)fd.out;
+    if (seen[o]) continue;
+    seen[o] = 1;
+    mp.q_exit[o] = std::exp(-(r.dt * fd.f0) / mp.V[o]);
+  }
+}
+
+int commit(mcst_engine* e) {
+  const ModelEntry& M = *e->model;
+  const uint32_t R = (uint32_t)e->runs.size();
+  if (R == 0) return fail(MCST_ERR_STATE, "no run defined");
+  uint64_t P = 0;
+  uint64_t T_max = 0;
+  uint32_t stride = 1, n_comp = 0, n_maps = 0;
+  for (uint32_t r = 0; r < R; ++r) {
+    HostRun& h = e->runs[r];
+    if (h.maps.empty()) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no flowmap");
+    if (!h.has_particles) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no particles");
+    for (const auto& fd : h.feeds)
+      if (fd.kind != MCST_FEED_CONSTANT)
+        return fail(MCST_ERR_ARG, "time-dependent feeds are not supported on the device path");
+    h.p_off = P;
+    h.tile_cap_off = (uint32_t)T_max;
+    h.comp_off = n_comp;
+    h.map_base = n_maps;
+    P += h.cap;
+    T_max += (h.cap + TILE - 1) / TILE;
+    stride = std::max(stride, h.stage_cap);
+    n_comp += h.n_c;
+    n_maps += (uint32_t)h.maps.size();
+  }
+  if (P >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "more than 2^32-1 particle slots in one engine");
+  if (T_max == 0) T_max = 1;
+  const uint32_t nv = (uint32_t)M.nv;
+  BatchDev& b = e->bd;
+  b = BatchDev{};
+  b.P = P;
+  b.T_max = (uint32_t)T_max;
+  b.stage_stride = stride;
+  b.ST = T_max * stride;
+  b.R = R;
+  b.n_comp_total = n_comp;
+  b.stages = STAGE_ALL;
+  int rc;
+  if ((rc = dev_alloc(e, (void**)&b.pos, P * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.status, P))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.state, P * nv * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.age_hyd, P * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.age_div, P * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.st_state, b.ST * nv * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.st_pos, b.ST * sizeof(uint32_t)))) return rc;
+  uint32_t** tile_arrays[] = {&b.tile_births, &b.tile_birth_off, &b.tile_gap, &b.tile_fill, &b.tile_gap_off,
+                              &b.tile_fill_off};
+  for (uint32_t** a : tile_arrays)
+    if ((rc = dev_alloc(e, (void**)a, T_max * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.tile_off, (R + 1) * sizeof(uint32_t)))) return rc;
+  HIP_TRY(hipMemsetAsync(b.tile_births, 0, T_max * sizeof(uint32_t), e->stream));
+  HIP_TRY(hipMemsetAsync(b.tile_off, 0, (R + 1) * sizeof(uint32_t), e->stream));
+
+  // particles
+  for (uint32_t r = 0; r < R; ++r) {
+    const HostRun& h = e->runs[r];
+    if (h.n == 0) continue;
+    HIP_TRY(hipMemcpy(b.pos + h.p_off, h.pos.data(), h.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.status + h.p_off, h.status.data(), h.n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.age_hyd + h.p_off, h.ah.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.age_div + h.p_off, h.ad.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
+    for (uint32_t v = 0; v < nv; ++v)
+      HIP_TRY(hipMemcpy(b.state + (uint64_t)v * P + h.p_off, h.state.data() + (size_t)v * h.n,
+                        h.n * sizeof(double), hipMemcpyHostToDevice));
+  }
+
+  // maps, liquid state, run descriptors
+  std::vector<MapDev> maps;
+  std::vector<uint32_t> comp_off;
+  e->h_runs.assign(R, RunDev{});
+  std::vector<RunState> rs(R);
+  for (uint32_t r = 0; r < R; ++r) {
+    HostRun& h = e->runs[r];
+    RunDev& d = e->h_runs[r];
+    comp_off.push_back(h.comp_off);
+    for (auto& mp : h.maps) {
+      exit_thresholds(h, mp);
+      MapDev md{};
+      md.m = mp.m;
+      if ((rc = dev_upload(e, (double**)&md.p_move, mp.p_move.data(), mp.p_move.size()))) return rc;
+      if ((rc = dev_upload(e, (double**)&md.q_exit, mp.q_exit.data(), mp.q_exit.size()))) return rc;
+      if ((rc = dev_upload(e, (double**)&md.cdf, mp.cdf.data(), mp.cdf.size()))) return rc;
+      if ((rc = dev_upload(e, (uint32_t**)&md.nbr, mp.nbr.data(), mp.nbr.size()))) return rc;
+      if ((rc = dev_upload(e, (uint32_t**)&md.col_ptr, mp.col_ptr.data(), mp.col_ptr.size()))) return rc;
+      if ((rc = dev_upload(e, (uint32_t**)&md.src, mp.src.data(), mp.src.size()))) return rc;
+      if ((rc = dev_upload(e, (double**)&md.val, mp.val.data(), mp.val.size()))) return rc;
+      if ((rc = dev_upload(e, (double**)&md.inv_v, mp.inv_v.data(), mp.inv_v.size()))) return rc;
+      maps.push_back(md);
+    }
+    const size_t nsc = (size_t)h.n_s * h.n_c;
+    std::vector<double> C0 = h.C0.empty() ? std::vector<double>(nsc, 0.0) : h.C0;
+    std::vector<double> m0(nsc);
+    for (uint32_t s = 0; s < h.n_s; ++s)
+      for (uint32_t j = 0; j < h.n_c; ++j) m0[(size_t)s * h.n_c + j] = C0[(size_t)s * h.n_c + j] * h.maps[0].V[j];
+    std::vector<double> F, sink;
+    feed_tables(h, F, sink);
+    const uint32_t rw = (uint32_t)rec_stride((int)h.n_s);
+    std::vector<double> rec((size_t)h.n_c * rw, 0.0);
+    if ((rc = dev_upload(e, &d.C[0], C0.data(), nsc))) return rc;
+    if ((rc = dev_upload(e, &d.C[1], C0.data(), nsc))) return rc;
+    if ((rc = dev_upload(e, &d.m, m0.data(), nsc))) return rc;
+    std::vector<double> zeros(nsc, 0.0);
+    if ((rc = dev_upload(e, &d.S, zeros.data(), nsc))) return rc;
+    if ((rc = dev_upload(e, (double**)&d.F, F.data(), nsc))) return rc;
+    if ((rc = dev_upload(e, (double**)&d.sink, sink.data(), sink.size()))) return rc;
+    if ((rc = dev_upload(e, &d.rec, rec.data(), rec.size()))) return rc;
+    if (h.probe_cap) {
+      if ((rc = dev_alloc(e, (void**)&d.probe_exit, (size_t)h.probe_cap * sizeof(double)))) return rc;
+      if ((rc = dev_alloc(e, (void**)&d.probe_div, (size_t)h.probe_cap * sizeof(double)))) return rc;
+    }
+    d.p_off = h.p_off;
+    d.seed = h.seed;
+    d.dt = h.dt;
+    d.weight = h.weight;
+    d.ratio = h.ratio;
+    d.min_removal = h.min_removal;
+    std::memcpy(d.params, h.params, sizeof(d.params));
+    d.cap = (uint32_t)h.cap;
+    d.n_c = h.n_c;
+    d.n_s = h.n_s;
+    d.rec_w = rw;
+    d.enable_move = h.n_c > 1 ? 1u : 0u;
+    d.enable_leave = 0;
+    for (const auto& fd : h.feeds)
+      if (fd.out >= 0) d.enable_leave = 1;
+    d.n_maps = (uint32_t)h.maps.size();
+    d.steps_per_map = h.steps_per_map;
+    d.map_base = h.map_base;
+    d.stage_cap = h.stage_cap;
+    d.comp_off = h.comp_off;
+    d.tile_cap_off = h.tile_cap_off;
+    d.probe_cap = h.probe_cap;
+    d.step_target = h.step_target;
+    RunState& s = rs[r];
+    std::memset(&s, 0, sizeof(s));
+    s.n = (uint32_t)h.n;
+    uint64_t inactive = 0;
+    for (uint64_t p = 0; p < h.n; ++p) inactive += h.status[p] != IDLE;
+    s.inactive = (uint32_t)inactive;
+    s.counters[C_N] = h.n;
+    s.counters[C_INACTIVE] = inactive;
+  }
+  comp_off.push_back(n_comp);
+  MapDev* d_maps = nullptr;
+  uint32_t* d_comp = nullptr;
+  if ((rc = dev_upload(e, &d_maps, maps.data(), maps.size()))) return rc;
+  if ((rc = dev_upload(e, &d_comp, comp_off.data(), comp_off.size()))) return rc;
+  if ((rc = dev_upload(e, &e->d_runs, e->h_runs.data(), R))) return rc;
+  if ((rc = dev_upload(e, &e->d_rs, rs.data(), R))) return rc;
+  b.maps = d_maps;
+  b.comp_off = d_comp;
+  b.runs = e->d_runs;
+  b.rs = e->d_rs;
+
+  // launch geometry: a persistent grid over tiles, sized by occupancy
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, e->device));
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id), BLOCK, 0));
+  per_cu = std::max(per_cu, 1);
+  e->grid_particle = (int)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * per_cu, T_max);
+  e->grid_transport = (int)std::min<uint64_t>((n_comp + 255) / 256, 8192);
+  e->grid_transport = std::max(e->grid_transport, 1);
+  e->grid_particle = std::max(e->grid_particle, 1);
+  e->host_step.assign(R, 0);
+  launch_layout(b, e->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->committed = true;
+  // host copies of the initial particles are no longer needed
+  for (auto& h : e->runs) {
+    std::vector<uint32_t>().swap(h.pos);
+    std::vector<uint8_t>().swap(h.status);
+    std::vector<double>().swap(h.state);
+    std::vector<double>().swap(h.ah);
+    std::vector<double>().swap(h.ad);
+  }
+  return MCST_OK;
+}
+
+void launch_step(mcst_engine* e, uint32_t stages) {
+  BatchDev b = e->bd;
+  b.stages = stages;
+  const uint32_t nv = (uint32_t)e->model->nv;
+  launch_transport(b, e->grid_transport, e->stream);
+  launch_particle(e->model->id, b, e->grid_particle, e->stream);
+  launch_bookkeep(b, e->stream);
+  launch_removal(b, nv, e->grid_particle, e->stream);
+  launch_merge(b, nv, e->grid_particle, e->stream);
+  launch_layout(b, e->stream);
+}
+
+int read_state(mcst_engine* e, int32_t run, RunState& s) {
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipMemcpy(&s, e->d_rs + run, sizeof(RunState), hipMemcpyDeviceToHost));
+  return MCST_OK;
+}
+
+}  // namespace
+}  // namespace mcst
+
+using namespace mcst;
+
+#define CHECK_ENGINE(e)                                                     \
+  do {                                                                      \
+    if (!(e)) return fail(MCST_ERR_ARG, "null engine");                     \
+  } while (0)
+#define CHECK_DEFINE(e, r, h)                                               \
+  CHECK_ENGINE(e);                                                          \
+  if ((e)->committed) return fail(MCST_ERR_STATE, "engine already committed"); \
+  HostRun* h = get_run(e, r);                                               \
+  if (!h) return fail(MCST_ERR_ARG, "bad run id")
+#define CHECK_READY(e, r)                                                   \
+  CHECK_ENGINE(e);                                                          \
+  if (!(e)->committed) return fail(MCST_ERR_STATE, "engine not committed"); \
+  if (!get_run(e, r)) return fail(MCST_ERR_ARG, "bad run id")
+
+extern "C" {
+
+int mcst_version(int* major, int* minor, int* patch) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  if (patch) *patch = 0;
+  return MCST_OK;
+}
+
+const char* mcst_last_error(void) { return g_err.c_str(); }
+
+int mcst_model_lookup(const char* name, int* model_id, int* n_var, int* n_species, int* n_params) {
+  const ModelEntry* m = find_model(name);
+  if (!m) return fail(MCST_ERR_ARG, std::string("unknown model: ") + (name ? name : "(null)"));
+  if (model_id) *model_id = m->id;
+  if (n_var) *n_var = m->nv;
+  if (n_species) *n_species = m->ns;
+  if (n_params) *n_params = m->np;
+  return MCST_OK;
+}
+
+int mcst_model_default_params(int model_id, double* params, int n) {
+  const ModelEntry* m = find_model(model_id);
+  if (!m || !params || n < m->np) return fail(MCST_ERR_ARG, "bad model id or parameter buffer");
+  double p[MAX_PARAMS] = {};
+  m->defaults(p);
+  for (int i = 0; i < m->np; ++i) params[i] = p[i];
+  return MCST_OK;
+}
+
+int mcst_engine_create(const char* model_name, int device, mcst_engine** out) {
+  if (!out) return fail(MCST_ERR_ARG, "null output handle");
+  *out = nullptr;
+  const ModelEntry* m = find_model(model_name);
+  if (!m) return fail(MCST_ERR_ARG, std::string("unknown model: ") + (model_name ? model_name : "(null)"));
+  int count = 0;
+  HIP_TRY(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(MCST_ERR_DEVICE, "no such HIP device");
+  HIP_TRY(hipSetDevice(device));
+  mcst_engine* e = new (std::nothrow) mcst_engine();
+  if (!e) return fail(MCST_ERR_MEMORY, "out of host memory");
+  e->model = m;
+  e->device = device;
+  hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (err != hipSuccess) {
+    delete e;
+    return fail(MCST_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(err));
+  }
+  *out = e;
+  return MCST_OK;
+}
+
+int mcst_engine_destroy(mcst_engine** pe) {
+  if (!pe || !*pe) return MCST_OK;
+  mcst_engine* e = *pe;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  release(e);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  *pe = nullptr;
+  return MCST_OK;
+}
+
+int mcst_run_create(mcst_engine* e, uint32_t n_compartments, uint32_t n_species, double dt, uint64_t seed,
+                    int32_t* run_id) {
+  CHECK_ENGINE(e);
+  if (e->committed) return fail(MCST_ERR_STATE, "engine already committed");
+  if (n_compartments == 0) return fail(MCST_ERR_ARG, "a run needs at least one compartment");
+  if ((int)n_species != e->model->ns)
+    return fail(MCST_ERR_ARG, "model " + std::string(e->model->name) + " needs " + std::to_string(e->model->ns) +
+                                  " species");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(MCST_ERR_ARG, "dt must be positive");
+  HostRun h;
+  h.n_c = n_compartments;
+  h.n_s = n_species;
+  h.dt = dt;
+  h.seed = seed;
+  e->model->defaults(h.params);
+  e->runs.push_back(std::move(h));
+  if (run_id) *run_id = (int32_t)e->runs.size() - 1;
+  return MCST_OK;
+}
+
+int mcst_run_set_model_params(mcst_engine* e, int32_t run, const double* params, int n) {
+  CHECK_DEFINE(e, run, h);
+  if (n < 0 || n > MAX_PARAMS || (n && !params)) return fail(MCST_ERR_ARG, "bad parameter vector");
+  for (int i = 0; i < n; ++i) h->params[i] = params[i];
+  return MCST_OK;
+}
+
+int mcst_run_add_flowmap(mcst_engine* e, int32_t run, const double* volumes, uint64_t nnz, const uint32_t* rows,
+                         const uint32_t* cols, const double* flows) {
+  CHECK_DEFINE(e, run, h);
+  if (!volumes || (nnz && (!rows || !cols || !flows))) return fail(MCST_ERR_ARG, "null flowmap array");
+  HostMap mp;
+  const int rc = build_map(h->n_c, volumes, nnz, rows, cols, flows, h->dt, mp);
+  if (rc) return rc;
+  h->maps.push_back(std::move(mp));
+  return MCST_OK;
+}
+
+int mcst_run_set_steps_per_flowmap(mcst_engine* e, int32_t run, uint32_t steps) {
+  CHECK_DEFINE(e, run, h);
+  if (steps == 0) return fail(MCST_ERR_ARG, "steps per flowmap must be positive");
+  h->steps_per_map = steps;
+  return MCST_OK;
+}
+
+int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
+                      int64_t output, uint32_t n_values, const uint32_t* species, const double* conc) {
+  CHECK_DEFINE(e, run, h);
+  if (kind < MCST_FEED_CONSTANT || kind > MCST_FEED_EXPONENTIAL) return fail(MCST_ERR_ARG, "bad feed kind");
+  if (input >= h->n_c || output >= (int64_t)h->n_c) return fail(MCST_ERR_ARG, "feed position out of range");
+  if (f0 < 0.0) return fail(MCST_ERR_ARG, "negative feed flow");  // feed_descriptor.cpp:53-57
+  if (n_values > MAX_SPECIES || (n_values && (!species || !conc))) return fail(MCST_ERR_ARG, "bad feed values");
+  HostFeed fd{kind, f0, param, input, output < 0 ? -1 : output, {}, {}};
+  for (uint32_t v = 0; v < n_values; ++v) {
+    if (species[v] >= h->n_s) return fail(MCST_ERR_ARG, "feed species out of range");
+    if (!(conc[v] > 0.0)) return fail(MCST_ERR_ARG, "feed concentration must be positive");
+    fd.sp.push_back(species[v]);
+    fd.conc.push_back(conc[v]);
+  }
+  h->feeds.push_back(std::move(fd));
+  return MCST_OK;
+}
+
+int mcst_run_set_concentrations(mcst_engine* e, int32_t run, const double* C) {
+  CHECK_DEFINE(e, run, h);
+  if (!C) return fail(MCST_ERR_ARG, "null concentrations");
+  const size_t nsc = (size_t)h->n_s * h->n_c;
+  for (size_t i = 0; i < nsc; ++i)
+    if (!(C[i] >= 0.0)) return fail(MCST_ERR_ARG, "negative initial concentration");
+  h->C0.assign(C, C + nsc);
+  return MCST_OK;
+}
+
+int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min_removal) {
+  CHECK_DEFINE(e, run, h);
+  if (!(ratio >= 0.0)) return fail(MCST_ERR_ARG, "removal ratio must be >= 0");
+  h->ratio = ratio;
+  h->min_removal = min_removal;
+  return MCST_OK;
+}
+
+int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap) {
+  CHECK_DEFINE(e, run, h);
+  if (cap == 0 || cap > TILE) return fail(MCST_ERR_ARG, "division cap must be in 1..1024");
+  h->stage_cap = cap;
+  return MCST_OK;
+}
+
+int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap) {
+  CHECK_DEFINE(e, run, h);
+  h->probe_cap = cap;
+  return MCST_OK;
+}
+
+int mcst_run_set_step_target(mcst_engine* e, int32_t run, uint32_t steps) {
+  CHECK_DEFINE(e, run, h);
+  h->step_target = steps;
+  return MCST_OK;
+}
+
+int mcst_run_init_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, double x0) {
+  CHECK_DEFINE(e, run, h);
+  if (h->maps.empty()) return fail(MCST_ERR_STATE, "add a flowmap before the particles");
+  if (capacity < n || capacity >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "capacity must be in [n, 2^32)");
+  const ModelEntry& M = *e->model;
+  const uint32_t nv = (uint32_t)M.nv;
+  h->n = n;
+  h->cap = capacity;
+  h->pos.assign(n, 0);
+  h->status.assign(n, IDLE);
+  h->state.assign((size_t)nv * n, 0.0);
+  h->ah.assign(n, 0.0);
+  h->ad.assign(n, 0.0);
+  std::vector<double> mass(n);
+  const uint32_t n_c = h->n_c;
+  auto work = [&](uint64_t a, uint64_t z) {
+    double x[MAX_VAR];
+    for (uint64_t p = a; p < z; ++p) {
+      const U4 w = draw4(h->seed, (uint32_t)p, 0u, TAG_INIT, 1u);
+      h->pos[p] = (uint32_t)(((uint64_t)w.x * n_c) >> 32);
+      M.init(h->params, x, h->seed, (uint32_t)p);
+      for (uint32_t v = 0; v < nv; ++v) h->state[(size_t)v * n + p] = x[v];
+      mass[p] = M.mass(h->params, x);
+    }
+  };
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 100000 || hw == 1) {
+    work(0, n);
+  } else {
+    std::vector<std::thread> th;
+    const uint64_t chunk = (n + hw - 1) / hw;
+    for (unsigned t = 0; t < hw; ++t) {
+      const uint64_t a = t * chunk, z = std::min<uint64_t>(n, a + chunk);
+      if (a < z) th.emplace_back(work, a, z);
+    }
+    for (auto& t : th) t.join();
+  }
+  double total = 0.0;  // summed in slot order (the reduction of mc_init_first)
+  for (uint64_t p = 0; p < n; ++p) total += mass[p];
+  double vt = 0.0;
+  for (uint32_t j = 0; j < n_c; ++j) vt += h->maps[0].V[j];
+  h->weight = n ? (x0 * vt) / total : 0.0;
+  h->has_particles = true;
+  return MCST_OK;
+}
+
+int mcst_run_set_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, const uint32_t* pos,
+                           const uint8_t* status, const double* state, const double* age_hyd,
+                           const double* age_div, double weight) {
+  CHECK_DEFINE(e, run, h);
+  if (capacity < n || capacity >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "capacity must be in [n, 2^32)");
+  if (n && (!pos || !status || !state || !age_hyd || !age_div)) return fail(MCST_ERR_ARG, "null particle array");
+  const uint32_t nv = (uint32_t)e->model->nv;
+  for (uint64_t p = 0; p < n; ++p) {
+    if (pos[p] >= h->n_c) return fail(MCST_ERR_ARG, "particle position out of range");
+    if (status[p] > DEAD) return fail(MCST_ERR_ARG, "bad particle status");
+  }
+  h->n = n;
+  h->cap = capacity;
+  h->pos.assign(pos, pos + n);
+  h->status.assign(status, status + n);
+  h->state.assign(state, state + (size_t)nv * n);
+  h->ah.assign(age_hyd, age_hyd + n);
+  h->ad.assign(age_div, age_div + n);
+  h->weight = weight;
+  h->has_particles = true;
+  return MCST_OK;
+}
+
+int mcst_engine_commit(mcst_engine* e) {
+  CHECK_ENGINE(e);
+  if (e->committed) return fail(MCST_ERR_STATE, "engine already committed");
+  HIP_TRY(hipSetDevice(e->device));
+  const int rc = commit(e);
+  if (rc) release(e);
+  return rc;
+}
+
+int mcst_engine_step(mcst_engine* e, uint32_t n_steps) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  HIP_TRY(hipSetDevice(e->device));
+  for (uint32_t k = 0; k < n_steps; ++k) launch_step(e, STAGE_ALL);
+  HIP_TRY(hipGetLastError());
+  return MCST_OK;
+}
+
+int mcst_engine_step_stages(mcst_engine* e, uint32_t stages) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  if (stages & ~STAGE_ALL) return fail(MCST_ERR_ARG, "unknown stage bits");
+  HIP_TRY(hipSetDevice(e->device));
+  launch_step(e, stages);
+  HIP_TRY(hipGetLastError());
+  return MCST_OK;
+}
+
+int mcst_engine_capture(mcst_engine* e, uint32_t n_steps) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  if (n_steps == 0) return fail(MCST_ERR_ARG, "capture at least one step");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->graph_exec) HIP_TRY(hipGraphExecDestroy(e->graph_exec));
+  if (e->graph) HIP_TRY(hipGraphDestroy(e->graph));
+  e->graph_exec = nullptr;
+  e->graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  for (uint32_t k = 0; k < n_steps; ++k) launch_step(e, STAGE_ALL);
+  HIP_TRY(hipStreamEndCapture(e->stream, &e->graph));
+  HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+  e->graph_steps = n_steps;
+  return MCST_OK;
+}
+
+int mcst_engine_replay(mcst_engine* e, uint32_t times) {
+  CHECK_ENGINE(e);
+  if (!e->graph_exec) return fail(MCST_ERR_STATE, "no captured graph");
+  HIP_TRY(hipSetDevice(e->device));
+  for (uint32_t k = 0; k < times; ++k) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+  return MCST_OK;
+}
+
+int mcst_engine_synchronize(mcst_engine* e) {
+  CHECK_ENGINE(e);
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return MCST_OK;
+}
+
+int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, double* per_kernel_ms) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  HIP_TRY(hipSetDevice(e->device));
+  constexpr int K = 6;
+  std::vector<hipEvent_t> ev((size_t)n_steps * (K + 1));
+  for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
+  hipEvent_t t0, t1;
+  HIP_TRY(hipEventCreate(&t0));
+  HIP_TRY(hipEventCreate(&t1));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipEventRecord(t0, e->stream));
+  const uint32_t nv = (uint32_t)e->model->nv;
+  for (uint32_t s = 0; s < n_steps; ++s) {
+    hipEvent_t* x = &ev[(size_t)s * (K + 1)];
+    BatchDev b = e->bd;
+    HIP_TRY(hipEventRecord(x[0], e->stream));
+    launch_transport(b, e->grid_transport, e->stream);
+    HIP_TRY(hipEventRecord(x[1], e->stream));
+    launch_particle(e->model->id, b, e->grid_particle, e->stream);
+    HIP_TRY(hipEventRecord(x[2], e->stream));
+    launch_bookkeep(b, e->stream);
+    HIP_TRY(hipEventRecord(x[3], e->stream));
+    launch_removal(b, nv, e->grid_particle, e->stream);
+    HIP_TRY(hipEventRecord(x[4], e->stream));
+    launch_merge(b, nv, e->grid_particle, e->stream);
+    HIP_TRY(hipEventRecord(x[5], e->stream));
+    launch_layout(b, e->stream);
+    HIP_TRY(hipEventRecord(x[6], e->stream));
+  }
+  HIP_TRY(hipEventRecord(t1, e->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(t1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+  if (total_ms) *total_ms = ms;
+  if (per_kernel_ms) {
+    for (int k = 0; k < K; ++k) {
+      double acc = 0.0;
+      for (uint32_t s = 0; s < n_steps; ++s) {
+        float d = 0.f;
+        HIP_TRY(hipEventElapsedTime(&d, ev[(size_t)s * (K + 1) + k], ev[(size_t)s * (K + 1) + k + 1]));
+        acc += d;
+      }
+      per_kernel_ms[k] = n_steps ? acc / n_steps : 0.0;
+    }
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return MCST_OK;
+}
+
+int mcst_run_get_counters(mcst_engine* e, int32_t run, uint64_t* counters) {
+  CHECK_READY(e, run);
+  if (!counters) return fail(MCST_ERR_ARG, "null counter buffer");
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  for (int i = 0; i < C_COUNT; ++i) counters[i] = s.counters[i];
+  counters[C_STEPS] = s.step;
+  counters[C_N] = s.n;
+  counters[C_INACTIVE] = s.inactive;
+  return MCST_OK;
+}
+
+int mcst_run_get_n(mcst_engine* e, int32_t run, uint64_t* n) {
+  CHECK_READY(e, run);
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  if (n) *n = s.n;
+  return MCST_OK;
+}
+
+int mcst_run_get_weight(mcst_engine* e, int32_t run, double* w) {
+  CHECK_ENGINE(e);
+  HostRun* h = get_run(e, run);
+  if (!h) return fail(MCST_ERR_ARG, "bad run id");
+  if (w) *w = h->weight;
+  return MCST_OK;
+}
+
+int mcst_run_get_particles(mcst_engine* e, int32_t run, uint32_t* pos, uint8_t* status, double* state,
+                           double* age_hyd, double* age_div) {
+  CHECK_READY(e, run);
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  const HostRun& h = e->runs[run];
+  const BatchDev& b = e->bd;
+  const uint64_t n = s.n;
+  if (n == 0) return MCST_OK;
+  if (pos) HIP_TRY(hipMemcpy(pos, b.pos + h.p_off, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (status) HIP_TRY(hipMemcpy(status, b.status + h.p_off, n, hipMemcpyDeviceToHost));
+  if (age_hyd) HIP_TRY(hipMemcpy(age_hyd, b.age_hyd + h.p_off, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (age_div) HIP_TRY(hipMemcpy(age_div, b.age_div + h.p_off, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (state)
+    for (int v = 0; v < e->model->nv; ++v)
+      HIP_TRY(hipMemcpy(state + (size_t)v * n, b.state + (uint64_t)v * b.P + h.p_off, n * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  return MCST_OK;
+}
+
+static int get_liquid(mcst_engine* e, int32_t run, double* out, int which) {
+  CHECK_READY(e, run);
+  if (!out) return fail(MCST_ERR_ARG, "null output");
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  const RunDev& d = e->h_runs[run];
+  const size_t nsc = (size_t)d.n_s * d.n_c;
+  const double* src = which == 0 ? d.C[s.step & 1u] : (which == 1 ? d.m : d.S);
+  HIP_TRY(hipMemcpy(out, src, nsc * sizeof(double), hipMemcpyDeviceToHost));
+  return MCST_OK;
+}
+
+int mcst_run_get_concentrations(mcst_engine* e, int32_t run, double* C) { return get_liquid(e, run, C, 0); }
+int mcst_run_get_mass(mcst_engine* e, int32_t run, double* m) { return get_liquid(e, run, m, 1); }
+int mcst_run_get_contributions(mcst_engine* e, int32_t run, double* S) { return get_liquid(e, run, S, 2); }
+
+int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist) {
+  CHECK_READY(e, run);
+  if (!hist) return fail(MCST_ERR_ARG, "null output");
+  const uint32_t n_c = e->h_runs[run].n_c;
+  unsigned long long* d = nullptr;
+  HIP_TRY(hipMalloc(&d, n_c * sizeof(unsigned long long)));
+  hipError_t err = hipMemsetAsync(d, 0, n_c * sizeof(unsigned long long), e->stream);
+  if (err == hipSuccess) {
+    launch_repartition(e->bd, (uint32_t)run, d, e->stream);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err == hipSuccess) err = hipMemcpy(hist, d, n_c * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) return fail(MCST_ERR_DEVICE, std::string("repartition: ") + hipGetErrorString(err));
+  return MCST_OK;
+}
+
+static int get_probe(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n, bool exit_ages) {
+  CHECK_READY(e, run);
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  const RunDev& d = e->h_runs[run];
+  const uint64_t have = std::min<uint64_t>(exit_ages ? s.probe_exit_n : s.probe_div_n, d.probe_cap);
+  if (n) *n = have;
+  const uint64_t k = std::min(have, max);
+  if (out && k) HIP_TRY(hipMemcpy(out, exit_ages ? d.probe_exit : d.probe_div, k * sizeof(double), hipMemcpyDeviceToHost));
+  return MCST_OK;
+}
+
+int mcst_run_get_exit_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n) {
+  return get_probe(e, run, out, max, n, true);
+}
+int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n) {
+  return get_probe(e, run, out, max, n, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// mcst_common.hpp -- layout of the device-resident batch of runs.
+//
+// A batch ("engine") holds R independent runs that share one particle model.
+// Every run owns a region [p_off, p_off + cap) of the batch-wide particle SoA
+// arrays, its own compartment network (flowmaps), liquid state and counters.
+// Particles are processed in tiles of TILE consecutive slots of one run; the
+// tile -> run map is a prefix table rebuilt on the device after every step,
+// so a step never needs the host to know particle counts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcst {
+
+constexpr uint32_t BLOCK = 256;            // threads per particle block (4 wave64)
+constexpr uint32_t PPT = 4;                // particles per thread per tile
+constexpr uint32_t TILE = BLOCK * PPT;     // 1024 slots per tile (the reference's 1024 particles/team)
+constexpr uint32_t WAVES = BLOCK / 64;
+constexpr int MAX_PARAMS = 16;
+constexpr int MAX_VAR = 16;
+constexpr int MAX_SPECIES = 8;
+
+// Particle status byte (apps/libs/mc/public/mc/alias.hpp:124-130).
+enum Status : uint8_t { IDLE = 0, DIVISION = 1, EXIT = 2, DEAD = 3 };
+
+// Stage mask of one step (values shared with the C ABI).
+enum Stage : uint32_t {
+  STAGE_TRANSPORT = 1u,
+  STAGE_MODEL = 2u,
+  STAGE_CONTRIB = 4u,
+  STAGE_MOVE = 8u,
+  STAGE_LEAVE = 16u,
+  STAGE_REMOVE = 32u,
+  STAGE_ALL = 63u
+};
+
+// Philox counter purpose tags (third counter word = tag << 16 | block).
+enum RngTag : uint32_t { TAG_TRANSPORT = 0, TAG_UPDATE = 1, TAG_DIVISION = 2, TAG_INIT = 3 };
+
+// Cumulative per-run counters (order shared with the C ABI).
+enum Counter : int {
+  C_STEPS = 0,
+  C_BIRTHS,
+  C_EXITS,
+  C_DEATHS,
+  C_MOVES,
+  C_OVERFLOW,
+  C_COMPACTIONS,
+  C_N,
+  C_INACTIVE,
+  C_PROCESSED,  // particle-steps: Idle particles that entered a pass
+  C_COUNT
+};
+
+// Tables of one flowmap of one run (device pointers).
+struct MapDev {
+  const double* p_move;     // [n_c]  dt*Q_out/V
+  const double* q_exit;     // [n_c]  exp(-dt*Q_leave/V) at outlets, 2.0 elsewhere
+  const double* cdf;        // [n_c][m] cumulative neighbour probabilities
+  const uint32_t* nbr;      // [n_c][m] neighbour ids (padding repeats the last)
+  const uint32_t* col_ptr;  // [n_c+1] transport matrix by destination column
+  const uint32_t* src;      // [nnz]   source compartment of each column entry
+  const double* val;        // [nnz]   F(i,j) off-diagonal, -Q_out(j) on the diagonal
+  const double* inv_v;      // [n_c]
+  uint32_t m;
+  uint32_t pad;
+};
+
+// Immutable description of one run.
+struct RunDev {
+  uint64_t p_off;           // first slot of the run in the batch arrays
+  uint64_t seed;
+  double dt;
+  double weight;            // numerical weight of every particle (unit.cpp:232-257)
+  double ratio;             // removal threshold ratio (particles_container.hpp:548-551)
+  uint64_t min_removal;
+  double params[MAX_PARAMS];
+  double* C[2];             // [n_s][n_c] concentrations, ping-pong by step parity
+  double* m;                // [n_s][n_c] liquid mass
+  double* S;                // [n_s][n_c] particle sources of the current step
+  const double* F;          // [n_s][n_c] constant feed sources
+  const double* sink;       // [n_c]      outlet flow per compartment
+  double* rec;              // [n_c][rec] per-compartment record read by the particle pass
+  double* probe_exit;       // exit ages (residence times), may be null
+  double* probe_div;        // division ages, may be null
+  uint32_t cap;             // slots reserved for the run
+  uint32_t n_c;
+  uint32_t n_s;
+  uint32_t rec_w;           // record stride in doubles: {C_0..C_{n_s-1}, p_move, q_exit, pad}
+  uint32_t enable_move;     // n_c > 1                        (kernels.hpp:54-55)
+  uint32_t enable_leave;    // at least one leaving flow      (kernels.hpp:56)
+  uint32_t n_maps;
+  uint32_t steps_per_map;
+  uint32_t map_base;        // index of the run's first MapDev
+  uint32_t stage_cap;       // staging slots per tile (division budget cap)
+  uint32_t comp_off;        // first compartment of the run in the batch transport space
+  uint32_t tile_cap_off;    // first staging tile id reserved for the run
+  uint32_t probe_cap;
+  uint32_t step_target;     // the run stops stepping at this step count
+};
+
+// Mutable per-run state, updated by the device every step.
+struct RunState {
+  uint32_t n;               // used slots (Idle + inactive)
+  uint32_t n_prev;          // slots during the last particle pass
+  uint32_t n_mid;           // slots after removal, before the merge
+  uint32_t inactive;        // Exit/Dead slots below n
+  uint32_t step;
+  uint32_t budget;          // divisions allowed per tile in the next pass
+  uint32_t compact;         // removal requested by the last bookkeeping
+  uint32_t births;          // newborns staged in the last pass
+  uint32_t tile_begin;      // first global tile of the run in the current layout
+  uint32_t n_tiles;         // tiles of the run in the current layout
+  uint32_t gap_total;
+  uint32_t active;          // 1 while step < step_target
+  unsigned long long step_exits, step_deaths, step_moves, step_overflow, step_processed;
+  unsigned long long counters[C_COUNT];
+  unsigned long long probe_exit_n, probe_div_n;
+};
+
+// Everything a step's kernels receive.
+struct BatchDev {
+  // particles, batch-wide SoA
+  uint32_t* pos;
+  uint8_t* status;
+  double* state;            // [n_var][P]
+  double* age_hyd;
+  double* age_div;
+  uint64_t P;               // total slots
+  // staging for newborns: tile t owns [t*stage_stride, (t+1)*stage_stride)
+  double* st_state;         // [n_var][T_max*stage_stride]
+  uint32_t* st_pos;
+  uint64_t ST;              // T_max * stage_stride
+  uint32_t stage_stride;
+  // per-tile bookkeeping of the current layout
+  uint32_t* tile_births;
+  uint32_t* tile_birth_off;
+  uint32_t* tile_gap;
+  uint32_t* tile_fill;
+  uint32_t* tile_gap_off;
+  uint32_t* tile_fill_off;
+  uint32_t* fill_src;       // [P] scratch list of filler slots during removal
+  uint32_t* tile_off;       // [R+1] first global tile of each run in the current layout
+  // runs
+  const RunDev* runs;
+  RunState* rs;
+  const MapDev* maps;
+  const uint32_t* comp_off; // [R+1]
+  uint32_t R;
+  uint32_t n_comp_total;
+  uint32_t T_max;
+  uint32_t stages;
+};
+
+}  // namespace mcst

@@ -1,0 +1,295 @@
+"""Python binding of libmcst.so, the MI355X particle Monte Carlo step.
+
+Thin ctypes layer over the C ABI in include/mcst.h.  There is no CPU
+fallback: if the HIP library is missing or no device is present, the calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libmcst.so")
+
+FEED_KINDS = {"constant": 0, "linear": 1, "exponential": 2}
+COUNTERS = ["steps", "births", "exits", "deaths", "moves", "overflow", "compactions", "n", "inactive", "processed"]
+STAGES = {"transport": 1, "model": 2, "contrib": 4, "move": 8, "leave": 16, "remove": 32}
+STAGE_ALL = 63
+IDLE, DIVISION, EXIT, DEAD = 0, 1, 2, 3
+
+# every symbol include/mcst.h declares
+EXPORTS = [
+    "mcst_version", "mcst_last_error", "mcst_model_lookup", "mcst_model_default_params",
+    "mcst_engine_create", "mcst_engine_destroy", "mcst_run_create", "mcst_run_set_model_params",
+    "mcst_run_add_flowmap", "mcst_run_set_steps_per_flowmap", "mcst_run_add_feed",
+    "mcst_run_set_concentrations", "mcst_run_set_removal", "mcst_run_set_division_cap",
+    "mcst_run_set_probes", "mcst_run_set_step_target", "mcst_run_init_particles", "mcst_run_set_particles",
+    "mcst_engine_commit", "mcst_engine_step", "mcst_engine_step_stages", "mcst_engine_capture",
+    "mcst_engine_replay", "mcst_engine_synchronize", "mcst_engine_profile", "mcst_run_get_counters",
+    "mcst_run_get_n", "mcst_run_get_weight", "mcst_run_get_particles", "mcst_run_get_concentrations",
+    "mcst_run_get_mass", "mcst_run_get_contributions", "mcst_run_get_repartition",
+    "mcst_run_get_exit_ages", "mcst_run_get_division_ages",
+]
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+_lib = None
+
+
+class MCSTError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmcst.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MCSTError(f"{LIB_PATH} is missing: run `python biocma-mcst_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        i32, u32, u64, i64, dbl = C.c_int32, C.c_uint32, C.c_uint64, C.c_int64, C.c_double
+        sig = {
+            "mcst_version": [C.POINTER(C.c_int)] * 3,
+            "mcst_model_lookup": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)],
+            "mcst_model_default_params": [C.c_int, _f64p, C.c_int],
+            "mcst_engine_create": [C.c_char_p, C.c_int, C.POINTER(_vp)],
+            "mcst_engine_destroy": [C.POINTER(_vp)],
+            "mcst_run_create": [_vp, u32, u32, dbl, u64, C.POINTER(i32)],
+            "mcst_run_set_model_params": [_vp, i32, _f64p, C.c_int],
+            "mcst_run_add_flowmap": [_vp, i32, _f64p, u64, _u32p, _u32p, _f64p],
+            "mcst_run_set_steps_per_flowmap": [_vp, i32, u32],
+            "mcst_run_add_feed": [_vp, i32, C.c_int, dbl, dbl, u32, i64, u32, _u32p, _f64p],
+            "mcst_run_set_concentrations": [_vp, i32, _f64p],
+            "mcst_run_set_removal": [_vp, i32, dbl, u64],
+            "mcst_run_set_division_cap": [_vp, i32, u32],
+            "mcst_run_set_probes": [_vp, i32, u32],
+            "mcst_run_set_step_target": [_vp, i32, u32],
+            "mcst_run_init_particles": [_vp, i32, u64, u64, dbl],
+            "mcst_run_set_particles": [_vp, i32, u64, u64, _u32p, _u8p, _f64p, _f64p, _f64p, dbl],
+            "mcst_engine_commit": [_vp],
+            "mcst_engine_step": [_vp, u32],
+            "mcst_engine_step_stages": [_vp, u32],
+            "mcst_engine_capture": [_vp, u32],
+            "mcst_engine_replay": [_vp, u32],
+            "mcst_engine_synchronize": [_vp],
+            "mcst_engine_profile": [_vp, u32, C.POINTER(dbl), _f64p],
+            "mcst_run_get_counters": [_vp, i32, _u64p],
+            "mcst_run_get_n": [_vp, i32, C.POINTER(u64)],
+            "mcst_run_get_weight": [_vp, i32, C.POINTER(dbl)],
+            "mcst_run_get_particles": [_vp, i32, _vp, _vp, _vp, _vp, _vp],
+            "mcst_run_get_concentrations": [_vp, i32, _f64p],
+            "mcst_run_get_mass": [_vp, i32, _f64p],
+            "mcst_run_get_contributions": [_vp, i32, _f64p],
+            "mcst_run_get_repartition": [_vp, i32, _u64p],
+            "mcst_run_get_exit_ages": [_vp, i32, _vp, u64, C.POINTER(u64)],
+            "mcst_run_get_division_ages": [_vp, i32, _vp, u64, C.POINTER(u64)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.mcst_last_error.argtypes = []
+        L.mcst_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ck(rc, what):
+    if rc != 0:
+        msg = lib().mcst_last_error().decode(errors="replace")
+        raise MCSTError(f"{what} failed ({rc}): {msg}")
+
+
+def model_info(name):
+    mid, nv, ns, npar = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _ck(lib().mcst_model_lookup(name.encode(), C.byref(mid), C.byref(nv), C.byref(ns), C.byref(npar)),
+        "model_lookup")
+    return mid.value, nv.value, ns.value, npar.value
+
+
+def default_params(name):
+    mid, nv, ns, npar = model_info(name)
+    p = np.zeros(16)
+    _ck(lib().mcst_model_default_params(mid, p, 16), "default_params")
+    return p[:npar].copy()
+
+
+class Engine:
+    """A device-resident batch of runs sharing one particle model."""
+
+    def __init__(self, model, device=0):
+        self.model = model
+        _, self.n_var, self.n_s, self.n_params = model_info(model)
+        h = _vp()
+        _ck(lib().mcst_engine_create(model.encode(), device, C.byref(h)), "engine_create")
+        self.h = h
+        self.runs = []
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().mcst_engine_destroy(C.byref(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_run(self, n_c, dt, seed):
+        rid = C.c_int32()
+        _ck(lib().mcst_run_create(self.h, n_c, self.n_s, dt, seed, C.byref(rid)), "run_create")
+        run = Run(self, rid.value, n_c)
+        self.runs.append(run)
+        return run
+
+    def commit(self):
+        _ck(lib().mcst_engine_commit(self.h), "commit")
+
+    def step(self, n=1):
+        _ck(lib().mcst_engine_step(self.h, n), "step")
+
+    def step_stages(self, stages):
+        _ck(lib().mcst_engine_step_stages(self.h, stages), "step_stages")
+
+    def capture(self, n_steps):
+        _ck(lib().mcst_engine_capture(self.h, n_steps), "capture")
+
+    def replay(self, times=1):
+        _ck(lib().mcst_engine_replay(self.h, times), "replay")
+
+    def synchronize(self):
+        _ck(lib().mcst_engine_synchronize(self.h), "synchronize")
+
+    def profile(self, n_steps):
+        """(total ms, per-kernel mean ms: transport, particle, bookkeep, removal, merge, layout)."""
+        tot = C.c_double()
+        per = np.zeros(6)
+        _ck(lib().mcst_engine_profile(self.h, n_steps, C.byref(tot), per), "profile")
+        return tot.value, per
+
+
+class Run:
+    """One run of an Engine; method names mirror oracle.OracleRun."""
+
+    def __init__(self, engine, rid, n_c):
+        self.e, self.id, self.n_c = engine, rid, n_c
+        self.n_s, self.n_var = engine.n_s, engine.n_var
+
+    @property
+    def _h(self):
+        return self.e.h
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float64)
+        _ck(lib().mcst_run_set_model_params(self._h, self.id, p, len(p)), "set_params")
+
+    def add_flowmap(self, V, rows, cols, flows):
+        _ck(lib().mcst_run_add_flowmap(self._h, self.id, np.ascontiguousarray(V, np.float64), len(flows),
+                                       np.ascontiguousarray(rows, np.uint32), np.ascontiguousarray(cols, np.uint32),
+                                       np.ascontiguousarray(flows, np.float64)), "add_flowmap")
+
+    def set_steps_per_map(self, k):
+        _ck(lib().mcst_run_set_steps_per_flowmap(self._h, self.id, k), "steps_per_map")
+
+    def add_feed(self, kind, f0, param, inp, out, species, conc):
+        sp = np.ascontiguousarray(species, np.uint32)
+        cc = np.ascontiguousarray(conc, np.float64)
+        _ck(lib().mcst_run_add_feed(self._h, self.id, FEED_KINDS[kind], f0, param, inp, -1 if out is None else out,
+                                    len(sp), sp, cc), "add_feed")
+
+    def set_concentrations(self, Cmat):
+        _ck(lib().mcst_run_set_concentrations(self._h, self.id, np.ascontiguousarray(Cmat, np.float64)),
+            "set_concentrations")
+
+    def set_compaction(self, ratio, min_removal):
+        _ck(lib().mcst_run_set_removal(self._h, self.id, ratio, min_removal), "set_removal")
+
+    def set_stage_cap(self, cap):
+        _ck(lib().mcst_run_set_division_cap(self._h, self.id, cap), "set_division_cap")
+
+    def set_probes(self, cap):
+        _ck(lib().mcst_run_set_probes(self._h, self.id, cap), "set_probes")
+
+    def set_step_target(self, steps):
+        _ck(lib().mcst_run_set_step_target(self._h, self.id, steps), "set_step_target")
+
+    def init_particles(self, n, capacity, x0):
+        _ck(lib().mcst_run_init_particles(self._h, self.id, n, capacity, x0), "init_particles")
+
+    def set_particles(self, capacity, pos, status, state, age_hyd, age_div, weight):
+        n = len(pos)
+        _ck(lib().mcst_run_set_particles(self._h, self.id, n, capacity, np.ascontiguousarray(pos, np.uint32),
+                                         np.ascontiguousarray(status, np.uint8),
+                                         np.ascontiguousarray(state, np.float64).reshape(-1),
+                                         np.ascontiguousarray(age_hyd, np.float64),
+                                         np.ascontiguousarray(age_div, np.float64), weight), "set_particles")
+
+    @property
+    def n(self):
+        v = C.c_uint64()
+        _ck(lib().mcst_run_get_n(self._h, self.id, C.byref(v)), "get_n")
+        return int(v.value)
+
+    @property
+    def weight(self):
+        v = C.c_double()
+        _ck(lib().mcst_run_get_weight(self._h, self.id, C.byref(v)), "get_weight")
+        return float(v.value)
+
+    def particles(self):
+        n = self.n
+        pos = np.zeros(n, np.uint32)
+        st = np.zeros(n, np.uint8)
+        state = np.zeros((self.n_var, n))
+        ah = np.zeros(n)
+        ad = np.zeros(n)
+        if n:
+            _ck(lib().mcst_run_get_particles(self._h, self.id, pos.ctypes.data, st.ctypes.data, state.ctypes.data,
+                                             ah.ctypes.data, ad.ctypes.data), "get_particles")
+        return {"pos": pos, "status": st, "state": state, "age_hyd": ah, "age_div": ad}
+
+    def _mat(self, fn):
+        out = np.zeros((self.n_s, self.n_c))
+        _ck(getattr(lib(), fn)(self._h, self.id, out), fn)
+        return out
+
+    def concentrations(self):
+        return self._mat("mcst_run_get_concentrations")
+
+    def mass(self):
+        return self._mat("mcst_run_get_mass")
+
+    def contributions(self):
+        return self._mat("mcst_run_get_contributions")
+
+    def counters(self):
+        c = np.zeros(len(COUNTERS), np.uint64)
+        _ck(lib().mcst_run_get_counters(self._h, self.id, c), "get_counters")
+        return {k: int(v) for k, v in zip(COUNTERS, c)}
+
+    def repartition(self):
+        h = np.zeros(self.n_c, np.uint64)
+        _ck(lib().mcst_run_get_repartition(self._h, self.id, h), "get_repartition")
+        return h
+
+    def _probe(self, fn):
+        n = C.c_uint64()
+        _ck(getattr(lib(), fn)(self._h, self.id, None, 0, C.byref(n)), fn)
+        out = np.zeros(max(n.value, 1))
+        _ck(getattr(lib(), fn)(self._h, self.id, out.ctypes.data, n.value, C.byref(n)), fn)
+        return out[: n.value]
+
+    def exit_ages(self):
+        return self._probe("mcst_run_get_exit_ages")
+
+    def division_ages(self):
+        return self._probe("mcst_run_get_division_ages")

@@ -1,0 +1,162 @@
+"""Synthetic compartment networks and run set-ups for the benchmark cases.
+
+Every generator is deterministic in its seed and returns a `Case`: raw
+volumes and COO flow matrices (one per flowmap), feeds, initial liquid
+concentrations and particle settings.  `Case.apply(run)` configures any object
+with the run API (the HIP engine's `Run`, or the CPU oracle's run), so the same
+inputs reach both.  Units are SI: m3, m3/s, kg/m3 (= g/L), kg, s.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+L = 1e-3  # one litre in m3
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    model: str
+    n_c: int
+    n_s: int
+    dt: float
+    seed: int
+    maps: list                      # [(V[n_c], rows, cols, flows)]
+    steps_per_map: int = 1
+    feeds: list = dataclasses.field(default_factory=list)  # (kind, f0, param, in, out|None, species, conc)
+    C0: np.ndarray = None           # [n_s][n_c]
+    params: np.ndarray = None
+    n_particles: int = 0
+    capacity: int = 0
+    x0: float = 1.0
+    n_steps: int = 1
+
+    def apply(self, run, particles=True):
+        if self.params is not None:
+            run.set_params(self.params)
+        for V, r, c, q in self.maps:
+            run.add_flowmap(V, r, c, q)
+        run.set_steps_per_map(self.steps_per_map)
+        for f in self.feeds:
+            run.add_feed(*f)
+        run.set_concentrations(self.C0)
+        if particles:
+            run.init_particles(self.n_particles, self.capacity or 2 * self.n_particles, self.x0)
+        return run
+
+    @property
+    def nnz(self):
+        return len(self.maps[0][3])
+
+
+def _symmetric(edges, q):
+    """Bidirectional exchange: F(i,j) = F(j,i) = q_edge."""
+    i, j = edges[:, 0].astype(np.uint32), edges[:, 1].astype(np.uint32)
+    return np.concatenate([i, j]), np.concatenate([j, i]), np.concatenate([q, q])
+
+
+def ring_with_chords(n=64, n_chords=16, rng=None):
+    rng = rng or np.random.default_rng(0)
+    ring = np.stack([np.arange(n), (np.arange(n) + 1) % n], axis=1)
+    chords = set()
+    while len(chords) < n_chords:
+        a, b = rng.integers(0, n, 2)
+        if a == b or abs(a - b) in (1, n - 1):
+            continue
+        chords.add((min(a, b), max(a, b)))
+    return np.concatenate([ring, np.array(sorted(chords))])
+
+
+def geometric_graph_cylinder(n, mean_degree=6.0, rng=None, radius=1.0, height=1.0):
+    """Random geometric graph in a cylinder: the n*mean_degree/2 closest pairs."""
+    from scipy.spatial import cKDTree
+
+    rng = rng or np.random.default_rng(0)
+    r = radius * np.sqrt(rng.random(n))
+    th = 2 * math.pi * rng.random(n)
+    pts = np.stack([r * np.cos(th), r * np.sin(th), height * rng.random(n)], axis=1)
+    n_edges = int(round(n * mean_degree / 2))
+    tree = cKDTree(pts)
+    vol = math.pi * radius * radius * height
+    d = (3 * vol * mean_degree / (4 * math.pi * n)) ** (1 / 3)
+    while True:
+        pairs = tree.query_pairs(d, output_type="ndarray")
+        if len(pairs) >= n_edges or d > 10:
+            break
+        d *= 1.25
+    dist = np.linalg.norm(pts[pairs[:, 0]] - pts[pairs[:, 1]], axis=1)
+    keep = np.argsort(dist, kind="stable")[:n_edges]
+    e = pairs[keep]
+    return e[np.lexsort((e[:, 1], e[:, 0]))], pts
+
+
+def config0(n_particles=100_000, n_steps=10_000, seed=1):
+    """64-compartment ring + 16 chords, 1 species, mass-based Monod, static flowmap."""
+    rng = np.random.default_rng(seed)
+    n = 64
+    edges = ring_with_chords(n, 16, rng)
+    V = rng.uniform(0.5, 2.0, n) * L
+    q = np.exp(rng.uniform(math.log(1e-5), math.log(1e-3), len(edges)))
+    rows, cols, flows = _symmetric(edges, q)
+    C0 = np.full((1, n), 5.0)
+    p = np.array([0.5 / 3600.0, 0.01, 0.5, 2.0, 1e-15, 0.1])
+    return Case("config0_ring64", "monod_mass", n, 1, 0.01, seed, [(V, rows, cols, flows)],
+                feeds=[("constant", 1e-5, 0.0, 0, 32, [0], [10.0])], C0=C0, params=p,
+                n_particles=n_particles, x0=1.0, n_steps=n_steps)
+
+
+def config1(n_particles=10_000_000, n_steps=1000, seed=7, n_c=5000, n_maps=20, steps_per_map=100):
+    """5000-compartment geometric graph, 20 cyclic flowmaps, 2 species, dual-substrate Monod."""
+    rng = np.random.default_rng(seed)
+    edges, _ = geometric_graph_cylinder(n_c, 6.0, rng)
+    V = rng.uniform(0.1, 1.0, n_c) * L
+    maps = []
+    for _ in range(n_maps):
+        q = rng.lognormal(math.log(1e-4), 1.0, len(edges))
+        rows, cols, flows = _symmetric(edges, q)
+        maps.append((V, rows, cols, flows))
+    C0 = np.stack([np.full(n_c, 5.0), np.full(n_c, 8e-3)])
+    p = np.array([2.8e-4, 0.01, 1e-4, 0.5, 1.0, 100.0, 1e-4, 2e-15, 1.4e-15, 0.2e-15])
+    return Case("config1_rgg5000", "dual_monod", n_c, 2, 0.01, seed, maps, steps_per_map=steps_per_map,
+                C0=C0, params=p, n_particles=n_particles, x0=1.0, n_steps=n_steps)
+
+
+def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, steps_per_map=3,
+               feeds=True, fast=True):
+    """A small network for parity tests: chords + feed/outlet, fast growth so
+    that divisions, moves and exits all happen within a few steps."""
+    rng = np.random.default_rng(seed)
+    n_s = {"monod_mass": 1, "dual_monod": 2, "monod": 1, "fixed_length": 1, "simple_acetate": 2}[model]
+    if n_c > 1:
+        edges = ring_with_chords(n_c, min(4, max(0, n_c * (n_c - 3) // 2)), rng) if n_c > 3 else \
+            np.array([[i, (i + 1) % n_c] for i in range(n_c)])
+        edges = np.unique(np.sort(edges, axis=1), axis=0)
+        edges = edges[edges[:, 0] != edges[:, 1]]
+    else:
+        edges = np.zeros((0, 2), np.int64)
+    V = rng.uniform(0.5, 2.0, n_c) * L
+    maps = []
+    for _ in range(n_maps):
+        q = np.exp(rng.uniform(math.log(2e-3), math.log(2e-2), len(edges)))
+        rows, cols, flows = _symmetric(edges, q) if len(edges) else (np.zeros(0, np.uint32),) * 2 + (np.zeros(0),)
+        maps.append((V, rows, cols, flows))
+    C0 = np.stack([np.full(n_c, 5.0 if s == 0 else 1.0) for s in range(n_s)])
+    fd = []
+    if feeds:
+        out = n_c // 2
+        fd.append(("constant", 2e-3, 0.0, 0, out, [0], [10.0]))
+    if model == "monod_mass":
+        p = np.array([0.5, 0.01, 0.5, 2.0, 1e-15, 0.1]) if fast else None
+    elif model == "dual_monod":
+        p = np.array([0.3, 0.01, 1e-4, 0.5, 1.0, 10.0, 0.05, 2e-15, 1.9e-15, 0.05e-15]) if fast else None
+    elif model == "monod":
+        p = np.array([0.3, 1e-3, 2.0, 2e-6, 0.6e-6, 3.0]) if fast else None
+    elif model == "fixed_length":
+        p = np.array([2e-7, 2e-6, 1e-3, 0.6e-6]) if fast else None
+    else:
+        p = None
+    return Case(f"small_{model}", model, n_c, n_s, 0.01, seed, maps, steps_per_map=steps_per_map, feeds=fd,
+                C0=C0, params=p, n_particles=n_particles, x0=1.0)

@@ -1,0 +1,174 @@
+/*
+ * mcst.h -- C ABI of the MI355X particle Monte Carlo step (libmcst.so).
+ *
+ * Replaces, for the particle step of BioCMA-MCST, the internal C++ boundary
+ * formed by (paths relative to the reference root):
+ *   CycleFunctors<Space,Model>::update / launch_model / launch_move /
+ *     get_host_reduction ......... apps/libs/simulation/public/simulation/kernels/kernels.hpp:25-224
+ *   SimulationUnit::cycleProcess / post_cycle / update_feed / ode_step / advance
+ *                                  apps/libs/simulation/public/simulation/simulation.hpp:183-239,
+ *                                  apps/libs/simulation/src/simulation.model.cpp:42-154
+ *   ScalarSimulation::performStep / set_transition / set_mass
+ *                                  apps/libs/simulation/src/implScalar.cpp:250-264,316-330
+ *   ReactorDomain::update / set_leaving_flow
+ *                                  apps/libs/mc/src/domain.cpp:43-108
+ *   ParticlesContainer<Model> (division buffer, removal, merge)
+ *                                  apps/libs/mc/public/mc/particles_container.hpp:56-239
+ *   MC::init / post_init_weight .. apps/libs/mc/public/mc/mcinit.hpp:67-105, apps/libs/mc/src/unit.cpp:232-306
+ *   MonteCarloUnit::getRepartition apps/libs/mc/src/unit.cpp:190-230
+ *
+ * Conventions follow the reference's own C ABI (apps/api/public/api/api_raw.h):
+ * every function returns an int, 0 on success and a negative code on error
+ * (mcst_last_error() describes the last failure of the calling thread);
+ * handles are opaque; host buffers are caller-owned and copied; device
+ * buffers are owned by the library.  One engine is driven by one host thread;
+ * its kernels run in order on the engine's HIP stream.
+ *
+ * An engine is a batch of independent runs that share one particle model.
+ * A single simulation is a batch of one run.
+ */
+#ifndef MCST_H
+#define MCST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCST_OK 0
+#define MCST_ERR_ARG (-1)      /* invalid argument or handle */
+#define MCST_ERR_STATE (-2)    /* call not valid in the engine's state */
+#define MCST_ERR_DEVICE (-3)   /* HIP runtime or device failure */
+#define MCST_ERR_MEMORY (-4)   /* allocation failure */
+
+/* particle status byte (apps/libs/mc/public/mc/alias.hpp:124-130) */
+#define MCST_IDLE 0
+#define MCST_DIVISION 1
+#define MCST_EXIT 2
+#define MCST_DEAD 3
+
+/* feed kinds (apps/libs/simulation/src/feed_descriptor.cpp:18-28) */
+#define MCST_FEED_CONSTANT 0
+#define MCST_FEED_LINEAR 1
+#define MCST_FEED_EXPONENTIAL 2
+
+/* stage mask of mcst_engine_step_stages */
+#define MCST_STAGE_TRANSPORT 1u
+#define MCST_STAGE_MODEL 2u
+#define MCST_STAGE_CONTRIB 4u
+#define MCST_STAGE_MOVE 8u
+#define MCST_STAGE_LEAVE 16u
+#define MCST_STAGE_REMOVE 32u
+#define MCST_STAGE_ALL 63u
+
+/* counters of mcst_run_get_counters (the reference's MC::EventContainer
+ * tallies, apps/libs/mc/public/mc/events.hpp:149-264, plus bookkeeping) */
+#define MCST_C_STEPS 0
+#define MCST_C_BIRTHS 1      /* successful divisions (NewParticle) */
+#define MCST_C_EXITS 2       /* particles that left through an outlet (Exit) */
+#define MCST_C_DEATHS 3      /* particles whose model returned Dead */
+#define MCST_C_MOVES 4       /* compartment changes (Move) */
+#define MCST_C_OVERFLOW 5    /* divisions refused for lack of room (Overflow) */
+#define MCST_C_COMPACTIONS 6 /* removal passes */
+#define MCST_C_N 7           /* used slots */
+#define MCST_C_INACTIVE 8    /* Exit/Dead slots not yet removed */
+#define MCST_C_PROCESSED 9   /* particle-steps: Idle particles that entered a pass */
+#define MCST_C_COUNT 10
+
+typedef struct mcst_engine mcst_engine;
+
+int mcst_version(int* major, int* minor, int* patch);
+const char* mcst_last_error(void);
+
+/* Model registry: the reference's ModelType implementations
+ * (apps/libs/models/public/models/{monod,fixed_length,simple_acetate}.hpp)
+ * compiled for gfx950, plus the mass-based models of the benchmark cases.
+ * Names: "monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate". */
+int mcst_model_lookup(const char* name, int* model_id, int* n_var, int* n_species, int* n_params);
+int mcst_model_default_params(int model_id, double* params, int n);
+
+/* Engine lifetime: binds HIP device `device` and creates the engine's stream. */
+int mcst_engine_create(const char* model_name, int device, mcst_engine** out);
+int mcst_engine_destroy(mcst_engine** engine);
+
+/* ---- run definition (before mcst_engine_commit) ---- */
+/* A run: compartments, species, time step, RNG seed.  (SimulationUnit ctor,
+ * apps/libs/simulation/src/simulation.cpp:202-236) */
+int mcst_run_create(mcst_engine* e, uint32_t n_compartments, uint32_t n_species, double dt, uint64_t seed,
+                    int32_t* run_id);
+int mcst_run_set_model_params(mcst_engine* e, int32_t run, const double* params, int n);
+/* One flowmap: liquid volumes [n_c] (m3) and the compartment flow matrix as
+ * COO (rows[k] -> cols[k], flows[k] m3/s; diagonal and non-positive entries
+ * ignored, duplicates summed).  The library builds the transition matrix,
+ * out-flows and cumulative neighbour probabilities from it (what rcmtool and
+ * ReactorDomain::update provide upstream, domain.cpp:43-74). */
+int mcst_run_add_flowmap(mcst_engine* e, int32_t run, const double* volumes, uint64_t nnz, const uint32_t* rows,
+                         const uint32_t* cols, const double* flows);
+/* The flowmap of step k is (k / steps) mod n_flowmaps. */
+int mcst_run_set_steps_per_flowmap(mcst_engine* e, int32_t run, uint32_t steps);
+/* A feed (FeedDescriptor, apps/libs/simulation/public/simulation/feed_descriptor.hpp:11-161):
+ * flow(t) = f0 (constant), f0 + t*param (linear), f0 + exp(param*t) (exponential);
+ * adds flow*conc[v] of species[v] at `input`; output >= 0 is a leaving flow. */
+int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
+                      int64_t output, uint32_t n_values, const uint32_t* species, const double* conc);
+/* Initial concentrations [n_species][n_c] (kg/m3); liquid mass = C * V. */
+int mcst_run_set_concentrations(mcst_engine* e, int32_t run, const double* C);
+/* Removal threshold: remove when inactive > max(min_removal, ratio * n)
+ * (particles_container.hpp:539-557; BIOMC_MC_REMOVE_RATIO_THRESHOLD /
+ * BIOMC_MC_MINIMUM_REMOVAL). */
+int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min_removal);
+/* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
+int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
+/* Keep up to `cap` exit ages and division ages (the reference's probes). */
+int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap);
+/* The run stops stepping once it has done `steps` steps (default: never). */
+int mcst_run_set_step_target(mcst_engine* e, int32_t run, uint32_t steps);
+/* Initial particles: n particles uniformly over compartments with
+ * Model::init, capacity slots reserved, weight = x0 * V_total / sum(mass)
+ * (MC::init + post_init_weight).  Needs a flowmap. */
+int mcst_run_init_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, double x0);
+/* Or explicit particles: pos[n], status[n], state[n_var][n], ages[n]. */
+int mcst_run_set_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, const uint32_t* pos,
+                           const uint8_t* status, const double* state, const double* age_hyd,
+                           const double* age_div, double weight);
+
+/* Allocate device memory, upload every run; the engine can then step. */
+int mcst_engine_commit(mcst_engine* e);
+
+/* ---- stepping ---- */
+/* n steps of every active run: feed, transport, particle cycle, removal,
+ * merge (the body of main_loop, apps/core/src/host_specific.cpp:280-291). */
+int mcst_engine_step(mcst_engine* e, uint32_t n_steps);
+/* One step restricted to the MCST_STAGE_* bits in `stages`. */
+int mcst_engine_step_stages(mcst_engine* e, uint32_t stages);
+/* Capture `n_steps` steps into a HIP graph (replaces any earlier one). */
+int mcst_engine_capture(mcst_engine* e, uint32_t n_steps);
+/* Replay the captured graph `times` times (times * n_steps steps). */
+int mcst_engine_replay(mcst_engine* e, uint32_t times);
+int mcst_engine_synchronize(mcst_engine* e);
+/* Time n steps with HIP events on the engine stream: total ms and the mean
+ * duration of each step kernel, in the order transport, particle, bookkeep,
+ * removal (4 kernels together), merge, layout -> per_kernel_ms[6]. */
+int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, double* per_kernel_ms);
+
+/* ---- results ---- */
+int mcst_run_get_counters(mcst_engine* e, int32_t run, uint64_t* counters /* MCST_C_COUNT */);
+int mcst_run_get_n(mcst_engine* e, int32_t run, uint64_t* n);
+int mcst_run_get_weight(mcst_engine* e, int32_t run, double* weight);
+/* The first n slots (n from mcst_run_get_n); any pointer may be NULL. */
+int mcst_run_get_particles(mcst_engine* e, int32_t run, uint32_t* pos, uint8_t* status, double* state,
+                           double* age_hyd, double* age_div);
+int mcst_run_get_concentrations(mcst_engine* e, int32_t run, double* C);
+int mcst_run_get_mass(mcst_engine* e, int32_t run, double* m);
+/* Particle sources of the last step, [n_species][n_c] (kg/s). */
+int mcst_run_get_contributions(mcst_engine* e, int32_t run, double* S);
+int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c] */);
+/* Exit / division ages recorded so far (up to the probe capacity). */
+int mcst_run_get_exit_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n);
+int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

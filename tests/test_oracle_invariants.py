@@ -1,0 +1,251 @@
+"""The oracle's step against closed forms, conservation laws and exact
+distributions (checks that a dropped term, a wrong index or a swapped
+operand in the restatement would fail)."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+TRANSPORT, MODEL, CONTRIB, MOVE, LEAVE, REMOVE = 1, 2, 4, 8, 16, 32
+
+
+def one_box(oracle_lib, model="monod_mass", V=1e-3, seed=1, dt=0.01):
+    r = oracle_lib.OracleRun(model, 1, oracle_lib.model_info(model)[1], dt, seed)
+    r.add_flowmap(np.array([V]), np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0))
+    return r
+
+
+def test_monod_euler_closed_form(oracle_lib):
+    r = one_box(oracle_lib)
+    mu_max, ks, Y = 0.5, 0.01, 0.5
+    r.set_params([mu_max, ks, Y, 2.0, 1e-15, 0.1])
+    s = 3.0
+    r.set_concentrations(np.full((1, 1), s))
+    n = 50
+    m0 = np.linspace(1e-15, 1.2e-15, n)
+    w = 7.0
+    r.set_particles(2 * n, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([m0, m0 / 1.0]),
+                    np.zeros(n), np.zeros(n), w)
+    r.set_params([mu_max, ks, Y, 1e9, 1e-15, 0.1])  # no division
+    mu = mu_max * s / (ks + s)
+    for k in range(1, 41):
+        r.step_stages(MODEL | CONTRIB)
+        st = r.particles()["state"]
+        np.testing.assert_allclose(st[0], m0 * (1 + 0.01 * mu) ** k, rtol=1e-12)
+        np.testing.assert_allclose(r.particles()["age_div"], np.full(n, 0.01 * k), rtol=1e-12)
+    # the last step's sources: S = w * sum(-mu*m/Y) at the pre-growth mass
+    m_prev = m0 * (1 + 0.01 * mu) ** 39
+    np.testing.assert_allclose(r.contributions()[0, 0], -w * np.sum(mu * m_prev / Y), rtol=1e-12)
+
+
+def test_division_conserves_mass_and_respects_budget(oracle_lib):
+    r = one_box(oracle_lib)
+    r.set_params([0.5, 0.01, 0.5, 2.0, 1e-15, 0.1])
+    r.set_concentrations(np.full((1, 1), 1.0))
+    n = 3000  # three tiles
+    m = np.full(n, 2.5e-15)
+    r.set_particles(n + 50, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([m, np.full(n, 1e-15)]),
+                    np.zeros(n), np.full(n, 3.0), 1.0)
+    r.set_stage_cap(64)
+    r.step_stages(MODEL)
+    c = r.counters()
+    # budget per tile = min(64, (cap - n) / tiles) = min(64, 50 // 3) = 16
+    assert c["births"] == 3 * 16 and c["overflow"] == n - 48 and r.n == n + 48
+    p = r.particles()
+    grown = 2.5e-15 + 0.01 * (0.5 * 1.0 / 1.01) * 2.5e-15
+    np.testing.assert_allclose(p["state"][0].sum(), n * grown, rtol=1e-13)
+    div_slots = np.nonzero(p["state"][0][:n] < 2e-15)[0]
+    # the first 16 requests of each tile, in slot order
+    assert list(div_slots) == [t * 1024 + k for t in range(3) for k in range(16)]
+    assert np.all(p["age_div"][div_slots] == 0.0) and np.all(p["age_div"][n:] == 0.0)
+    np.testing.assert_array_equal(p["state"][0][n:], p["state"][0][div_slots])
+
+
+def test_closed_system_mass_balance(oracle_lib):
+    """No feed, no outlet: liquid substrate + pending sources + biomass/Y is
+    invariant, to 1e-10 relative."""
+    from mcst import cases
+
+    c = cases.small_case("monod_mass", n_c=10, n_particles=4000, feeds=False)
+    r = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    c.apply(r)
+    Y = c.params[2]
+
+    def total():
+        return r.mass().sum() + c.dt * r.contributions().sum() + r.weight * r.particles()["state"][0].sum() / Y
+
+    t0 = total()
+    for _ in range(25):
+        r.step(1)
+        assert abs(total() - t0) <= 1e-10 * abs(t0)
+    assert r.counters()["births"] > 100  # divisions happened
+
+
+def test_cstr_exit_probability_and_residence_time(oracle_lib):
+    """0-D reactor with an outlet: exit probability per step is
+    1 - exp(-dt Q/V); residence times (in steps) are geometric."""
+    V, Q, dt = 1e-3, 2e-3, 0.01
+    r = one_box(oracle_lib, V=V, dt=dt)
+    r.add_feed("constant", Q, 0.0, 0, 0, [0], [1.0])
+    r.set_concentrations(np.full((1, 1), 1.0))
+    r.init_particles(200000, 200000, 1.0)
+    r.set_compaction(0.0, 0)
+    p = 1 - math.exp(-dt * Q / V)
+    alive = 200000
+    exits = []
+    for _ in range(60):
+        r.step_stages(TRANSPORT | LEAVE | REMOVE)
+        n_now = r.n
+        exits.append(alive - n_now)
+        alive = n_now
+    total_exit = 200000 - alive
+    expect = 200000 * (1 - (1 - p) ** 60)
+    sd = math.sqrt(200000 * (1 - (1 - p) ** 60) * (1 - p) ** 60)
+    assert abs(total_exit - expect) < 5 * sd
+    ages = np.rint(r.exit_ages() / dt).astype(int)
+    assert len(ages) == total_exit and ages.min() >= 1 and ages.max() <= 60
+    # geometric(p) truncated at 60 steps: chi-square on the first 40 steps
+    k = np.arange(1, 41)
+    pk = (1 - p) ** (k - 1) * p
+    obs = np.bincount(ages, minlength=61)[1:41]
+    expct = 200000 * pk
+    chi2 = np.sum((obs - expct) ** 2 / expct)
+    assert stats.chi2.sf(chi2, len(k) - 1) > 1e-3
+
+
+def two_box(oracle_lib, F=(3e-3, 1e-3), V=(1e-3, 2e-3), n=100000, dt=0.01, seed=2):
+    r = oracle_lib.OracleRun("monod_mass", 2, 1, dt, seed)
+    r.add_flowmap(np.array(V), np.array([0, 1], np.uint32), np.array([1, 0], np.uint32), np.array(F))
+    r.set_concentrations(np.ones((1, 2)))
+    r.init_particles(n, n, 1.0)
+    return r
+
+
+def test_move_probability_is_linear(oracle_lib):
+    """Move test dt*Q/V > u (probability_leaving.hpp:313-325)."""
+    r = two_box(oracle_lib)
+    before = r.particles()["pos"]
+    r.step_stages(MOVE)
+    after = r.particles()["pos"]
+    for i, (q, v) in enumerate(((3e-3, 1e-3), (1e-3, 2e-3))):
+        n_i = int(np.sum(before == i))
+        moved = int(np.sum((before == i) & (after != i)))
+        p = 0.01 * q / v
+        assert abs(moved - n_i * p) < 5 * math.sqrt(n_i * p * (1 - p))
+
+
+def test_neighbour_choice_follows_flows(oracle_lib):
+    n = 4
+    rows = np.array([0, 0, 0, 1, 2, 3], np.uint32)
+    cols = np.array([1, 2, 3, 0, 0, 0], np.uint32)
+    F = np.array([1e-2, 3e-2, 6e-2, 1e-2, 3e-2, 6e-2])
+    r = oracle_lib.OracleRun("monod_mass", n, 1, 0.1, 9)
+    r.add_flowmap(np.full(n, 1.0), rows, cols, F)
+    r.set_concentrations(np.ones((1, n)))
+    N = 120000
+    r.set_particles(N, np.zeros(N, np.uint32), np.zeros(N, np.uint8), np.ones((2, N)) * 1e-15,
+                    np.zeros(N), np.zeros(N), 1.0)
+    r.step_stages(MOVE)
+    pos = r.particles()["pos"]
+    p_move = 0.1 * F[:3].sum() / 1.0
+    counts = np.bincount(pos, minlength=n)
+    expct = N * np.array([1 - p_move, p_move * 0.1, p_move * 0.3, p_move * 0.6])
+    chi2 = np.sum((counts - expct) ** 2 / expct)
+    assert stats.chi2.sf(chi2, 3) > 1e-3
+
+
+def test_stationary_distribution_follows_volumes(oracle_lib):
+    """A conservative closed network has the volume-proportional stationary
+    distribution: each step's inflow dt*sum_i F(i,j) equals outflow dt*Q_j."""
+    from mcst import cases
+
+    c = cases.small_case("monod_mass", n_c=12, n_particles=60000, feeds=False, n_maps=1)
+    r = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    c.apply(r)
+    for _ in range(300):  # mixing: the slowest compartment moves with p ~ 0.045 per step
+        r.step_stages(MOVE)
+    h = r.repartition().astype(float)
+    V = c.maps[0][0]
+    expct = h.sum() * V / V.sum()
+    chi2 = np.sum((h - expct) ** 2 / expct)
+    assert stats.chi2.sf(chi2, c.n_c - 1) > 1e-3
+
+
+def test_removal_bookkeeping(oracle_lib):
+    """apps/libs/mc/tests/test_container.cpp:108-148: removing k inactive
+    particles leaves n-k live ones, all below n, with the same states."""
+    for size, dead in ((1000, 10), (100, 99), (5000, 2500)):
+        r = one_box(oracle_lib)
+        rng = np.random.default_rng(size)
+        st = np.zeros(size, np.uint8)
+        idx = rng.choice(size, dead, replace=False)
+        st[idx] = 3
+        m = rng.uniform(1e-15, 2e-15, size)
+        r.set_particles(size, np.zeros(size, np.uint32), st, np.stack([m, m]), np.arange(size) * 1.0,
+                        np.zeros(size), 1.0)
+        r.set_compaction(0.0, 0)
+        r.step_stages(REMOVE)
+        p = r.particles()
+        assert r.n == size - dead and np.all(p["status"] == 0)
+        np.testing.assert_array_equal(np.sort(p["state"][0]), np.sort(m[st == 0]))
+        # gaps are filled from the tail: live particles below n - dead stay put
+        keep = np.nonzero(st[: size - dead] == 0)[0]
+        np.testing.assert_array_equal(p["age_hyd"][keep], keep * 1.0)
+
+
+def test_removal_threshold(oracle_lib):
+    """Removal only when inactive > max(min_removal, ratio * n)."""
+    r = one_box(oracle_lib)
+    n = 1000
+    st = np.zeros(n, np.uint8)
+    st[:10] = 2
+    m = np.full(n, 1e-15)
+    r.set_particles(n, np.zeros(n, np.uint32), st, np.stack([m, m]), np.zeros(n), np.zeros(n), 1.0)
+    r.set_compaction(0.01, 0)  # threshold 10: 10 inactive is not > 10
+    r.step_stages(REMOVE)
+    assert r.n == 1000 and r.counters()["inactive"] == 10
+    r.set_compaction(0.009, 0)  # threshold 9
+    r.step_stages(REMOVE)
+    assert r.n == 990 and r.counters()["inactive"] == 0 and r.counters()["compactions"] == 1
+
+
+def test_init_distribution_and_weight(oracle_lib):
+    r = one_box(oracle_lib, V=2e-3)
+    n = 50000
+    r.init_particles(n, n, 3.0)
+    m = r.particles()["state"][0]
+    a = (0 - 1e-15) / 1e-16
+    assert stats.kstest(m, stats.truncnorm(a, np.inf, loc=1e-15, scale=1e-16).cdf).pvalue > 1e-3
+    assert r.weight == (3.0 * 2e-3) / math.fsum(m) or abs(r.weight - 3.0 * 2e-3 / math.fsum(m)) < 1e-12 * r.weight
+    mb = r.particles()["state"][1]
+    ratio = m / mb - 1.0
+    assert stats.kstest(ratio, "uniform").pvalue > 1e-3
+
+
+def test_initial_positions_uniform(oracle_lib):
+    r = oracle_lib.OracleRun("monod_mass", 37, 1, 0.01, 5)
+    r.add_flowmap(np.full(37, 1e-3), np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0))
+    r.set_concentrations(np.ones((1, 37)))
+    r.init_particles(74000, 74000, 1.0)
+    h = np.bincount(r.particles()["pos"], minlength=37)
+    assert stats.chisquare(h).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate"])
+def test_threads_do_not_change_events(oracle_lib, model):
+    from mcst import cases
+
+    c = cases.small_case(model, n_particles=6000)
+    runs = []
+    for th in (1, 4):
+        r = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+        c.apply(r)
+        r.step(12, threads=th)
+        runs.append(r)
+    a, b = (x.particles() for x in runs)
+    assert runs[0].counters() == runs[1].counters()
+    np.testing.assert_array_equal(a["pos"], b["pos"])
+    np.testing.assert_array_equal(a["status"], b["status"])
+    np.testing.assert_allclose(a["state"], b["state"], rtol=1e-12)
+    np.testing.assert_allclose(runs[0].concentrations(), runs[1].concentrations(), rtol=1e-12)

@@ -1,0 +1,221 @@
+"""Parity of the HIP path with the CPU oracle on the same seeded inputs:
+positions, statuses, counters and repartition bit-exact; FP64 particle and
+liquid state within 1e-12 relative."""
+import numpy as np
+import pytest
+
+from parity import assert_parity, close, make_pair
+
+pytestmark = pytest.mark.gpu
+
+MODELS = ["monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate"]
+TRANSPORT, MODEL, CONTRIB, MOVE, LEAVE, REMOVE = 1, 2, 4, 8, 16, 32
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_full_step_parity(model):
+    from mcst import cases
+
+    c = cases.small_case(model, n_particles=5000)
+    eng, g, o = make_pair(c)
+    for k in range(4):
+        eng.step(5)
+        o.step(5)
+        assert_parity(g, o)
+    cnt = g.counters()
+    assert cnt["moves"] > 0 and cnt["exits"] > 0 and cnt["births"] > 0 and cnt["compactions"] > 0
+
+
+@pytest.mark.parametrize("stages", [TRANSPORT, MOVE, LEAVE | REMOVE, MODEL, MODEL | CONTRIB,
+                                    TRANSPORT | MODEL | CONTRIB, MOVE | LEAVE])
+def test_single_stage_parity(stages):
+    from mcst import cases
+
+    c = cases.small_case("dual_monod", n_particles=3000)
+    eng, g, o = make_pair(c)
+    for _ in range(6):
+        eng.step_stages(stages)
+        o.step_stages(stages)
+    assert_parity(g, o)
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 1023, 1024, 1025, 5000])
+def test_ragged_particle_counts(n):
+    from mcst import cases
+
+    c = cases.small_case("monod_mass", n_particles=n)
+    eng, g, o = make_pair(c, capacity=max(2 * n, 8))
+    eng.step(8)
+    o.step(8)
+    assert_parity(g, o)
+
+
+def test_zero_d_reactor_without_move():
+    from mcst import cases
+
+    c = cases.small_case("monod_mass", n_c=1, n_particles=4000)
+    eng, g, o = make_pair(c)
+    eng.step(15)
+    o.step(15)
+    assert_parity(g, o)
+    assert g.counters()["moves"] == 0 and g.counters()["exits"] > 0
+
+
+def test_closed_network_without_leave():
+    from mcst import cases
+
+    c = cases.small_case("monod", n_particles=4000, feeds=False)
+    eng, g, o = make_pair(c)
+    eng.step(12)
+    o.step(12)
+    assert_parity(g, o)
+    assert g.counters()["exits"] == 0
+
+
+def test_division_overflow_and_tight_capacity():
+    from mcst import cases
+
+    c = cases.small_case("simple_acetate", n_particles=3000)
+    eng, g, o = make_pair(c, stage_cap=8, capacity=3100)
+    eng.step(10)
+    o.step(10)
+    assert_parity(g, o)
+    assert g.counters()["overflow"] > 0
+
+
+def test_everything_leaves():
+    """A fast outlet empties the reactor: removal of all slots, then nothing."""
+    from mcst import cases
+
+    c = cases.small_case("fixed_length", n_c=1, n_particles=3000)
+    c.feeds = [("constant", 0.5, 0.0, 0, 0, [0], [1.0])]
+    eng, g, o = make_pair(c, ratio=0.0)
+    eng.step(30)
+    o.step(30)
+    assert_parity(g, o)
+    assert g.n < 3000
+
+
+def test_removal_thresholds():
+    from mcst import cases
+
+    c = cases.small_case("monod_mass", n_particles=6000)
+    for ratio, mn in ((0.0, 0), (0.05, 0), (0.0, 200), (1.0, 0)):
+        eng, g, o = make_pair(c, ratio=ratio, min_removal=mn)
+        eng.step(12)
+        o.step(12)
+        assert_parity(g, o)
+
+
+def test_graph_replay_matches_eager():
+    """Every replay of a captured step sequence equals eager stepping, with
+    host reads between replays."""
+    from mcst import cases
+
+    c = cases.small_case("dual_monod", n_particles=4000)
+    eng, g, o = make_pair(c)
+    eng.capture(3)
+    for _ in range(4):
+        eng.replay(1)
+        o.step(3)
+        assert_parity(g, o)
+
+
+def test_config0_full_size():
+    """The 64-compartment ring at its full 1e5 particles, 300 steps."""
+    from mcst import cases
+
+    c = cases.config0(n_particles=100_000)
+    eng, g, o = make_pair(c)
+    for _ in range(3):
+        eng.step(100)
+        o.step(100)
+    assert_parity(g, o)
+
+
+def test_config1_full_size_few_steps():
+    """The 5000-compartment network at 1e7 particles: parity over 3 steps
+    (the oracle is single-threaded here)."""
+    from mcst import cases
+
+    c = cases.config1(n_particles=10_000_000)
+    eng, g, o = make_pair(c)
+    eng.step(3)
+    o.step(3)
+    assert_parity(g, o, probes=False)
+
+
+def test_batch_of_runs_equals_runs_alone():
+    """A ragged batch: every run of the batch matches its own oracle."""
+    import mcst
+    from mcst import cases
+    from oracle.oracle import OracleRun
+
+    specs = [("monod_mass", 12, 3000, 1), ("monod_mass", 7, 500, 2), ("monod_mass", 30, 9000, 3),
+             ("monod_mass", 1, 1500, 4), ("monod_mass", 20, 0, 5)]
+    eng = mcst.Engine("monod_mass")
+    pairs = []
+    for model, n_c, n, seed in specs:
+        c = cases.small_case(model, n_c=n_c, n_particles=n, seed=seed)
+        g = eng.add_run(c.n_c, c.dt, c.seed)
+        o = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+        c.capacity = max(2 * n, 8)
+        for r in (g, o):
+            c.apply(r)
+        g.set_probes(1 << 16)
+        pairs.append((g, o))
+    eng.commit()
+    eng.step(10)
+    for g, o in pairs:
+        o.step(10)
+        assert_parity(g, o)
+
+
+def test_step_target_stops_a_run():
+    import mcst
+    from mcst import cases
+    from oracle.oracle import OracleRun
+
+    eng = mcst.Engine("monod_mass")
+    c = cases.small_case("monod_mass", n_particles=2000)
+    a = eng.add_run(c.n_c, c.dt, c.seed)
+    b = eng.add_run(c.n_c, c.dt, c.seed)
+    for r in (a, b):
+        c.apply(r)
+        r.set_probes(1 << 16)
+    a.set_step_target(4)
+    eng.commit()
+    eng.step(9)
+    oa = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    ob = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    for r in (oa, ob):
+        c.apply(r)
+    oa.step(4)
+    ob.step(9)
+    assert a.counters()["steps"] == 4 and b.counters()["steps"] == 9
+    assert_parity(a, oa)
+    assert_parity(b, ob)
+
+
+def test_glucose_equivalent_mass_balance_config1():
+    """Closed dual-substrate system: liquid glucose + pending sources +
+    weight * sum(m/Yxs + X) is conserved to 1e-10 relative."""
+    from mcst import cases
+    import mcst
+
+    c = cases.config1(n_particles=1_000_000, n_maps=3, steps_per_map=2)
+    eng = mcst.Engine(c.model)
+    g = eng.add_run(c.n_c, c.dt, c.seed)
+    c.apply(g)
+    eng.commit()
+    Yxs = c.params[3]
+
+    def total():
+        st = g.particles()["state"]
+        return (g.mass()[0].sum() + c.dt * g.contributions()[0].sum()
+                + g.weight * (np.sum(st[0]) / Yxs + np.sum(st[2])))
+
+    t0 = total()
+    for _ in range(4):
+        eng.step(5)
+        assert abs(total() - t0) <= 1e-10 * abs(t0)
