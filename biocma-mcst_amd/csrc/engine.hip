@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -229,6 +230,7 @@ struct mcst_engine {
   mcst::RunState* d_rs = nullptr;
   std::vector<mcst::RunDev> h_runs;
   int grid_particle = 1, grid_transport = 1;
+  mcst::ParticleLaunch plaunch;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_steps = 0;
@@ -447,16 +449,59 @@ int commit(mcst_engine* e) {
   b.runs = e->d_runs;
   b.rs = e->d_rs;
 
-  // launch geometry: a persistent grid over tiles, sized by occupancy
+  // launch geometry of the particle pass: LDS contribution accumulator when
+  // the largest run's [n_s][n_c] fits, then the block size with the most
+  // resident waves; a grid of resident blocks, each taking a tile range.
+  // MCST_PARTICLE_BLOCK (256/512/1024) and MCST_CONTRIB_LDS (0/1) override.
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, e->device));
-  int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id), BLOCK, 0));
-  per_cu = std::max(per_cu, 1);
-  e->grid_particle = (int)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * per_cu, T_max);
+  size_t lds_need = 0;
+  for (const auto& h : e->runs) lds_need = std::max(lds_need, (size_t)h.n_s * h.n_c * sizeof(double));
+  const size_t lds_cap = std::min<size_t>(std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin),
+                                          96 * 1024);
+  bool priv = lds_need <= lds_cap;
+  if (const char* v = std::getenv("MCST_CONTRIB_LDS")) priv = priv && std::atoi(v) != 0;
+  int forced_block = 0;
+  if (const char* v = std::getenv("MCST_PARTICLE_BLOCK")) forced_block = std::atoi(v);
+  ParticleLaunch best;
+  int best_waves = -1;
+  for (int blk : {256, 512, 1024}) {
+    if (!priv && blk != 256) continue;
+    if (forced_block && blk != forced_block) continue;
+    const void* fn = particle_kernel_ptr(M.id, blk, priv);
+    const size_t lds = priv ? lds_need : 0;
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, blk, lds) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    const int waves = per_cu * blk / 64;
+    if (per_cu > 0 && waves > best_waves) {
+      best_waves = waves;
+      best.block = blk;
+      best.priv = priv;
+      best.lds = lds;
+      best.grid = per_cu * prop.multiProcessorCount;
+    }
+  }
+  if (best_waves < 0) {  // no resident configuration with the accumulator: go direct
+    best = ParticleLaunch{};
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false), 256, 0));
+    best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
+  }
+  best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, T_max));
+  e->plaunch = best;
+  e->grid_particle = (int)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, T_max);
+  e->grid_particle = std::max(e->grid_particle, 1);
   e->grid_transport = (int)std::min<uint64_t>((n_comp + 255) / 256, 8192);
   e->grid_transport = std::max(e->grid_transport, 1);
-  e->grid_particle = std::max(e->grid_particle, 1);
+  if (std::getenv("MCST_VERBOSE"))
+    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d (lds cap %zu, per-block %zu, optin %zu)\n",
+                 best.block, best.grid, best.lds, (int)best.priv, lds_cap, (size_t)prop.sharedMemPerBlock,
+                 (size_t)prop.sharedMemPerBlockOptin);
   e->host_step.assign(R, 0);
   launch_layout(b, e->stream);
   HIP_TRY(hipGetLastError());
@@ -478,7 +523,7 @@ void launch_step(mcst_engine* e, uint32_t stages) {
   b.stages = stages;
   const uint32_t nv = (uint32_t)e->model->nv;
   launch_transport(b, e->grid_transport, e->stream);
-  launch_particle(e->model->id, b, e->grid_particle, e->stream);
+  launch_particle(e->model->id, b, e->plaunch, e->stream);
   launch_bookkeep(b, e->stream);
   launch_removal(b, nv, e->grid_particle, e->stream);
   launch_merge(b, nv, e->grid_particle, e->stream);
@@ -823,7 +868,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
     HIP_TRY(hipEventRecord(x[0], e->stream));
     launch_transport(b, e->grid_transport, e->stream);
     HIP_TRY(hipEventRecord(x[1], e->stream));
-    launch_particle(e->model->id, b, e->grid_particle, e->stream);
+    launch_particle(e->model->id, b, e->plaunch, e->stream);
     HIP_TRY(hipEventRecord(x[2], e->stream));
     launch_bookkeep(b, e->stream);
     HIP_TRY(hipEventRecord(x[3], e->stream));

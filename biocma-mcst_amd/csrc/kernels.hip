@@ -30,6 +30,20 @@ __device__ __forceinline__ GLOBAL_AS T* gp(T* p) {
   return (GLOBAL_AS T*)p;
 }
 
+// A value every lane of the wave holds identically, moved to scalar registers.
+template <class T>
+__device__ __forceinline__ T uniform(T v) {
+  static_assert(sizeof(T) % 4 == 0, "uniform() works on whole dwords");
+  union {
+    T t;
+    uint32_t w[sizeof(T) / 4];
+  } u;
+  u.t = v;
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) u.w[i] = __builtin_amdgcn_readfirstlane(u.w[i]);
+  return u.t;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane) { return lane ? (~0ull >> (64u - lane)) : 0ull; }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -141,102 +155,180 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
 }
 
 // ------------------------------------------------------------------ particles
-template <class M>
-__global__ void __launch_bounds__(BLOCK) k_particle(BatchDev b) {
+// Counters of the tiles a block processed for one run, flushed with one
+// atomic each when the block moves on to another run (or finishes).
+struct TileCounts {
+  uint32_t exits = 0, deaths = 0, moves = 0, overflow = 0, processed = 0;
+};
+
+__device__ __forceinline__ void flush_counts(RunState* rs, TileCounts& c) {
+  if (c.exits) atomicAdd(&rs->step_exits, (unsigned long long)c.exits);
+  if (c.deaths) atomicAdd(&rs->step_deaths, (unsigned long long)c.deaths);
+  if (c.moves) atomicAdd(&rs->step_moves, (unsigned long long)c.moves);
+  if (c.overflow) atomicAdd(&rs->step_overflow, (unsigned long long)c.overflow);
+  if (c.processed) atomicAdd(&rs->step_processed, (unsigned long long)c.processed);
+  c = TileCounts{};
+}
+
+// The particle pass.  BLK threads process a TILE = 1024-slot tile, PPT = TILE/BLK
+// slots each.  Each block takes a contiguous range of tiles, so it meets few
+// runs.  With PRIV the block accumulates its contributions in LDS (ds_add_f64)
+// and flushes them to the run's S with coalesced atomics when it moves to
+// another run; without PRIV every particle adds to S directly.
+template <class M, int BLK, bool PRIV>
+__global__ void __launch_bounds__(BLK)
+    k_particle(BatchDev b, const RunDev* __restrict__ runs, const MapDev* __restrict__ maps,
+               const uint32_t* __restrict__ tile_off) {
   constexpr int NV = M::NV, NS = M::NS, REC = rec_stride(M::NS);
-  __shared__ uint32_t s_div[PPT][WAVES];
-  __shared__ uint32_t s_red[4][WAVES];
+  constexpr int NP = M::NP > 0 ? M::NP : 1;
+  constexpr uint32_t NPT = TILE / BLK;   // slots per thread
+  constexpr uint32_t NW = BLK / 64;      // waves per block
+  extern __shared__ double s_acc[];      // [n_s][n_c] of the current run (PRIV)
+  __shared__ uint32_t s_div[NPT][NW];
+  __shared__ uint32_t s_red[4][NW];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t T = b.tile_off[b.R];
   const uint32_t stages = b.stages;
   const uint64_t P = b.P;
+  const uint64_t ST = b.ST;
+  const uint32_t stride = b.stage_stride;
+  const uint32_t R = b.R;
+  const uint32_t T = tile_off[R];
+  const uint32_t chunk = (T + gridDim.x - 1) / gridDim.x;
+  const uint32_t t_begin = blockIdx.x * chunk;
+  const uint32_t t_end = (t_begin + chunk < T) ? t_begin + chunk : T;
+  TileCounts cnt_acc;
+  uint32_t acc_run = 0xFFFFFFFFu;
+  GLOBAL_AS double* acc_S = nullptr;
+  uint32_t acc_len = 0;
 
-  for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
-    const uint32_t r = find_run(b.tile_off, b.R, t);
-    const RunDev& rd = b.runs[r];
+  for (uint32_t t = t_begin; t < t_end; ++t) {
+    // ---- tile context, loaded once into scalar registers
+    const uint32_t r = find_run(tile_off, R, t);
     RunState* rs = b.rs + r;
-    const uint32_t n = rs->n;
-    const uint32_t step = rs->step;
-    const uint32_t budget = rs->budget;
-    const uint32_t t_local = t - b.tile_off[r];
+    const RunDev& rd = runs[r];
+    if (r != acc_run) {
+      if (PRIV && acc_run != 0xFFFFFFFFu) {  // flush the previous run's sums
+        __syncthreads();
+        for (uint32_t i = tid; i < acc_len; i += BLK) {
+          const double v = s_acc[i];
+          if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
+        }
+      }
+      if (tid == 0) {
+        if (acc_run != 0xFFFFFFFFu) flush_counts(b.rs + acc_run, cnt_acc);
+      }
+      acc_run = r;
+      if (PRIV) {
+        acc_S = gp(rd.S);
+        acc_len = rd.n_s * rd.n_c;
+        __syncthreads();
+        for (uint32_t i = tid; i < acc_len; i += BLK) s_acc[i] = 0.0;
+        __syncthreads();
+      }
+    }
+    const uint32_t n = uniform(rs->n);
+    const uint32_t step = uniform(rs->step);
+    const uint32_t budget = uniform(rs->budget);
+    const uint32_t t_local = t - tile_off[r];
+    const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
     const uint32_t slot0 = t_local * TILE;
     const uint32_t cnt = (n - slot0 < TILE) ? n - slot0 : TILE;
     const uint64_t base = rd.p_off + slot0;
-    const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
     const double dt = rd.dt;
     const double w = rd.weight;
     const uint32_t n_c = rd.n_c;
-    double p[M::NP > 0 ? M::NP : 1];
+    const uint64_t seed = rd.seed;
+    const GLOBAL_AS double* __restrict__ rec = gp(rd.rec);
+    GLOBAL_AS double* __restrict__ S = gp(rd.S);
+    double p[NP];
 #pragma unroll
-    for (int q = 0; q < M::NP; ++q) p[q] = rd.params[q];
+    for (int q = 0; q < NP; ++q) p[q] = rd.params[q];
     const bool do_model = (stages & STAGE_MODEL) != 0;
     const bool do_contrib = (stages & STAGE_CONTRIB) != 0;
     const bool do_move = rd.enable_move && (stages & STAGE_MOVE);
     const bool do_leave = rd.enable_leave && (stages & STAGE_LEAVE);
+    // uniform bases of the tile's slots; lanes address them with 32-bit offsets
+    GLOBAL_AS uint32_t* __restrict__ pos_b = gp(b.pos) + base;
+    GLOBAL_AS uint8_t* __restrict__ st_b = gp(b.status) + base;
+    GLOBAL_AS double* __restrict__ ah_b = gp(b.age_hyd) + base;
+    GLOBAL_AS double* __restrict__ ad_b = gp(b.age_div) + base;
+    GLOBAL_AS double* __restrict__ x_b[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) x_b[v] = gp(b.state) + (uint64_t)v * P + base;
 
-    uint8_t st[PPT];
-    uint32_t pos0[PPT];
-    double x[PPT][NV];
-    double ad[PPT], ah[PPT];
-    bool idle[PPT], want[PPT];
+    // ---- one burst of loads: every slot index is clamped into the tile, so
+    // no load waits on another
+    uint32_t li[NPT];
+    bool valid[NPT];
+    uint8_t st[NPT];
+    uint32_t pos0[NPT];
+    double x[NPT][NV];
+    double ad[NPT], ah[NPT];
+    double rr[NPT][NS + 2];
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+      const uint32_t i = k * BLK + tid;
+      valid[k] = i < cnt;
+      li[k] = valid[k] ? i : 0u;
+      st[k] = st_b[li[k]];
+      pos0[k] = pos_b[li[k]];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[k][v] = x_b[v][li[k]];
+      ad[k] = ad_b[li[k]];
+      ah[k] = do_leave ? ah_b[li[k]] : 0.0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+      const GLOBAL_AS double* rp = rec + (size_t)pos0[k] * REC;
+#pragma unroll
+      for (int q = 0; q < NS + 2; ++q) rr[k][q] = rp[q];
+    }
+
+    // ---- per-particle work
+    bool idle[NPT], want[NPT];
     uint32_t n_exit = 0, n_dead = 0, n_move = 0, n_proc = 0;
-
 #pragma unroll
-    for (uint32_t k = 0; k < PPT; ++k) {
-      const uint32_t i = k * BLOCK + tid;
-      const bool valid = i < cnt;
-      st[k] = valid ? b.status[base + i] : (uint8_t)EXIT;
-      pos0[k] = valid ? b.pos[base + i] : 0u;
-      idle[k] = st[k] == IDLE;
-      n_proc += idle[k] ? 1u : 0u;
+    for (uint32_t k = 0; k < NPT; ++k) {
+      idle[k] = valid[k] && st[k] == IDLE;
       want[k] = false;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < PPT; ++k) {
-      const uint64_t g = base + k * BLOCK + tid;
-      if (idle[k]) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) x[k][v] = b.state[(uint64_t)v * P + g];
-        ad[k] = b.age_div[g];
-        ah[k] = do_leave ? b.age_hyd[g] : 0.0;
-      }
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < PPT; ++k) {
       if (!idle[k]) continue;
-      const uint32_t i = k * BLOCK + tid;
-      const uint64_t g = base + i;
-      const uint32_t slot = slot0 + i;
-      const GLOBAL_AS double* rp = gp(rd.rec) + (size_t)pos0[k] * REC;
-      double rr[NS + 2];
-#pragma unroll
-      for (int q = 0; q < NS + 2; ++q) rr[q] = rp[q];
+      ++n_proc;
+      const uint32_t slot = slot0 + k * BLK + tid;
       double contrib[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) contrib[s] = 0.0;
       int code = IDLE;
       if (do_model) {
         ad[k] += dt;
-        code = M::update(p, dt, x[k], rr, contrib);
+        code = M::update(p, dt, x[k], rr[k], contrib);
       }
       if (code == DEAD) {  // a dead cell stops contributing, moving and leaving
-        b.status[g] = DEAD;
+        st_b[li[k]] = DEAD;
         idle[k] = false;
         ++n_dead;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * P + g] = x[k][v];
-        b.age_div[g] = ad[k];
+        for (int v = 0; v < NV; ++v) x_b[v][li[k]] = x[k][v];
+        ad_b[li[k]] = ad[k];
         continue;
       }
       want[k] = code == DIVISION;
       if (do_contrib) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) unsafeAtomicAdd((double*)(gp(rd.S) + (size_t)s * n_c + pos0[k]), w * contrib[s]);
+        for (int s = 0; s < NS; ++s) {
+          const double c = w * contrib[s];
+          if (PRIV)
+            atomicAdd(s_acc + s * n_c + pos0[k], c);
+          else
+            unsafeAtomicAdd((double*)(S + (size_t)s * n_c + pos0[k]), c);
+        }
       }
-      const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
+      const U4 u = draw4(seed, slot, step, TAG_TRANSPORT, 0u);
       uint32_t pos = pos0[k];
       bool moved = false;
-      if (do_move && rr[NS] > u01(u.x)) {  // dt*Q/V > u  (probability_leaving.hpp:313-325)
+      if (do_move && rr[k][NS] > u01(u.x)) {  // dt*Q/V > u  (probability_leaving.hpp:313-325)
         const double u2 = u01(u.y);
         const uint32_t m = mp.m;
         const GLOBAL_AS double* cp = gp(mp.cdf) + (size_t)pos * m;
@@ -251,29 +343,30 @@ __global__ void __launch_bounds__(BLOCK) k_particle(BatchDev b) {
         pos = gp(mp.nbr)[(size_t)pos * m + left];
         moved = true;
         ++n_move;
-        b.pos[g] = pos;
+        pos_b[li[k]] = pos;
       }
       if (do_leave) {
         ah[k] += dt;
-        const double q = moved ? gp(rd.rec)[(size_t)pos * REC + NS + 1] : rr[NS + 1];
+        const double q = moved ? rec[(size_t)pos * REC + NS + 1] : rr[k][NS + 1];
         if (q < 1.0 && u01(u.z) > q) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
           if (rd.probe_exit) {
             const unsigned long long e = atomicAdd(&rs->probe_exit_n, 1ull);
             if (e < rd.probe_cap) rd.probe_exit[e] = ah[k];
           }
           ah[k] = 0.0;
-          b.status[g] = EXIT;
+          st_b[li[k]] = EXIT;
           ++n_exit;
         }
-        b.age_hyd[g] = ah[k];
+        ah_b[li[k]] = ah[k];
       }
     }
 
-    // rank the division requests of the tile in slot order
+    // ---- rank the division requests of the tile in slot order
+    uint64_t mk[NPT];
 #pragma unroll
-    for (uint32_t k = 0; k < PPT; ++k) {
-      const uint64_t mk = __ballot(want[k]);
-      if (lane == 0) s_div[k][wave] = (uint32_t)__popcll(mk);
+    for (uint32_t k = 0; k < NPT; ++k) {
+      mk[k] = __ballot(want[k]);
+      if (lane == 0) s_div[k][wave] = (uint32_t)__popcll(mk[k]);
     }
     n_exit = wave_sum(n_exit);
     n_dead = wave_sum(n_dead);
@@ -288,57 +381,60 @@ __global__ void __launch_bounds__(BLOCK) k_particle(BatchDev b) {
     __syncthreads();
     uint32_t total = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < PPT; ++k)
+    for (uint32_t k = 0; k < NPT; ++k)
 #pragma unroll
-      for (uint32_t wv = 0; wv < WAVES; ++wv) total += s_div[k][wv];
+      for (uint32_t wv = 0; wv < NW; ++wv) total += s_div[k][wv];
     const uint32_t births = total < budget ? total : budget;
 
     uint32_t before = 0;  // requests in earlier k-rows
 #pragma unroll
-    for (uint32_t k = 0; k < PPT; ++k) {
-      const uint32_t i = k * BLOCK + tid;
-      const uint64_t g = base + i;
-      const uint64_t mk = __ballot(want[k]);
-      uint32_t rank = before + (uint32_t)__popcll(mk & lanemask_lt(lane));
+    for (uint32_t k = 0; k < NPT; ++k) {
+      uint32_t rank = before + (uint32_t)__popcll(mk[k] & lanemask_lt(lane));
 #pragma unroll
-      for (uint32_t wv = 0; wv < WAVES; ++wv) rank += (wv < wave) ? s_div[k][wv] : 0u;
-#pragma unroll
-      for (uint32_t wv = 0; wv < WAVES; ++wv) before += s_div[k][wv];
+      for (uint32_t wv = 0; wv < NW; ++wv) {
+        rank += (wv < wave) ? s_div[k][wv] : 0u;
+        before += s_div[k][wv];
+      }
       if (want[k] && rank < budget) {
         if (rd.probe_div) {
           const unsigned long long e = atomicAdd(&rs->probe_div_n, 1ull);
           if (e < rd.probe_cap) rd.probe_div[e] = ad[k];
         }
         double xd[NV];
-        M::division(p, x[k], xd, RngKey{rd.seed, slot0 + i, step});
-        const uint64_t sidx = (uint64_t)t * b.stage_stride + rank;
+        M::division(p, x[k], xd, RngKey{seed, slot0 + k * BLK + tid, step});
+        const uint64_t sidx = (uint64_t)t * stride + rank;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) b.st_state[(uint64_t)v * b.ST + sidx] = xd[v];
-        b.st_pos[sidx] = pos0[k];
+        for (int v = 0; v < NV; ++v) gp(b.st_state)[(uint64_t)v * ST + sidx] = xd[v];
+        gp(b.st_pos)[sidx] = pos0[k];
         ad[k] = 0.0;
       }
       if (idle[k]) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * P + g] = x[k][v];
-        if (do_model) b.age_div[g] = ad[k];
+        for (int v = 0; v < NV; ++v) x_b[v][li[k]] = x[k][v];
+        if (do_model) ad_b[li[k]] = ad[k];
       }
     }
     if (tid == 0) {
       b.tile_births[t] = births;
-      uint32_t e = 0, d = 0, mv = 0, pr = 0;
-      for (uint32_t wv = 0; wv < WAVES; ++wv) {
-        e += s_red[0][wv];
-        d += s_red[1][wv];
-        mv += s_red[2][wv];
-        pr += s_red[3][wv];
+      for (uint32_t wv = 0; wv < NW; ++wv) {
+        cnt_acc.exits += s_red[0][wv];
+        cnt_acc.deaths += s_red[1][wv];
+        cnt_acc.moves += s_red[2][wv];
+        cnt_acc.processed += s_red[3][wv];
       }
-      if (pr) atomicAdd(&rs->step_processed, (unsigned long long)pr);
-      if (e) atomicAdd(&rs->step_exits, (unsigned long long)e);
-      if (d) atomicAdd(&rs->step_deaths, (unsigned long long)d);
-      if (mv) atomicAdd(&rs->step_moves, (unsigned long long)mv);
-      if (total > births) atomicAdd(&rs->step_overflow, (unsigned long long)(total - births));
+      cnt_acc.overflow += total - births;
     }
     __syncthreads();
+  }
+  if (acc_run != 0xFFFFFFFFu) {
+    if (PRIV) {
+      __syncthreads();
+      for (uint32_t i = tid; i < acc_len; i += BLK) {
+        const double v = s_acc[i];
+        if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
+      }
+    }
+    if (tid == 0) flush_counts(b.rs + acc_run, cnt_acc);
   }
 }
 
@@ -617,29 +713,52 @@ __global__ void __launch_bounds__(256) k_repartition(BatchDev b, uint32_t r, uns
 }
 
 // ------------------------------------------------------------------ launchers
-template <class M>
-static void launch_particle_t(const BatchDev& b, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_particle<M>, dim3(grid), dim3(BLOCK), 0, s, b);
+template <class M, int BLK, bool PRIV>
+static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_particle<M, BLK, PRIV>), dim3(grid), dim3(BLK), lds, s, b, b.runs, b.maps,
+                     (const uint32_t*)b.tile_off);
 }
 
-void launch_particle(int model_id, const BatchDev& b, int grid, hipStream_t s) {
+template <class M>
+static const void* kernel_ptr_t(int blk, bool priv) {
+  if (priv) {
+    if (blk == 256) return (const void*)&k_particle<M, 256, true>;
+    if (blk == 512) return (const void*)&k_particle<M, 512, true>;
+    return (const void*)&k_particle<M, 1024, true>;
+  }
+  return (const void*)&k_particle<M, 256, false>;
+}
+
+template <class M>
+static void launch_t(const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
+  if (!L.priv)
+    launch_particle_t<M, 256, false>(b, L.grid, 0, s);
+  else if (L.block == 256)
+    launch_particle_t<M, 256, true>(b, L.grid, L.lds, s);
+  else if (L.block == 512)
+    launch_particle_t<M, 512, true>(b, L.grid, L.lds, s);
+  else
+    launch_particle_t<M, 1024, true>(b, L.grid, L.lds, s);
+}
+
+void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
   switch (model_id) {
-    case MonodMass::ID: launch_particle_t<MonodMass>(b, grid, s); break;
-    case DualMonod::ID: launch_particle_t<DualMonod>(b, grid, s); break;
-    case Monod::ID: launch_particle_t<Monod>(b, grid, s); break;
-    case FixedLength::ID: launch_particle_t<FixedLength>(b, grid, s); break;
-    case SimpleAcetate::ID: launch_particle_t<SimpleAcetate>(b, grid, s); break;
+    case MonodMass::ID: launch_t<MonodMass>(b, L, s); break;
+    case DualMonod::ID: launch_t<DualMonod>(b, L, s); break;
+    case Monod::ID: launch_t<Monod>(b, L, s); break;
+    case FixedLength::ID: launch_t<FixedLength>(b, L, s); break;
+    case SimpleAcetate::ID: launch_t<SimpleAcetate>(b, L, s); break;
     default: break;
   }
 }
 
-const void* particle_kernel_ptr(int model_id) {
+const void* particle_kernel_ptr(int model_id, int block, bool priv) {
   switch (model_id) {
-    case MonodMass::ID: return (const void*)&k_particle<MonodMass>;
-    case DualMonod::ID: return (const void*)&k_particle<DualMonod>;
-    case Monod::ID: return (const void*)&k_particle<Monod>;
-    case FixedLength::ID: return (const void*)&k_particle<FixedLength>;
-    case SimpleAcetate::ID: return (const void*)&k_particle<SimpleAcetate>;
+    case MonodMass::ID: return kernel_ptr_t<MonodMass>(block, priv);
+    case DualMonod::ID: return kernel_ptr_t<DualMonod>(block, priv);
+    case Monod::ID: return kernel_ptr_t<Monod>(block, priv);
+    case FixedLength::ID: return kernel_ptr_t<FixedLength>(block, priv);
+    case SimpleAcetate::ID: return kernel_ptr_t<SimpleAcetate>(block, priv);
     default: return nullptr;
   }
 }
