@@ -3,7 +3,8 @@
 
 Workload: 5000-compartment random geometric network (mean degree 6) with 20
 cyclic flowmaps swapped every 100 steps, 2 liquid species, 1e7 particles of
-the dual-substrate Monod model (3 FP64 states + 2 FP64 ages), dt = 0.01 s.
+the dual-substrate Monod model (3 FP64 states + the FP64 division age; the
+network has no outlet, so no hydraulic age), dt = 0.01 s.
 A step is one pass of the whole pipeline (transport, fused particle pass,
 bookkeeping, removal, merge) over all particles, with every input resident
 in HBM.  particle-steps = sum over steps of the Idle particles processed.
@@ -62,6 +63,24 @@ def cpu_baseline(case_fn, seconds=8.0, max_steps=400, n_particles=1_000_000):
                       f"{n_particles:.0e} particles, {done} steps, {el:.1f} s"}
 
 
+def copy_calibration(mib, reps=5):
+    """Plain device-to-device copies of a known size: what FETCH_SIZE and
+    WRITE_SIZE read for them calibrates the particle kernel's counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes_per_dispatch": n, "dispatches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +90,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=0, help="replay a captured graph of this many steps")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of this many MiB (PMC counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "see DESIGN.md for how the step shards across GPUs", file=sys.stderr)
@@ -89,7 +113,11 @@ def main():
     case.apply(run)
     eng.commit()
     _, nv, ns, _ = mcst.model_info(case.model)
-    S = nv + 2  # FP64 fields per particle: model state + two ages
+    # FP64 fields a step reads and writes per particle: the model state, the
+    # division age, and the hydraulic age only when a leaving flow exists
+    # (the leave pass is not run otherwise, kernels.hpp:56)
+    leave = any(f[4] is not None for f in case.feeds)
+    S = nv + 1 + (1 if leave else 0)
     b_ps = 2 * (8 * S + 4 + 1)
     import numpy as np
 
@@ -164,6 +192,7 @@ def main():
             "compartments": case.n_c,
             "species": ns,
             "fp64_fields_per_particle": S,
+            "hydraulic_age_tracked": leave,
             "B_ps": b_ps,
             "B_fix": case.n_c * (8 * ns * 4 + 16 + 12 * m_width) + 12 * nnz,
             "graph_steps": a.graph,

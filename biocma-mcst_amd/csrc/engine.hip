@@ -477,6 +477,9 @@ int commit(mcst_engine* e) {
       (void)hipGetLastError();
       continue;
     }
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, fn) == hipSuccess && fa.localSizeBytes > 0 && !forced_block)
+      per_cu = per_cu / 2;  // register spills: prefer a variant that fits
     const int waves = per_cu * blk / 64;
     if (per_cu > 0 && waves > best_waves) {
       best_waves = waves;

@@ -44,6 +44,17 @@ __device__ __forceinline__ T uniform(T v) {
   return u.t;
 }
 
+// Streaming accesses to the particle arrays: non-temporal, so the particle
+// stream does not evict the flowmap and compartment tables from L2.
+template <class T>
+__device__ __forceinline__ T ld_stream(const GLOBAL_AS T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_stream(GLOBAL_AS T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane) { return lane ? (~0ull >> (64u - lane)) : 0ull; }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -170,11 +181,31 @@ __device__ __forceinline__ void flush_counts(RunState* rs, TileCounts& c) {
   c = TileCounts{};
 }
 
-// The particle pass.  BLK threads process a TILE = 1024-slot tile, PPT = TILE/BLK
+// First column of a cumulative row with cp[c] >= u, clamped to m-1: for a
+// non-decreasing row this is #{c < m-1 : cp[c] < u}, the result of the
+// reference's bisection (move_kernel.hpp:89-102).  Counting over chunks of 8
+// independent loads takes one memory round trip per 8 columns instead of one
+// per bisection step.
+__device__ __forceinline__ uint32_t cdf_column(const GLOBAL_AS double* cp, uint32_t m, double u) {
+  uint32_t left = 0;
+  const int last = (int)m - 1;
+  for (int c0 = 0; c0 < last; c0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (c0 + j < last) ? cp[c0 + j] : 2.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) left += (u > v[j]) ? 1u : 0u;
+  }
+  return left;
+}
+
+// The particle pass.  BLK threads process a TILE = 1024-slot tile, NPT = TILE/BLK
 // slots each.  Each block takes a contiguous range of tiles, so it meets few
-// runs.  With PRIV the block accumulates its contributions in LDS (ds_add_f64)
-// and flushes them to the run's S with coalesced atomics when it moves to
-// another run; without PRIV every particle adds to S directly.
+// runs, and runs it as a two-stage pipeline: the slot loads of tile t+1 are
+// in flight while tile t is computed.  With PRIV the block accumulates its
+// contributions in LDS (ds_add_f64) and flushes them to the run's S with
+// coalesced atomics when it moves to another run; without PRIV every
+// particle adds to S directly.
 template <class M, int BLK, bool PRIV>
 __global__ void __launch_bounds__(BLK)
     k_particle(BatchDev b, const RunDev* __restrict__ runs, const MapDev* __restrict__ maps,
@@ -184,8 +215,8 @@ __global__ void __launch_bounds__(BLK)
   constexpr uint32_t NPT = TILE / BLK;   // slots per thread
   constexpr uint32_t NW = BLK / 64;      // waves per block
   extern __shared__ double s_acc[];      // [n_s][n_c] of the current run (PRIV)
-  __shared__ uint32_t s_div[NPT][NW];
-  __shared__ uint32_t s_red[4][NW];
+  __shared__ uint32_t s_div[2][NPT][NW];  // double-buffered by tile parity:
+  __shared__ uint32_t s_red[2][4][NW];    // one barrier per tile suffices
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t stages = b.stages;
   const uint64_t P = b.P;
@@ -196,27 +227,83 @@ __global__ void __launch_bounds__(BLK)
   const uint32_t chunk = (T + gridDim.x - 1) / gridDim.x;
   const uint32_t t_begin = blockIdx.x * chunk;
   const uint32_t t_end = (t_begin + chunk < T) ? t_begin + chunk : T;
+  if (t_begin >= t_end) return;
+  const bool do_model = (stages & STAGE_MODEL) != 0;
+  const bool do_contrib = (stages & STAGE_CONTRIB) != 0;
+
+  // run whose step/count/budget are cached in scalar registers
+  uint32_t c_run = 0xFFFFFFFFu, c_n = 0, c_step = 0, c_budget = 0;
+  auto run_state = [&](uint32_t r) {
+    if (r != c_run) {
+      const RunState* rs = b.rs + r;
+      c_n = uniform(rs->n);
+      c_step = uniform(rs->step);
+      c_budget = uniform(rs->budget);
+      c_run = r;
+    }
+  };
+
+  // slots of one tile, as loaded
+  struct Slots {
+    uint8_t st[NPT];
+    uint32_t pos[NPT];
+    double x[NPT][NV];
+    double ad[NPT], ah[NPT];
+  };
+  struct TileRef {
+    uint32_t r, slot0, cnt;
+    uint64_t base;
+  };
+  auto tile_ref = [&](uint32_t t) {
+    TileRef q;
+    q.r = find_run(tile_off, R, t);
+    run_state(q.r);
+    q.slot0 = (t - tile_off[q.r]) * TILE;
+    q.cnt = (c_n - q.slot0 < TILE) ? c_n - q.slot0 : TILE;
+    q.base = runs[q.r].p_off + q.slot0;
+    return q;
+  };
+  auto load = [&](const TileRef& q, Slots& a) {
+    const bool leave = runs[q.r].enable_leave && (stages & STAGE_LEAVE);
+    const GLOBAL_AS uint8_t* st_b = gp(b.status) + q.base;
+    const GLOBAL_AS uint32_t* pos_b = gp(b.pos) + q.base;
+    const GLOBAL_AS double* ad_b = gp(b.age_div) + q.base;
+    const GLOBAL_AS double* ah_b = gp(b.age_hyd) + q.base;
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+      const uint32_t i = k * BLK + tid;
+      const uint32_t li = i < q.cnt ? i : 0u;  // clamped: no load depends on another
+      const uint8_t stv = ld_stream(st_b + li);
+      a.st[k] = i < q.cnt ? stv : (uint8_t)EXIT;
+      a.pos[k] = ld_stream(pos_b + li);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a.x[k][v] = ld_stream(gp(b.state) + (uint64_t)v * P + q.base + li);
+      a.ad[k] = ld_stream(ad_b + li);
+      a.ah[k] = leave ? ld_stream(ah_b + li) : 0.0;
+    }
+  };
+
   TileCounts cnt_acc;
   uint32_t acc_run = 0xFFFFFFFFu;
   GLOBAL_AS double* acc_S = nullptr;
   uint32_t acc_len = 0;
 
+  TileRef cur = tile_ref(t_begin);
+  Slots A;
+  load(cur, A);
   for (uint32_t t = t_begin; t < t_end; ++t) {
-    // ---- tile context, loaded once into scalar registers
-    const uint32_t r = find_run(tile_off, R, t);
-    RunState* rs = b.rs + r;
+    const uint32_t par = t & 1u;
+    const uint32_t r = cur.r;
     const RunDev& rd = runs[r];
-    if (r != acc_run) {
-      if (PRIV && acc_run != 0xFFFFFFFFu) {  // flush the previous run's sums
+    if (r != acc_run) {  // block moves to another run: flush the last one's sums
+      if (PRIV && acc_run != 0xFFFFFFFFu) {
         __syncthreads();
         for (uint32_t i = tid; i < acc_len; i += BLK) {
           const double v = s_acc[i];
           if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
         }
       }
-      if (tid == 0) {
-        if (acc_run != 0xFFFFFFFFu) flush_counts(b.rs + acc_run, cnt_acc);
-      }
+      if (tid == 0 && acc_run != 0xFFFFFFFFu) flush_counts(b.rs + acc_run, cnt_acc);
       acc_run = r;
       if (PRIV) {
         acc_S = gp(rd.S);
@@ -226,92 +313,70 @@ __global__ void __launch_bounds__(BLK)
         __syncthreads();
       }
     }
-    const uint32_t n = uniform(rs->n);
-    const uint32_t step = uniform(rs->step);
-    const uint32_t budget = uniform(rs->budget);
-    const uint32_t t_local = t - tile_off[r];
+    run_state(r);
+    const uint32_t step = c_step;
+    const uint32_t budget = c_budget;
     const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
-    const uint32_t slot0 = t_local * TILE;
-    const uint32_t cnt = (n - slot0 < TILE) ? n - slot0 : TILE;
-    const uint64_t base = rd.p_off + slot0;
+    const GLOBAL_AS double* __restrict__ rec = gp(rd.rec);
+
+    // ---- compartment records of the current tile (needs its positions)
+    double rr[NPT][NS + 2];
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+      const GLOBAL_AS double* rp = rec + (size_t)A.pos[k] * REC;
+#pragma unroll
+      for (int q = 0; q < NS + 2; ++q) rr[k][q] = rp[q];
+    }
+    // ---- prefetch the next tile's slots
+    Slots B;
+    TileRef nxt = cur;
+    if (t + 1 < t_end) {
+      nxt = tile_ref(t + 1);
+      load(nxt, B);
+      run_state(r);
+    }
+
+    // ---- per-particle work on the current tile
     const double dt = rd.dt;
     const double w = rd.weight;
     const uint32_t n_c = rd.n_c;
     const uint64_t seed = rd.seed;
-    const GLOBAL_AS double* __restrict__ rec = gp(rd.rec);
     GLOBAL_AS double* __restrict__ S = gp(rd.S);
     double p[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) p[q] = rd.params[q];
-    const bool do_model = (stages & STAGE_MODEL) != 0;
-    const bool do_contrib = (stages & STAGE_CONTRIB) != 0;
     const bool do_move = rd.enable_move && (stages & STAGE_MOVE);
     const bool do_leave = rd.enable_leave && (stages & STAGE_LEAVE);
-    // uniform bases of the tile's slots; lanes address them with 32-bit offsets
+    const uint64_t base = cur.base;
+    const uint32_t slot0 = cur.slot0;
     GLOBAL_AS uint32_t* __restrict__ pos_b = gp(b.pos) + base;
     GLOBAL_AS uint8_t* __restrict__ st_b = gp(b.status) + base;
     GLOBAL_AS double* __restrict__ ah_b = gp(b.age_hyd) + base;
     GLOBAL_AS double* __restrict__ ad_b = gp(b.age_div) + base;
-    GLOBAL_AS double* __restrict__ x_b[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) x_b[v] = gp(b.state) + (uint64_t)v * P + base;
-
-    // ---- one burst of loads: every slot index is clamped into the tile, so
-    // no load waits on another
-    uint32_t li[NPT];
-    bool valid[NPT];
-    uint8_t st[NPT];
-    uint32_t pos0[NPT];
-    double x[NPT][NV];
-    double ad[NPT], ah[NPT];
-    double rr[NPT][NS + 2];
-#pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      const uint32_t i = k * BLK + tid;
-      valid[k] = i < cnt;
-      li[k] = valid[k] ? i : 0u;
-      st[k] = st_b[li[k]];
-      pos0[k] = pos_b[li[k]];
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) x[k][v] = x_b[v][li[k]];
-      ad[k] = ad_b[li[k]];
-      ah[k] = do_leave ? ah_b[li[k]] : 0.0;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      const GLOBAL_AS double* rp = rec + (size_t)pos0[k] * REC;
-#pragma unroll
-      for (int q = 0; q < NS + 2; ++q) rr[k][q] = rp[q];
-    }
-
-    // ---- per-particle work
     bool idle[NPT], want[NPT];
     uint32_t n_exit = 0, n_dead = 0, n_move = 0, n_proc = 0;
 #pragma unroll
     for (uint32_t k = 0; k < NPT; ++k) {
-      idle[k] = valid[k] && st[k] == IDLE;
+      idle[k] = A.st[k] == IDLE;
       want[k] = false;
       if (!idle[k]) continue;
       ++n_proc;
-      const uint32_t slot = slot0 + k * BLK + tid;
+      const uint32_t i = k * BLK + tid;
       double contrib[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) contrib[s] = 0.0;
       int code = IDLE;
       if (do_model) {
-        ad[k] += dt;
-        code = M::update(p, dt, x[k], rr[k], contrib);
+        A.ad[k] += dt;
+        code = M::update(p, dt, A.x[k], rr[k], contrib);
       }
       if (code == DEAD) {  // a dead cell stops contributing, moving and leaving
-        st_b[li[k]] = DEAD;
+        st_b[i] = DEAD;
         idle[k] = false;
         ++n_dead;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) x_b[v][li[k]] = x[k][v];
-        ad_b[li[k]] = ad[k];
+        for (int v = 0; v < NV; ++v) (gp(b.state) + (uint64_t)v * P + base)[i] = A.x[k][v];
+        ad_b[i] = A.ad[k];
         continue;
       }
       want[k] = code == DIVISION;
@@ -320,44 +385,40 @@ __global__ void __launch_bounds__(BLK)
         for (int s = 0; s < NS; ++s) {
           const double c = w * contrib[s];
           if (PRIV)
-            atomicAdd(s_acc + s * n_c + pos0[k], c);
+            atomicAdd(s_acc + s * n_c + A.pos[k], c);
           else
-            unsafeAtomicAdd((double*)(S + (size_t)s * n_c + pos0[k]), c);
+            unsafeAtomicAdd((double*)(S + (size_t)s * n_c + A.pos[k]), c);
         }
       }
-      const U4 u = draw4(seed, slot, step, TAG_TRANSPORT, 0u);
-      uint32_t pos = pos0[k];
+      const U4 u = draw4(seed, slot0 + i, step, TAG_TRANSPORT, 0u);
+      uint32_t pos = A.pos[k];
       bool moved = false;
       if (do_move && rr[k][NS] > u01(u.x)) {  // dt*Q/V > u  (probability_leaving.hpp:313-325)
-        const double u2 = u01(u.y);
         const uint32_t m = mp.m;
-        const GLOBAL_AS double* cp = gp(mp.cdf) + (size_t)pos * m;
-        int left = 0, right = (int)m - 1;
-        while (left < right) {  // first column with cp >= u  (move_kernel.hpp:89-98)
-          const int mid = (left + right) >> 1;
-          if (u2 > cp[mid])
-            left = mid + 1;
-          else
-            right = mid;
-        }
-        pos = gp(mp.nbr)[(size_t)pos * m + left];
+        const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
+        pos = gp(mp.nbr)[(size_t)pos * m + col];
         moved = true;
         ++n_move;
-        pos_b[li[k]] = pos;
+        pos_b[i] = pos;
       }
       if (do_leave) {
-        ah[k] += dt;
+        A.ah[k] += dt;
         const double q = moved ? rec[(size_t)pos * REC + NS + 1] : rr[k][NS + 1];
         if (q < 1.0 && u01(u.z) > q) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
           if (rd.probe_exit) {
-            const unsigned long long e = atomicAdd(&rs->probe_exit_n, 1ull);
-            if (e < rd.probe_cap) rd.probe_exit[e] = ah[k];
+            const unsigned long long e = atomicAdd(&b.rs[r].probe_exit_n, 1ull);
+            if (e < rd.probe_cap) rd.probe_exit[e] = A.ah[k];
           }
-          ah[k] = 0.0;
-          st_b[li[k]] = EXIT;
+          A.ah[k] = 0.0;
+          st_b[i] = EXIT;
           ++n_exit;
         }
-        ah_b[li[k]] = ah[k];
+        st_stream(ah_b + i, A.ah[k]);
+      }
+      if (!want[k]) {  // final state now; dividers are written after ranking
+#pragma unroll
+        for (int v = 0; v < NV; ++v) st_stream(gp(b.state) + (uint64_t)v * P + base + i, A.x[k][v]);
+        if (do_model) st_stream(ad_b + i, A.ad[k]);
       }
     }
 
@@ -366,76 +427,84 @@ __global__ void __launch_bounds__(BLK)
 #pragma unroll
     for (uint32_t k = 0; k < NPT; ++k) {
       mk[k] = __ballot(want[k]);
-      if (lane == 0) s_div[k][wave] = (uint32_t)__popcll(mk[k]);
+      if (lane == 0) s_div[par][k][wave] = (uint32_t)__popcll(mk[k]);
     }
     n_exit = wave_sum(n_exit);
     n_dead = wave_sum(n_dead);
     n_move = wave_sum(n_move);
     n_proc = wave_sum(n_proc);
     if (lane == 0) {
-      s_red[0][wave] = n_exit;
-      s_red[1][wave] = n_dead;
-      s_red[2][wave] = n_move;
-      s_red[3][wave] = n_proc;
+      s_red[par][0][wave] = n_exit;
+      s_red[par][1][wave] = n_dead;
+      s_red[par][2][wave] = n_move;
+      s_red[par][3][wave] = n_proc;
     }
     __syncthreads();
     uint32_t total = 0;
 #pragma unroll
     for (uint32_t k = 0; k < NPT; ++k)
 #pragma unroll
-      for (uint32_t wv = 0; wv < NW; ++wv) total += s_div[k][wv];
+      for (uint32_t wv = 0; wv < NW; ++wv) total += s_div[par][k][wv];
     const uint32_t births = total < budget ? total : budget;
-
-    uint32_t before = 0;  // requests in earlier k-rows
+    if (total) {
+      uint32_t before = 0;  // requests in earlier k-rows
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      uint32_t rank = before + (uint32_t)__popcll(mk[k] & lanemask_lt(lane));
+      for (uint32_t k = 0; k < NPT; ++k) {
+        uint32_t rank = before + (uint32_t)__popcll(mk[k] & lanemask_lt(lane));
 #pragma unroll
-      for (uint32_t wv = 0; wv < NW; ++wv) {
-        rank += (wv < wave) ? s_div[k][wv] : 0u;
-        before += s_div[k][wv];
-      }
-      if (want[k] && rank < budget) {
-        if (rd.probe_div) {
-          const unsigned long long e = atomicAdd(&rs->probe_div_n, 1ull);
-          if (e < rd.probe_cap) rd.probe_div[e] = ad[k];
+        for (uint32_t wv = 0; wv < NW; ++wv) {
+          rank += (wv < wave) ? s_div[par][k][wv] : 0u;
+          before += s_div[par][k][wv];
         }
-        double xd[NV];
-        M::division(p, x[k], xd, RngKey{seed, slot0 + k * BLK + tid, step});
-        const uint64_t sidx = (uint64_t)t * stride + rank;
+        if (!want[k]) continue;
+        const uint32_t i = k * BLK + tid;
+        if (rank < budget) {
+          if (rd.probe_div) {
+            const unsigned long long e = atomicAdd(&b.rs[r].probe_div_n, 1ull);
+            if (e < rd.probe_cap) rd.probe_div[e] = A.ad[k];
+          }
+          double xd[NV];
+          M::division(p, A.x[k], xd, RngKey{seed, slot0 + i, step});
+          const uint64_t sidx = (uint64_t)t * stride + rank;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) gp(b.st_state)[(uint64_t)v * ST + sidx] = xd[v];
-        gp(b.st_pos)[sidx] = pos0[k];
-        ad[k] = 0.0;
-      }
-      if (idle[k]) {
+          for (int v = 0; v < NV; ++v) gp(b.st_state)[(uint64_t)v * ST + sidx] = xd[v];
+          gp(b.st_pos)[sidx] = A.pos[k];
+          A.ad[k] = 0.0;
+        }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) x_b[v][li[k]] = x[k][v];
-        if (do_model) ad_b[li[k]] = ad[k];
+        for (int v = 0; v < NV; ++v) (gp(b.state) + (uint64_t)v * P + base)[i] = A.x[k][v];
+        if (do_model) ad_b[i] = A.ad[k];
       }
     }
     if (tid == 0) {
       b.tile_births[t] = births;
       for (uint32_t wv = 0; wv < NW; ++wv) {
-        cnt_acc.exits += s_red[0][wv];
-        cnt_acc.deaths += s_red[1][wv];
-        cnt_acc.moves += s_red[2][wv];
-        cnt_acc.processed += s_red[3][wv];
+        cnt_acc.exits += s_red[par][0][wv];
+        cnt_acc.deaths += s_red[par][1][wv];
+        cnt_acc.moves += s_red[par][2][wv];
+        cnt_acc.processed += s_red[par][3][wv];
       }
       cnt_acc.overflow += total - births;
     }
-    __syncthreads();
-  }
-  if (acc_run != 0xFFFFFFFFu) {
-    if (PRIV) {
-      __syncthreads();
-      for (uint32_t i = tid; i < acc_len; i += BLK) {
-        const double v = s_acc[i];
-        if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
-      }
+    cur = nxt;
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+      A.st[k] = B.st[k];
+      A.pos[k] = B.pos[k];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) A.x[k][v] = B.x[k][v];
+      A.ad[k] = B.ad[k];
+      A.ah[k] = B.ah[k];
     }
-    if (tid == 0) flush_counts(b.rs + acc_run, cnt_acc);
   }
+  if (PRIV) {
+    __syncthreads();
+    for (uint32_t i = tid; i < acc_len; i += BLK) {
+      const double v = s_acc[i];
+      if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
+    }
+  }
+  if (tid == 0) flush_counts(b.rs + acc_run, cnt_acc);
 }
 
 // ------------------------------------------------------------------ bookkeeping
