@@ -43,6 +43,7 @@ int fail(int code, const std::string& msg) {
 struct ModelEntry {
   const char* name;
   int id, nv, ns, np;
+  bool can_die;
   void (*defaults)(double*);
   void (*init)(const double*, double*, uint64_t, uint32_t);
   double (*mass)(const double*, const double*);
@@ -55,7 +56,7 @@ double host_mass(const double* p, const double* x) {
 
 template <class M>
 ModelEntry entry() {
-  return ModelEntry{M::NAME, M::ID, M::NV, M::NS, M::NP, &M::defaults, &M::init, &host_mass<M>};
+  return ModelEntry{M::NAME, M::ID, M::NV, M::NS, M::NP, M::CAN_DIE, &M::defaults, &M::init, &host_mass<M>};
 }
 
 const std::vector<ModelEntry>& registry() {
@@ -231,6 +232,8 @@ struct mcst_engine {
   std::vector<mcst::RunDev> h_runs;
   int grid_particle = 1, grid_transport = 1;
   mcst::ParticleLaunch plaunch;
+  int grid_move = 1;
+  bool removal_possible = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_steps = 0;
@@ -345,8 +348,11 @@ int commit(mcst_engine* e) {
     if ((rc = dev_alloc(e, (void**)a, T_max * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.tile_off, (R + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.move_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.tile_moves, T_max * sizeof(uint32_t)))) return rc;
   HIP_TRY(hipMemsetAsync(b.tile_births, 0, T_max * sizeof(uint32_t), e->stream));
   HIP_TRY(hipMemsetAsync(b.tile_off, 0, (R + 1) * sizeof(uint32_t), e->stream));
+  HIP_TRY(hipMemsetAsync(b.tile_moves, 0, T_max * sizeof(uint32_t), e->stream));
 
   // particles
   for (uint32_t r = 0; r < R; ++r) {
@@ -438,6 +444,11 @@ int commit(mcst_engine* e) {
     s.counters[C_INACTIVE] = inactive;
   }
   comp_off.push_back(n_comp);
+  // Removal can only ever trigger if some particle can become inactive: a
+  // leaving flow, a model that can return Dead, or inactive initial slots.
+  e->removal_possible = e->model->can_die;
+  for (uint32_t r = 0; r < R; ++r)
+    if (e->h_runs[r].enable_leave || rs[r].inactive) e->removal_possible = true;
   MapDev* d_maps = nullptr;
   uint32_t* d_comp = nullptr;
   if ((rc = dev_upload(e, &d_maps, maps.data(), maps.size()))) return rc;
@@ -501,6 +512,8 @@ int commit(mcst_engine* e) {
   e->grid_particle = std::max(e->grid_particle, 1);
   e->grid_transport = (int)std::min<uint64_t>((n_comp + 255) / 256, 8192);
   e->grid_transport = std::max(e->grid_transport, 1);
+  e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
+                                                              (T_max + 15) / 16));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d (lds cap %zu, per-block %zu, optin %zu)\n",
                  best.block, best.grid, best.lds, (int)best.priv, lds_cap, (size_t)prop.sharedMemPerBlock,
@@ -527,8 +540,9 @@ void launch_step(mcst_engine* e, uint32_t stages) {
   const uint32_t nv = (uint32_t)e->model->nv;
   launch_transport(b, e->grid_transport, e->stream);
   launch_particle(e->model->id, b, e->plaunch, e->stream);
+  launch_move(b, e->grid_move, e->stream);
   launch_bookkeep(b, e->stream);
-  launch_removal(b, nv, e->grid_particle, e->stream);
+  if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   launch_merge(b, nv, e->grid_particle, e->stream);
   launch_layout(b, e->stream);
 }
@@ -873,12 +887,13 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
     HIP_TRY(hipEventRecord(x[1], e->stream));
     launch_particle(e->model->id, b, e->plaunch, e->stream);
     HIP_TRY(hipEventRecord(x[2], e->stream));
-    launch_bookkeep(b, e->stream);
+    launch_move(b, e->grid_move, e->stream);
     HIP_TRY(hipEventRecord(x[3], e->stream));
-    launch_removal(b, nv, e->grid_particle, e->stream);
+    launch_bookkeep(b, e->stream);
     HIP_TRY(hipEventRecord(x[4], e->stream));
-    launch_merge(b, nv, e->grid_particle, e->stream);
+    if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
     HIP_TRY(hipEventRecord(x[5], e->stream));
+    launch_merge(b, nv, e->grid_particle, e->stream);
     launch_layout(b, e->stream);
     HIP_TRY(hipEventRecord(x[6], e->stream));
   }

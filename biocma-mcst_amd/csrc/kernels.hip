@@ -6,6 +6,7 @@
 //                    + contribution scatter + move + leave
 //                    (model_kernel.hpp:163-268, contribution_kernel.hpp:144-186,
 //                     move_kernel.hpp:209-273,347-437,586-648)
+//   k_move           neighbour pick + exit test of the particles that left their compartment
 //   k_bookkeep       per-run counters, removal decision  (particles_container.hpp:539-557)
 //   k_gap_count / k_gap_scan / k_gap_list / k_gap_fill
 //                    removal of Exit/Dead slots by gap filling (particles_container.hpp:292-385,735-796)
@@ -216,7 +217,8 @@ __global__ void __launch_bounds__(BLK)
   constexpr uint32_t NW = BLK / 64;      // waves per block
   extern __shared__ double s_acc[];      // [n_s][n_c] of the current run (PRIV)
   __shared__ uint32_t s_div[2][NPT][NW];  // double-buffered by tile parity:
-  __shared__ uint32_t s_red[2][4][NW];    // one barrier per tile suffices
+  __shared__ uint32_t s_mov[2][NPT][NW];  // one barrier per tile suffices
+  __shared__ uint32_t s_red[2][4][NW];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t stages = b.stages;
   const uint64_t P = b.P;
@@ -353,12 +355,13 @@ __global__ void __launch_bounds__(BLK)
     GLOBAL_AS uint8_t* __restrict__ st_b = gp(b.status) + base;
     GLOBAL_AS double* __restrict__ ah_b = gp(b.age_hyd) + base;
     GLOBAL_AS double* __restrict__ ad_b = gp(b.age_div) + base;
-    bool idle[NPT], want[NPT];
+    bool idle[NPT], want[NPT], mover[NPT];
     uint32_t n_exit = 0, n_dead = 0, n_move = 0, n_proc = 0;
 #pragma unroll
     for (uint32_t k = 0; k < NPT; ++k) {
       idle[k] = A.st[k] == IDLE;
       want[k] = false;
+      mover[k] = false;
       if (!idle[k]) continue;
       ++n_proc;
       const uint32_t i = k * BLK + tid;
@@ -391,20 +394,14 @@ __global__ void __launch_bounds__(BLK)
         }
       }
       const U4 u = draw4(seed, slot0 + i, step, TAG_TRANSPORT, 0u);
-      uint32_t pos = A.pos[k];
-      bool moved = false;
-      if (do_move && rr[k][NS] > u01(u.x)) {  // dt*Q/V > u  (probability_leaving.hpp:313-325)
-        const uint32_t m = mp.m;
-        const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
-        pos = gp(mp.nbr)[(size_t)pos * m + col];
-        moved = true;
-        ++n_move;
-        pos_b[i] = pos;
-      }
+      // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:313-325)
+      // is queued; k_move finds its neighbour and runs its exit test
+      mover[k] = do_move && rr[k][NS] > u01(u.x);
+      n_move += mover[k] ? 1u : 0u;
       if (do_leave) {
         A.ah[k] += dt;
-        const double q = moved ? rec[(size_t)pos * REC + NS + 1] : rr[k][NS + 1];
-        if (q < 1.0 && u01(u.z) > q) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
+        const double q = rr[k][NS + 1];
+        if (!mover[k] && q < 1.0 && u01(u.z) > q) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
           if (rd.probe_exit) {
             const unsigned long long e = atomicAdd(&b.rs[r].probe_exit_n, 1ull);
             if (e < rd.probe_cap) rd.probe_exit[e] = A.ah[k];
@@ -423,11 +420,15 @@ __global__ void __launch_bounds__(BLK)
     }
 
     // ---- rank the division requests of the tile in slot order
-    uint64_t mk[NPT];
+    uint64_t mk[NPT], mv[NPT];
 #pragma unroll
     for (uint32_t k = 0; k < NPT; ++k) {
       mk[k] = __ballot(want[k]);
-      if (lane == 0) s_div[par][k][wave] = (uint32_t)__popcll(mk[k]);
+      mv[k] = __ballot(mover[k]);
+      if (lane == 0) {
+        s_div[par][k][wave] = (uint32_t)__popcll(mk[k]);
+        s_mov[par][k][wave] = (uint32_t)__popcll(mv[k]);
+      }
     }
     n_exit = wave_sum(n_exit);
     n_dead = wave_sum(n_dead);
@@ -476,7 +477,20 @@ __global__ void __launch_bounds__(BLK)
         if (do_model) ad_b[i] = A.ad[k];
       }
     }
+    // ---- the tile's movers, in slot order, for k_move
+    uint32_t n_moves = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < NPT; ++k) {
+      uint32_t rank = n_moves + (uint32_t)__popcll(mv[k] & lanemask_lt(lane));
+#pragma unroll
+      for (uint32_t wv = 0; wv < NW; ++wv) {
+        rank += (wv < wave) ? s_mov[par][k][wv] : 0u;
+        n_moves += s_mov[par][k][wv];
+      }
+      if (mover[k]) b.move_slot[(uint64_t)t * TILE + rank] = (uint16_t)(k * BLK + tid);
+    }
     if (tid == 0) {
+      b.tile_moves[t] = n_moves;
       b.tile_births[t] = births;
       for (uint32_t wv = 0; wv < NW; ++wv) {
         cnt_acc.exits += s_red[par][0][wv];
@@ -505,6 +519,72 @@ __global__ void __launch_bounds__(BLK)
     }
   }
   if (tid == 0) flush_counts(b.rs + acc_run, cnt_acc);
+}
+
+// ------------------------------------------------------------------ moves
+// Neighbour pick for the particles the pass listed: the first CDF column at or
+// above u, then the exit test in the new compartment (move_kernel.hpp:392-437,
+// 586-648).  A block takes MOVE_GROUP tiles and spreads their movers over its
+// threads; every mover is independent, so the processing order is free.
+constexpr uint32_t MOVE_GROUP = 16;
+
+__global__ void __launch_bounds__(256) k_move(BatchDev b, const RunDev* __restrict__ runs,
+                                              const MapDev* __restrict__ maps,
+                                              const uint32_t* __restrict__ tile_off) {
+  __shared__ uint32_t s_pre[MOVE_GROUP + 1];
+  __shared__ uint32_t s_run[MOVE_GROUP];
+  const uint32_t T = tile_off[b.R];
+  const uint32_t n_groups = (T + MOVE_GROUP - 1) / MOVE_GROUP;
+  const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
+  for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+    const uint32_t t0 = grp * MOVE_GROUP;
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (uint32_t j = 0; j < MOVE_GROUP; ++j) {
+        s_pre[j] = acc;
+        const uint32_t t = t0 + j;
+        if (t < T) {
+          acc += b.tile_moves[t];
+          s_run[j] = find_run(tile_off, b.R, t);
+        }
+      }
+      s_pre[MOVE_GROUP] = acc;
+    }
+    __syncthreads();
+    const uint32_t total = s_pre[MOVE_GROUP];
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      uint32_t j = 0;
+      while (s_pre[j + 1] <= e) ++j;
+      const uint32_t t = t0 + j;
+      const uint32_t r = s_run[j];
+      const RunDev& rd = runs[r];
+      RunState* rs = b.rs + r;
+      const uint32_t slot = (t - tile_off[r]) * TILE + b.move_slot[(uint64_t)t * TILE + (e - s_pre[j])];
+      const uint64_t g = rd.p_off + slot;
+      const uint32_t step = rs->step;
+      const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
+      const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
+      const uint32_t m = mp.m;
+      uint32_t pos = b.pos[g];
+      const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
+      pos = gp(mp.nbr)[(size_t)pos * m + col];
+      b.pos[g] = pos;
+      if (rd.enable_leave && leave_stage) {
+        const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + rd.n_s + 1];
+        if (qx < 1.0 && u01(u.z) > qx) {
+          const double age = b.age_hyd[g];
+          if (rd.probe_exit) {
+            const unsigned long long k = atomicAdd(&rs->probe_exit_n, 1ull);
+            if (k < rd.probe_cap) rd.probe_exit[k] = age;
+          }
+          b.age_hyd[g] = 0.0;
+          b.status[g] = EXIT;
+          atomicAdd(&rs->step_exits, 1ull);
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------ bookkeeping
@@ -702,38 +782,18 @@ __global__ void __launch_bounds__(BLOCK) k_gap_fill(BatchDev b, uint32_t n_var) 
   }
 }
 
-// ------------------------------------------------------------------ merge
-__global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
-  const uint32_t T = b.tile_off[b.R];
-  for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
-    const uint32_t nb = b.tile_births[t];
-    if (nb == 0) continue;
-    const uint32_t r = find_run(b.tile_off, b.R, t);
-    const RunDev& rd = b.runs[r];
-    const RunState* rs = b.rs + r;
-    const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
-    const uint64_t src0 = (uint64_t)t * b.stage_stride;
-    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
-      const uint64_t d = dst0 + j, s = src0 + j;
-      b.pos[d] = b.st_pos[s];
-      b.status[d] = IDLE;
-      b.age_hyd[d] = 0.0;
-      b.age_div[d] = 0.0;
-      for (uint32_t v = 0; v < n_var; ++v) b.state[(uint64_t)v * b.P + d] = b.st_state[(uint64_t)v * b.ST + s];
-    }
-  }
-}
-
 // ------------------------------------------------------------------ layout
 // Tile -> run table and per-tile division budget for the next pass:
 // budget = min(stage_cap, (cap - n) / tiles), so staged newborns always fit.
-__global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) {
-  __shared__ uint32_t s_w[SCAN_WAVES];
+template <uint32_t NT>
+__device__ void layout_pass(const BatchDev& b) {
+  constexpr uint32_t NWV = NT / 64;
+  __shared__ uint32_t s_w[NWV];
   __shared__ uint32_t s_carry;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   if (tid == 0) s_carry = 0;
   __syncthreads();
-  for (uint32_t r0 = 0; r0 < b.R; r0 += SCAN_THREADS) {
+  for (uint32_t r0 = 0; r0 < b.R; r0 += NT) {
     const uint32_t r = r0 + tid;
     uint32_t tiles = 0;
     if (r < b.R) {
@@ -754,7 +814,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) {
     if (lane == 63) s_w[wave] = inc;
     __syncthreads();
     uint32_t wpre = 0, tot = 0;
-    for (uint32_t w = 0; w < SCAN_WAVES; ++w) {
+    for (uint32_t w = 0; w < NWV; ++w) {
       if (w < wave) wpre += s_w[w];
       tot += s_w[w];
     }
@@ -768,6 +828,31 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) {
     __syncthreads();
   }
   if (tid == 0) b.tile_off[b.R] = s_carry;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) { layout_pass<SCAN_THREADS>(b); }
+
+// ------------------------------------------------------------------ merge
+// Appends staged newborns in tile order.
+__global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
+  const uint32_t T = b.tile_off[b.R];
+  for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
+    const uint32_t nb = b.tile_births[t];
+    if (nb == 0) continue;
+    const uint32_t r = find_run(b.tile_off, b.R, t);
+    const RunDev& rd = b.runs[r];
+    const RunState* rs = b.rs + r;
+    const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
+    const uint64_t src0 = (uint64_t)t * b.stage_stride;
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+      const uint64_t d = dst0 + j, s = src0 + j;
+      b.pos[d] = b.st_pos[s];
+      b.status[d] = IDLE;
+      b.age_hyd[d] = 0.0;
+      b.age_div[d] = 0.0;
+      for (uint32_t v = 0; v < n_var; ++v) b.state[(uint64_t)v * b.P + d] = b.st_state[(uint64_t)v * b.ST + s];
+    }
+  }
 }
 
 // ------------------------------------------------------------------ histogram
@@ -846,6 +931,9 @@ void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) 
 }
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_merge, dim3(grid), dim3(BLOCK), 0, s, b, n_var);
+}
+void launch_move(const BatchDev& b, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_move, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, (const uint32_t*)b.tile_off);
 }
 void launch_layout(const BatchDev& b, hipStream_t s) {
   hipLaunchKernelGGL(k_layout, dim3(1), dim3(SCAN_THREADS), 0, s, b);

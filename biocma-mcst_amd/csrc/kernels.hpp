@@ -22,6 +22,7 @@ void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, h
 const void* particle_kernel_ptr(int model_id, int block, bool priv);
 void launch_bookkeep(const BatchDev& b, hipStream_t s);
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
+void launch_move(const BatchDev& b, int grid, hipStream_t s);
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_layout(const BatchDev& b, hipStream_t s);
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s);

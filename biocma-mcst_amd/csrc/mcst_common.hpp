@@ -142,6 +142,8 @@ struct BatchDev {
   uint32_t* tile_fill_off;
   uint32_t* fill_src;       // [P] scratch list of filler slots during removal
   uint32_t* tile_off;       // [R+1] first global tile of each run in the current layout
+  uint16_t* move_slot;      // [T_max*TILE] tile-local slots of the particles leaving their compartment
+  uint32_t* tile_moves;     // [T_max] entries of each tile in move_slot
   // runs
   const RunDev* runs;
   RunState* rs;

@@ -7,6 +7,7 @@
 //   update(p, dt, x, c, contrib) -> Status device+host               (Model::update)
 //   division(p, x, xd, key)                device+host              (Model::division)
 //   mass(p, x)                             device+host              (Model::mass)
+//   CAN_DIE                                whether update() may return DEAD
 // State is FP64; contributions (kg/s per particle) are returned in
 // registers instead of a per-particle contribs row, because the step kernel
 // scatters them immediately.  All code here is built with -ffp-contract=off,
@@ -131,6 +132,7 @@ constexpr int rec_stride(int ns) { return (ns + 2) <= 4 ? 4 : ((ns + 2) <= 8 ? 8
 // x = {m, m_birth}; p = {mu_max, Ks, Y, ratio, m0_mean, m0_cv}.
 struct MonodMass {
   static constexpr int ID = 0, NV = 2, NS = 1, NP = 6;
+  static constexpr bool CAN_DIE = false;  // update() never returns DEAD
   static constexpr const char* NAME = "monod_mass";
   static void defaults(double* p) {
     p[0] = 0.5 / 3600.0; p[1] = 0.01; p[2] = 0.5; p[3] = 2.0; p[4] = 1e-15; p[5] = 0.1;
@@ -165,6 +167,7 @@ struct MonodMass {
 // p = {qs_max, Ks, Ko, Yxs, Yos, tau, k_st, m_div, m0_mean, m0_sd}.
 struct DualMonod {
   static constexpr int ID = 1, NV = 3, NS = 2, NP = 10;
+  static constexpr bool CAN_DIE = false;  // update() never returns DEAD
   static constexpr const char* NAME = "dual_monod";
   static void defaults(double* p) {
     p[0] = 2.8e-4; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.5; p[4] = 1.0; p[5] = 100.0; p[6] = 1e-4;
@@ -210,6 +213,7 @@ struct DualMonod {
 // p = {mu_max, k_s, y_s_x, l_max, d_m, tau_meta}.
 struct Monod {
   static constexpr int ID = 2, NV = 6, NS = 1, NP = 6;
+  static constexpr bool CAN_DIE = false;  // update() never returns DEAD
   static constexpr const char* NAME = "monod";
   static void defaults(double* p) {
     p[0] = 0.77 / 3600.0; p[1] = 1e-3; p[2] = 2.0; p[3] = 2e-6; p[4] = 0.6e-6; p[5] = 3600.0 / 0.77;
@@ -253,6 +257,7 @@ struct Monod {
 // x = {l, l_max}; p = {l_dot_max, l_max, k, d_m}.
 struct FixedLength {
   static constexpr int ID = 3, NV = 2, NS = 1, NP = 4;
+  static constexpr bool CAN_DIE = false;  // update() never returns DEAD
   static constexpr const char* NAME = "fixed_length";
   static void defaults(double* p) {
     p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6;
@@ -290,6 +295,7 @@ struct FixedLength {
 // x = {length, l_max, a_p, a_max, a_e, a_e_s, a_e_a, phi_s, phi_a}.
 struct SimpleAcetate {
   static constexpr int ID = 4, NV = 9, NS = 2, NP = 0;
+  static constexpr bool CAN_DIE = false;  // update() never returns DEAD
   static constexpr const char* NAME = "simple_acetate";
   static constexpr double A_MAX = 2e-6 / 3600.0;
   static constexpr double L_MAX = 2e-6;
