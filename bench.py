@@ -158,7 +158,7 @@ def main():
     per_launch = (run.counters()["processed"] - p0) / a.profile_steps
     t_particle = per[1] * 1e-3
     achieved = per_launch * b_ps / t_particle / 1e9
-    names = ["transport", "particle", "move", "bookkeep", "removal", "merge_layout"]
+    names = ["transport_layout", "particle", "move", "bookkeep", "removal", "merge"]
     log("kernel ms/step: " + ", ".join(f"{n}={v:.4f}" for n, v in zip(names, per)))
 
     traffic = read_traffic()

@@ -333,6 +333,7 @@ int commit(mcst_engine* e) {
   b.ST = T_max * stride;
   b.R = R;
   b.n_comp_total = n_comp;
+  b.ns_max = std::max<uint32_t>(1, (uint32_t)M.ns);  // one model per engine; thread s == 0 also writes the map fields
   b.stages = STAGE_ALL;
   int rc;
   if ((rc = dev_alloc(e, (void**)&b.pos, P * sizeof(uint32_t)))) return rc;
@@ -349,6 +350,7 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.tile_off, (R + 1) * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.move_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.tile_moves, T_max * sizeof(uint32_t)))) return rc;
   HIP_TRY(hipMemsetAsync(b.tile_births, 0, T_max * sizeof(uint32_t), e->stream));
   HIP_TRY(hipMemsetAsync(b.tile_off, 0, (R + 1) * sizeof(uint32_t), e->stream));
@@ -510,10 +512,10 @@ int commit(mcst_engine* e) {
   e->plaunch = best;
   e->grid_particle = (int)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, T_max);
   e->grid_particle = std::max(e->grid_particle, 1);
-  e->grid_transport = (int)std::min<uint64_t>((n_comp + 255) / 256, 8192);
-  e->grid_transport = std::max(e->grid_transport, 1);
+  // one thread per (compartment, species), plus the block that lays out the tiles
+  e->grid_transport = (int)std::min<uint64_t>((n_comp * b.ns_max + 255) / 256, 8192) + 1;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
-                                                              (T_max + 15) / 16));
+                                                              (T_max + MOVE_GROUP - 1) / MOVE_GROUP));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d (lds cap %zu, per-block %zu, optin %zu)\n",
                  best.block, best.grid, best.lds, (int)best.priv, lds_cap, (size_t)prop.sharedMemPerBlock,
@@ -544,7 +546,6 @@ void launch_step(mcst_engine* e, uint32_t stages) {
   launch_bookkeep(b, e->stream);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   launch_merge(b, nv, e->grid_particle, e->stream);
-  launch_layout(b, e->stream);
 }
 
 int read_state(mcst_engine* e, int32_t run, RunState& s) {
@@ -894,7 +895,6 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
     if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
     HIP_TRY(hipEventRecord(x[5], e->stream));
     launch_merge(b, nv, e->grid_particle, e->stream);
-    launch_layout(b, e->stream);
     HIP_TRY(hipEventRecord(x[6], e->stream));
   }
   HIP_TRY(hipEventRecord(t1, e->stream));

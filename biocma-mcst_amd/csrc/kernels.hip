@@ -1,7 +1,9 @@
 // kernels.hip -- the step of a batch of runs as gfx950 kernels.
 //
 // One step of every active run is, in stream order:
-//   k_transport      liquid transport of all runs      (implScalar.cpp:250-264)
+//   k_transport      liquid transport of all runs      (implScalar.cpp:250-264);
+//                    its last block lays out the step's tiles: tile -> run
+//                    table and division budgets (k_layout alone at commit)
 //   k_particle<M>    fused per-particle pass: model update + division staging
 //                    + contribution scatter + move + leave
 //                    (model_kernel.hpp:163-268, contribution_kernel.hpp:144-186,
@@ -11,7 +13,6 @@
 //   k_gap_count / k_gap_scan / k_gap_list / k_gap_fill
 //                    removal of Exit/Dead slots by gap filling (particles_container.hpp:292-385,735-796)
 //   k_merge          append staged newborns                (particles_container.hpp:403-443,575-599)
-//   k_layout         tile -> run table and division budgets for the next step
 // Every kernel reads particle counts from device memory, so the sequence can
 // be replayed without host round trips.
 #include <hip/hip_runtime.h>
@@ -92,77 +93,93 @@ constexpr uint32_t SCAN_THREADS = 1024;
 constexpr uint32_t SCAN_WAVES = SCAN_THREADS / 64;
 
 __device__ uint32_t block_scan(const uint32_t* in, uint32_t* out, uint32_t n, bool descending) {
+  // each thread owns a contiguous run of `per` items: one read pass, one
+  // block-wide scan of the per-thread sums, one write pass
   __shared__ uint32_t s_w[SCAN_WAVES];
-  __shared__ uint32_t s_carry;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  __syncthreads();  // earlier readers of s_carry are done
-  if (tid == 0) s_carry = 0;
+  const uint32_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+  const uint32_t k0 = tid * per;
+  const uint32_t k1 = (k0 + per < n) ? k0 + per : n;
+  uint32_t sum = 0;
+  for (uint32_t k = k0; k < k1; ++k) sum += in[descending ? n - 1u - k : k];
+  const uint32_t inc = wave_incl_scan(sum, lane);
+  __syncthreads();  // earlier readers of s_w are done
+  if (lane == 63) s_w[wave] = inc;
   __syncthreads();
-  for (uint32_t b0 = 0; b0 < n; b0 += SCAN_THREADS) {
-    const uint32_t k = b0 + tid;
-    const uint32_t idx = descending ? (n - 1u - k) : k;
-    const uint32_t v = (k < n) ? in[idx] : 0u;
-    const uint32_t inc = wave_incl_scan(v, lane);
-    if (lane == 63) s_w[wave] = inc;
-    __syncthreads();
-    uint32_t wpre = 0, tot = 0;
-    for (uint32_t w = 0; w < SCAN_WAVES; ++w) {
-      const uint32_t x = s_w[w];
-      if (w < wave) wpre += x;
-      tot += x;
-    }
-    const uint32_t carry = s_carry;
-    if (k < n && out) out[idx] = carry + wpre + inc - v;
-    __syncthreads();
-    if (tid == 0) s_carry = carry + tot;
-    __syncthreads();
+  uint32_t wpre = 0, tot = 0;
+  for (uint32_t w = 0; w < SCAN_WAVES; ++w) {
+    const uint32_t x = s_w[w];
+    if (w < wave) wpre += x;
+    tot += x;
   }
-  return s_carry;
+  if (out) {
+    uint32_t run = wpre + inc - sum;
+    for (uint32_t k = k0; k < k1; ++k) {
+      const uint32_t idx = descending ? n - 1u - k : k;
+      const uint32_t v = in[idx];
+      out[idx] = run;
+      run += v;
+    }
+  }
+  return tot;
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------ transport
 // m += dt*(C*M - C*diag(sink) + S + F);  C = m*diag(1/V)  for every
-// compartment of every active run; also refreshes the per-compartment record
-// {C_0..C_{n_s-1}, p_move, q_exit} the particle pass gathers, and clears S.
+// (compartment, species) of every active run, one thread each; also refreshes
+// the per-compartment record {C_0..C_{n_s-1}, p_move, q_exit} the particle
+// pass gathers, and clears S.  The extra last block lays out the step's
+// tiles (layout_pass), which needs only the counts of the previous step.
+template <uint32_t NT>
+__device__ void layout_pass(const BatchDev& b);
+
 __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
+  if (blockIdx.x == gridDim.x - 1) {
+    layout_pass<256>(b);
+    return;
+  }
   const uint32_t total = b.n_comp_total;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+  const uint32_t stride = (gridDim.x - 1) * blockDim.x;
+  const uint32_t pairs = total * b.ns_max;
+  for (uint32_t gs = blockIdx.x * blockDim.x + threadIdx.x; gs < pairs; gs += stride) {
+    // gs enumerates (compartment g, species s) pairs, species-major
+    const uint32_t g = gs % total;
+    const uint32_t s = gs / total;
     const uint32_t r = find_run(b.comp_off, b.R, g);
-    const RunState* rs = b.rs + r;
-    if (!rs->active) continue;
     const RunDev& rd = b.runs[r];
+    if (s >= rd.n_s && s != 0) continue;
+    const RunState* rs = b.rs + r;
+    // the run's activity is recomputed here rather than read from the
+    // layout block, which writes it concurrently
+    if (rs->step >= rd.step_target) continue;
     const uint32_t j = g - b.comp_off[r];
     const uint32_t n_c = rd.n_c, n_s = rd.n_s;
     const uint32_t step = rs->step;
     const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
-    const double* Co = rd.C[step & 1u];
-    double* Cn = rd.C[(step + 1u) & 1u];
     double* rec = rd.rec + (size_t)j * rd.rec_w;
-    const bool on = (b.stages & STAGE_TRANSPORT) != 0;
-    const uint32_t e0 = mp.col_ptr[j], e1 = mp.col_ptr[j + 1];
-    const double sink = rd.sink[j];
-    const double inv_v = mp.inv_v[j];
-    const double dt = rd.dt;
-    for (uint32_t s = 0; s < n_s; ++s) {
-      const double* c = Co + (size_t)s * n_c;
-      const size_t o = (size_t)s * n_c + j;
-      double cn = c[j];
-      if (on) {
-        double acc = 0.0;
-        for (uint32_t e = e0; e < e1; ++e) acc += c[mp.src[e]] * mp.val[e];
-        const double dmdt = (acc - c[j] * sink) + (rd.S[o] + rd.F[o]);
-        const double mm = rd.m[o] + dt * dmdt;
-        rd.m[o] = mm;
-        cn = mm * inv_v;
-      }
-      Cn[o] = cn;
-      rec[s] = cn;
-      rd.S[o] = 0.0;
+    if (s == 0) {
+      rec[n_s] = mp.p_move[j];
+      rec[n_s + 1] = mp.q_exit[j];
     }
-    rec[n_s] = mp.p_move[j];
-    rec[n_s + 1] = mp.q_exit[j];
+    if (s >= n_s) continue;
+    const double* c = rd.C[step & 1u] + (size_t)s * n_c;
+    double* Cn = rd.C[(step + 1u) & 1u];
+    const size_t o = (size_t)s * n_c + j;
+    double cn = c[j];
+    if (b.stages & STAGE_TRANSPORT) {
+      const uint32_t e0 = mp.col_ptr[j], e1 = mp.col_ptr[j + 1];
+      double acc = 0.0;
+      for (uint32_t e = e0; e < e1; ++e) acc += c[mp.src[e]] * mp.val[e];
+      const double dmdt = (acc - c[j] * rd.sink[j]) + (rd.S[o] + rd.F[o]);
+      const double mm = rd.m[o] + rd.dt * dmdt;
+      rd.m[o] = mm;
+      cn = mm * mp.inv_v[j];
+    }
+    Cn[o] = cn;
+    rec[s] = cn;
+    rd.S[o] = 0.0;
   }
 }
 
@@ -184,18 +201,23 @@ __device__ __forceinline__ void flush_counts(RunState* rs, TileCounts& c) {
 
 // First column of a cumulative row with cp[c] >= u, clamped to m-1: for a
 // non-decreasing row this is #{c < m-1 : cp[c] < u}, the result of the
-// reference's bisection (move_kernel.hpp:89-102).  Counting over chunks of 8
-// independent loads takes one memory round trip per 8 columns instead of one
-// per bisection step.
+// reference's bisection (move_kernel.hpp:89-102).  The count runs over chunks
+// of 8 independent loads, one memory round trip each, and stops at the first
+// chunk holding a column with cp[c] >= u: in a non-decreasing row no later
+// column can count.  Most rows end within the first chunk.
 __device__ __forceinline__ uint32_t cdf_column(const GLOBAL_AS double* cp, uint32_t m, double u) {
+  constexpr int CH = 8;
   uint32_t left = 0;
   const int last = (int)m - 1;
-  for (int c0 = 0; c0 < last; c0 += 8) {
-    double v[8];
+  for (int c0 = 0; c0 < last; c0 += CH) {
+    double v[CH];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = (c0 + j < last) ? cp[c0 + j] : 2.0;
+    for (int j = 0; j < CH; ++j) v[j] = (c0 + j < last) ? cp[c0 + j] : 2.0;
+    uint32_t k = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) left += (u > v[j]) ? 1u : 0u;
+    for (int j = 0; j < CH; ++j) k += (u > v[j]) ? 1u : 0u;
+    left += k;
+    if (k < (uint32_t)CH) break;
   }
   return left;
 }
@@ -318,7 +340,6 @@ __global__ void __launch_bounds__(BLK)
     run_state(r);
     const uint32_t step = c_step;
     const uint32_t budget = c_budget;
-    const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
     const GLOBAL_AS double* __restrict__ rec = gp(rd.rec);
 
     // ---- compartment records of the current tile (needs its positions)
@@ -351,7 +372,6 @@ __global__ void __launch_bounds__(BLK)
     const bool do_leave = rd.enable_leave && (stages & STAGE_LEAVE);
     const uint64_t base = cur.base;
     const uint32_t slot0 = cur.slot0;
-    GLOBAL_AS uint32_t* __restrict__ pos_b = gp(b.pos) + base;
     GLOBAL_AS uint8_t* __restrict__ st_b = gp(b.status) + base;
     GLOBAL_AS double* __restrict__ ah_b = gp(b.age_hyd) + base;
     GLOBAL_AS double* __restrict__ ad_b = gp(b.age_div) + base;
@@ -487,7 +507,10 @@ __global__ void __launch_bounds__(BLK)
         rank += (wv < wave) ? s_mov[par][k][wv] : 0u;
         n_moves += s_mov[par][k][wv];
       }
-      if (mover[k]) b.move_slot[(uint64_t)t * TILE + rank] = (uint16_t)(k * BLK + tid);
+      if (mover[k]) {
+        b.move_slot[(uint64_t)t * TILE + rank] = (uint16_t)(k * BLK + tid);
+        b.move_pos[(uint64_t)t * TILE + rank] = A.pos[k];
+      }
     }
     if (tid == 0) {
       b.tile_moves[t] = n_moves;
@@ -526,7 +549,6 @@ __global__ void __launch_bounds__(BLK)
 // above u, then the exit test in the new compartment (move_kernel.hpp:392-437,
 // 586-648).  A block takes MOVE_GROUP tiles and spreads their movers over its
 // threads; every mover is independent, so the processing order is free.
-constexpr uint32_t MOVE_GROUP = 16;
 
 __global__ void __launch_bounds__(256) k_move(BatchDev b, const RunDev* __restrict__ runs,
                                               const MapDev* __restrict__ maps,
@@ -538,17 +560,19 @@ __global__ void __launch_bounds__(256) k_move(BatchDev b, const RunDev* __restri
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
   for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
     const uint32_t t0 = grp * MOVE_GROUP;
-    if (threadIdx.x == 0) {
-      uint32_t acc = 0;
-      for (uint32_t j = 0; j < MOVE_GROUP; ++j) {
-        s_pre[j] = acc;
-        const uint32_t t = t0 + j;
-        if (t < T) {
-          acc += b.tile_moves[t];
-          s_run[j] = find_run(tile_off, b.R, t);
-        }
+    if (threadIdx.x < 64) {  // the group's tiles, one lane each
+      const uint32_t lane = threadIdx.x, t = t0 + lane;
+      uint32_t cnt = 0, r = 0;
+      if (lane < MOVE_GROUP && t < T) {
+        cnt = b.tile_moves[t];
+        r = find_run(tile_off, b.R, t);
       }
-      s_pre[MOVE_GROUP] = acc;
+      const uint32_t inc = wave_incl_scan(cnt, lane);
+      if (lane < MOVE_GROUP) {
+        s_pre[lane] = inc - cnt;
+        s_run[lane] = r;
+      }
+      if (lane == MOVE_GROUP - 1) s_pre[MOVE_GROUP] = inc;
     }
     __syncthreads();
     const uint32_t total = s_pre[MOVE_GROUP];
@@ -559,13 +583,14 @@ __global__ void __launch_bounds__(256) k_move(BatchDev b, const RunDev* __restri
       const uint32_t r = s_run[j];
       const RunDev& rd = runs[r];
       RunState* rs = b.rs + r;
-      const uint32_t slot = (t - tile_off[r]) * TILE + b.move_slot[(uint64_t)t * TILE + (e - s_pre[j])];
+      const uint64_t q = (uint64_t)t * TILE + (e - s_pre[j]);
+      const uint32_t slot = (t - tile_off[r]) * TILE + b.move_slot[q];
+      uint32_t pos = b.move_pos[q];
       const uint64_t g = rd.p_off + slot;
       const uint32_t step = rs->step;
       const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
       const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
       const uint32_t m = mp.m;
-      uint32_t pos = b.pos[g];
       const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
       pos = gp(mp.nbr)[(size_t)pos * m + col];
       b.pos[g] = pos;

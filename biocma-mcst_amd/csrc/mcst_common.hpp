@@ -16,6 +16,7 @@ namespace mcst {
 constexpr uint32_t BLOCK = 256;            // threads per particle block (4 wave64)
 constexpr uint32_t PPT = 4;                // particles per thread per tile
 constexpr uint32_t TILE = BLOCK * PPT;     // 1024 slots per tile (the reference's 1024 particles/team)
+constexpr uint32_t MOVE_GROUP = 8;         // tiles whose movers one block of the mover pass resolves
 constexpr uint32_t WAVES = BLOCK / 64;
 constexpr int MAX_PARAMS = 16;
 constexpr int MAX_VAR = 16;
@@ -143,6 +144,7 @@ struct BatchDev {
   uint32_t* fill_src;       // [P] scratch list of filler slots during removal
   uint32_t* tile_off;       // [R+1] first global tile of each run in the current layout
   uint16_t* move_slot;      // [T_max*TILE] tile-local slots of the particles leaving their compartment
+  uint32_t* move_pos;       // [T_max*TILE] their compartment before the move
   uint32_t* tile_moves;     // [T_max] entries of each tile in move_slot
   // runs
   const RunDev* runs;
@@ -151,6 +153,7 @@ struct BatchDev {
   const uint32_t* comp_off; // [R+1]
   uint32_t R;
   uint32_t n_comp_total;
+  uint32_t ns_max;          // largest species count of the batch
   uint32_t T_max;
   uint32_t stages;
 };
