@@ -43,7 +43,7 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(case_fn, seconds=8.0, max_steps=400, n_particles=1_000_000):
+def cpu_baseline(case_fn, seconds=12.0, max_steps=20000, n_particles=1_000_000):
     from oracle.oracle import OracleRun
 
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -158,7 +158,7 @@ def main():
     per_launch = (run.counters()["processed"] - p0) / a.profile_steps
     t_particle = per[1] * 1e-3
     achieved = per_launch * b_ps / t_particle / 1e9
-    names = ["transport_layout", "particle", "move", "bookkeep", "removal", "merge"]
+    names = ["transport_layout", "particle", "resolve", "bookkeep", "removal", "merge"]
     log("kernel ms/step: " + ", ".join(f"{n}={v:.4f}" for n, v in zip(names, per)))
 
     traffic = read_traffic()

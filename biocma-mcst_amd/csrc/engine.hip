@@ -316,7 +316,9 @@ int commit(mcst_engine* e) {
     h.tile_cap_off = (uint32_t)T_max;
     h.comp_off = n_comp;
     h.map_base = n_maps;
-    P += h.cap;
+    // each run's slots start on a tile boundary: the particle pass moves
+    // whole tiles, so the padding after cap is addressable but never used
+    P += (h.cap + TILE - 1) / TILE * TILE;
     T_max += (h.cap + TILE - 1) / TILE;
     stride = std::max(stride, h.stage_cap);
     n_comp += h.n_c;
@@ -333,6 +335,7 @@ int commit(mcst_engine* e) {
   b.ST = T_max * stride;
   b.R = R;
   b.n_comp_total = n_comp;
+  b.nsub = NSUB;
   b.ns_max = std::max<uint32_t>(1, (uint32_t)M.ns);  // one model per engine; thread s == 0 also writes the map fields
   b.stages = STAGE_ALL;
   int rc;
@@ -351,10 +354,14 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.tile_off, (R + 1) * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.move_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.tile_moves, T_max * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * NSUB * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * NSUB * sizeof(uint32_t)))) return rc;
   HIP_TRY(hipMemsetAsync(b.tile_births, 0, T_max * sizeof(uint32_t), e->stream));
   HIP_TRY(hipMemsetAsync(b.tile_off, 0, (R + 1) * sizeof(uint32_t), e->stream));
-  HIP_TRY(hipMemsetAsync(b.tile_moves, 0, T_max * sizeof(uint32_t), e->stream));
+  HIP_TRY(hipMemsetAsync(b.sub_moves, 0, T_max * NSUB * sizeof(uint32_t), e->stream));
+  HIP_TRY(hipMemsetAsync(b.sub_divs, 0, T_max * NSUB * sizeof(uint32_t), e->stream));
 
   // particles
   for (uint32_t r = 0; r < R; ++r) {
@@ -481,7 +488,7 @@ int commit(mcst_engine* e) {
   for (int blk : {256, 512, 1024}) {
     if (!priv && blk != 256) continue;
     if (forced_block && blk != forced_block) continue;
-    const void* fn = particle_kernel_ptr(M.id, blk, priv);
+    const void* fn = particle_kernel_ptr(M.id, blk, priv, false);
     const size_t lds = priv ? lds_need : 0;
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -505,17 +512,26 @@ int commit(mcst_engine* e) {
   if (best_waves < 0) {  // no resident configuration with the accumulator: go direct
     best = ParticleLaunch{};
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false), 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false, false), 256, 0));
     best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
   }
   best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, T_max));
+  for (const auto& h : e->runs) best.probe = best.probe || h.probe_cap > 0;
+  // one run: blocks take groups in turn, which keeps the wave front of the
+  // whole grid inside a narrow window of the particle arrays; a batch: each
+  // block takes one contiguous range, so it meets few runs
+  b.tile_group = R == 1 ? 1u : 0u;
+  if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
+  if (best.probe && best.lds > 64 * 1024)
+    (void)hipFuncSetAttribute(particle_kernel_ptr(M.id, 256, best.priv, true),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)best.lds);
   e->plaunch = best;
   e->grid_particle = (int)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, T_max);
   e->grid_particle = std::max(e->grid_particle, 1);
   // one thread per (compartment, species), plus the block that lays out the tiles
   e->grid_transport = (int)std::min<uint64_t>((n_comp * b.ns_max + 255) / 256, 8192) + 1;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
-                                                              (T_max + MOVE_GROUP - 1) / MOVE_GROUP));
+                                                              (T_max * NSUB + 63) / 64));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d (lds cap %zu, per-block %zu, optin %zu)\n",
                  best.block, best.grid, best.lds, (int)best.priv, lds_cap, (size_t)prop.sharedMemPerBlock,
@@ -542,7 +558,7 @@ void launch_step(mcst_engine* e, uint32_t stages) {
   const uint32_t nv = (uint32_t)e->model->nv;
   launch_transport(b, e->grid_transport, e->stream);
   launch_particle(e->model->id, b, e->plaunch, e->stream);
-  launch_move(b, e->grid_move, e->stream);
+  launch_resolve(e->model->id, b, e->grid_move, e->stream);
   launch_bookkeep(b, e->stream);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   launch_merge(b, nv, e->grid_particle, e->stream);
@@ -888,7 +904,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
     HIP_TRY(hipEventRecord(x[1], e->stream));
     launch_particle(e->model->id, b, e->plaunch, e->stream);
     HIP_TRY(hipEventRecord(x[2], e->stream));
-    launch_move(b, e->grid_move, e->stream);
+    launch_resolve(e->model->id, b, e->grid_move, e->stream);
     HIP_TRY(hipEventRecord(x[3], e->stream));
     launch_bookkeep(b, e->stream);
     HIP_TRY(hipEventRecord(x[4], e->stream));

@@ -8,7 +8,8 @@
 //                    + contribution scatter + move + leave
 //                    (model_kernel.hpp:163-268, contribution_kernel.hpp:144-186,
 //                     move_kernel.hpp:209-273,347-437,586-648)
-//   k_move           neighbour pick + exit test of the particles that left their compartment
+//   k_resolve<M>     neighbour pick + exit test of the particles that left their
+//                    compartment; division acceptance, mother halving, staging
 //   k_bookkeep       per-run counters, removal decision  (particles_container.hpp:539-557)
 //   k_gap_count / k_gap_scan / k_gap_list / k_gap_fill
 //                    removal of Exit/Dead slots by gap filling (particles_container.hpp:292-385,735-796)
@@ -222,317 +223,439 @@ __device__ __forceinline__ uint32_t cdf_column(const GLOBAL_AS double* cp, uint3
   return left;
 }
 
-// The particle pass.  BLK threads process a TILE = 1024-slot tile, NPT = TILE/BLK
-// slots each.  Each block takes a contiguous range of tiles, so it meets few
-// runs, and runs it as a two-stage pipeline: the slot loads of tile t+1 are
-// in flight while tile t is computed.  With PRIV the block accumulates its
-// contributions in LDS (ds_add_f64) and flushes them to the run's S with
-// coalesced atomics when it moves to another run; without PRIV every
-// particle adds to S directly.
-template <class M, int BLK, bool PRIV>
+// Loads and stores of the NL consecutive slots of one lane: 8-byte fields
+// move in 16-byte accesses per lane when NL > 1 (aligned: every run's slots
+// start on a tile boundary), 4-byte fields in one access of 4*NL bytes.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int NL>
+__device__ __forceinline__ void ld_lane(const GLOBAL_AS double* p, double (&o)[NL]) {
+  if constexpr (NL == 1) {
+    o[0] = ld_stream(p);
+  } else {
+#pragma unroll
+    for (int h = 0; h < NL / 2; ++h) {
+      const f64x2 v = ld_stream((const GLOBAL_AS f64x2*)p + h);
+      o[2 * h] = v[0];
+      o[2 * h + 1] = v[1];
+    }
+  }
+}
+template <int NL>
+__device__ __forceinline__ void ld_lane(const GLOBAL_AS uint32_t* p, uint32_t (&o)[NL]) {
+  if constexpr (NL == 1) {
+    o[0] = ld_stream(p);
+  } else if constexpr (NL == 2) {
+    const u32x2 v = ld_stream((const GLOBAL_AS u32x2*)p);
+    o[0] = v[0];
+    o[1] = v[1];
+  } else {
+    const u32x4 v = ld_stream((const GLOBAL_AS u32x4*)p);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = v[j];
+  }
+}
+template <int NL>
+__device__ __forceinline__ uint32_t ld_lane(const GLOBAL_AS uint8_t* p) {  // NL status bytes
+  if constexpr (NL == 1)
+    return ld_stream(p);
+  else if constexpr (NL == 2)
+    return ld_stream((const GLOBAL_AS uint16_t*)p);
+  else
+    return ld_stream((const GLOBAL_AS uint32_t*)p);
+}
+template <int NL>
+__device__ __forceinline__ void st_lane(GLOBAL_AS double* p, const double (&v)[NL]) {
+  if constexpr (NL == 1) {
+    st_stream(p, v[0]);
+  } else {
+#pragma unroll
+    for (int h = 0; h < NL / 2; ++h) {
+      f64x2 w;
+      w[0] = v[2 * h];
+      w[1] = v[2 * h + 1];
+      st_stream((GLOBAL_AS f64x2*)p + h, w);
+    }
+  }
+}
+template <int NL>
+__device__ __forceinline__ void st_status(GLOBAL_AS uint8_t* p, uint32_t v) {
+  if constexpr (NL == 1)
+    *p = (uint8_t)v;
+  else if constexpr (NL == 2)
+    *(GLOBAL_AS uint16_t*)p = (uint16_t)v;
+  else
+    *(GLOBAL_AS uint32_t*)p = v;
+}
+
+// The particle pass.  The unit of work is a sub-chunk of SUB = 64*NL slots;
+// lane l holds the slots j*64 + l (ILV: every field access of the wave is a
+// contiguous 64-slot run) or NL*l + j (16-byte accesses per lane).  A
+// sub-chunk lists its movers and its division requests in its own segment of
+// the tile's lists, so sub-chunks are independent: the block's waves take
+// the sub-chunks of one group (one per wave) and groups are dealt to blocks G
+// at a time (b.tile_group; 0: one contiguous range per block).  Which
+// requests a tile accepts (the first `budget` in slot order) is decided
+// after the pass (k_resolve).  With PRIV the block accumulates the
+// contributions of one run in LDS (ds_add_f64), switching runs only between
+// groups; sub-chunks of another run add to S directly.
+//
+// Memory pipeline of a wave, per sub-chunk: issue the next sub-chunk's loads;
+// compute on the current one with its compartment records (gathered one step
+// earlier); gather the records of the next sub-chunk; store the current one.
+// Every load is issued for every lane (slots past the count read valid
+// memory of the tile and are masked where used), so the waits the compiler
+// places for a load's operands do not depend on the data.
+template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE>
 __global__ void __launch_bounds__(BLK)
-    k_particle(BatchDev b, const RunDev* __restrict__ runs, const MapDev* __restrict__ maps,
+    k_particle(BatchDev b, const RunDev* __restrict__ runs, const RunState* __restrict__ rsr,
                const uint32_t* __restrict__ tile_off) {
   constexpr int NV = M::NV, NS = M::NS, REC = rec_stride(M::NS);
+  constexpr int NQ = NS + 2;             // record fields: C_0..C_{n_s-1}, p_move, q_exit
   constexpr int NP = M::NP > 0 ? M::NP : 1;
-  constexpr uint32_t NPT = TILE / BLK;   // slots per thread
-  constexpr uint32_t NW = BLK / 64;      // waves per block
-  extern __shared__ double s_acc[];      // [n_s][n_c] of the current run (PRIV)
-  __shared__ uint32_t s_div[2][NPT][NW];  // double-buffered by tile parity:
-  __shared__ uint32_t s_mov[2][NPT][NW];  // one barrier per tile suffices
-  __shared__ uint32_t s_red[2][4][NW];
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  constexpr uint32_t WPB = BLK / 64;     // waves per block = sub-chunks per group
+  constexpr uint32_t SUB = 64 * NL;       // slots per sub-chunk
+  static_assert(NL == 1 || NL == 2 || NL == 4, "1, 2 or 4 slots per lane");
+  static_assert(TILE / SUB == NSUB, "the lists of the pass are laid out for NSUB sub-chunks per tile");
+  extern __shared__ double s_acc[];      // [n_s][n_c] of run acc_run (PRIV)
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = uniform(tid >> 6);  // wave-uniform: scalar bookkeeping
   const uint32_t stages = b.stages;
   const uint64_t P = b.P;
-  const uint64_t ST = b.ST;
-  const uint32_t stride = b.stage_stride;
   const uint32_t R = b.R;
   const uint32_t T = tile_off[R];
-  const uint32_t chunk = (T + gridDim.x - 1) / gridDim.x;
-  const uint32_t t_begin = blockIdx.x * chunk;
-  const uint32_t t_end = (t_begin + chunk < T) ? t_begin + chunk : T;
-  if (t_begin >= t_end) return;
+  const uint32_t n_sc = T * NSUB;        // sub-chunks of the pass
+  const uint32_t n_groups = (n_sc + WPB - 1) / WPB;
+  const uint32_t chunk = (n_groups + gridDim.x - 1) / gridDim.x;
+  const uint32_t G = b.tile_group ? b.tile_group : (chunk ? chunk : 1u);
+  uint32_t grp = blockIdx.x * G;
+  if (grp >= n_groups) return;  // the whole block
+  auto next_group = [&](uint32_t g) -> uint32_t {
+    const uint32_t g0 = g - g % G;
+    return (g + 1 < g0 + G) ? g + 1 : g0 + gridDim.x * G;
+  };
   const bool do_model = (stages & STAGE_MODEL) != 0;
   const bool do_contrib = (stages & STAGE_CONTRIB) != 0;
 
-  // run whose step/count/budget are cached in scalar registers
-  uint32_t c_run = 0xFFFFFFFFu, c_n = 0, c_step = 0, c_budget = 0;
-  auto run_state = [&](uint32_t r) {
-    if (r != c_run) {
-      const RunState* rs = b.rs + r;
-      c_n = uniform(rs->n);
-      c_step = uniform(rs->step);
-      c_budget = uniform(rs->budget);
-      c_run = r;
+  // one sub-chunk as the wave sees it; sc >= n_sc is a phantom: no slots,
+  // no stores.  cnt counts the sub-chunk's slots below the run's count.
+  struct SubRef {
+    uint32_t sc, t, r, slot0, cnt;  // slot0: run-relative slot of the sub-chunk's first slot
+    uint64_t base;                   // its batch-wide slot
+  };
+  auto sub_ref = [&](uint32_t sc) -> SubRef {
+    SubRef q{sc, 0u, 0u, 0u, 0u, 0ull};
+    if (sc < n_sc) {
+      q.t = sc / NSUB;
+      q.r = find_run(tile_off, R, q.t);
+      q.slot0 = (q.t - tile_off[q.r]) * TILE + (sc % NSUB) * SUB;
+      const uint32_t n = rsr[q.r].n;
+      q.cnt = n > q.slot0 ? (n - q.slot0 < SUB ? n - q.slot0 : SUB) : 0u;
+      q.base = runs[q.r].p_off + q.slot0;
     }
-  };
-
-  // slots of one tile, as loaded
-  struct Slots {
-    uint8_t st[NPT];
-    uint32_t pos[NPT];
-    double x[NPT][NV];
-    double ad[NPT], ah[NPT];
-  };
-  struct TileRef {
-    uint32_t r, slot0, cnt;
-    uint64_t base;
-  };
-  auto tile_ref = [&](uint32_t t) {
-    TileRef q;
-    q.r = find_run(tile_off, R, t);
-    run_state(q.r);
-    q.slot0 = (t - tile_off[q.r]) * TILE;
-    q.cnt = (c_n - q.slot0 < TILE) ? c_n - q.slot0 : TILE;
-    q.base = runs[q.r].p_off + q.slot0;
     return q;
   };
-  auto load = [&](const TileRef& q, Slots& a) {
+  // sub-chunk-relative slot of the lane's j-th slot: interleaved (64 apart,
+  // every field access a contiguous 64-lane run) or NL consecutive slots
+  auto off = [&](int j) -> uint32_t { return ILV ? (uint32_t)j * 64u + lane : NL * lane + (uint32_t)j; };
+  // the NL slots of a lane in one sub-chunk, as loaded
+  struct Sub {
+    uint32_t st;       // NL status bytes
+    uint32_t pos[NL];
+    double x[NV][NL];
+    double ad[NL], ah[NL];
+  };
+  auto load_sub = [&](const SubRef& q, Sub& a) {
     const bool leave = runs[q.r].enable_leave && (stages & STAGE_LEAVE);
-    const GLOBAL_AS uint8_t* st_b = gp(b.status) + q.base;
-    const GLOBAL_AS uint32_t* pos_b = gp(b.pos) + q.base;
-    const GLOBAL_AS double* ad_b = gp(b.age_div) + q.base;
-    const GLOBAL_AS double* ah_b = gp(b.age_hyd) + q.base;
+    if constexpr (ILV) {
+      uint32_t st = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      const uint32_t i = k * BLK + tid;
-      const uint32_t li = i < q.cnt ? i : 0u;  // clamped: no load depends on another
-      const uint8_t stv = ld_stream(st_b + li);
-      a.st[k] = i < q.cnt ? stv : (uint8_t)EXIT;
-      a.pos[k] = ld_stream(pos_b + li);
+      for (int j = 0; j < NL; ++j) {
+        const uint64_t ij = q.base + (q.cnt ? off(j) : 0u);  // an empty sub-chunk reads its first slot
+        st |= (uint32_t)ld_stream(gp(b.status) + ij) << (8 * j);
+        a.pos[j] = ld_stream(gp(b.pos) + ij);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) a.x[k][v] = ld_stream(gp(b.state) + (uint64_t)v * P + q.base + li);
-      a.ad[k] = ld_stream(ad_b + li);
-      a.ah[k] = leave ? ld_stream(ah_b + li) : 0.0;
+        for (int v = 0; v < NV; ++v) a.x[v][j] = ld_stream(gp(b.state) + (uint64_t)v * P + ij);
+        a.ad[j] = ld_stream(gp(b.age_div) + ij);
+        a.ah[j] = ld_stream(gp(b.age_hyd) + (leave ? ij : q.base));
+      }
+      a.st = st;
+    } else {
+      const uint64_t i0 = q.base + (q.cnt ? NL * lane : 0u);
+      a.st = ld_lane<NL>(gp(b.status) + i0);
+      ld_lane<NL>(gp(b.pos) + i0, a.pos);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ld_lane<NL>(gp(b.state) + (uint64_t)v * P + i0, a.x[v]);
+      ld_lane<NL>(gp(b.age_div) + i0, a.ad);
+      ld_lane<NL>(gp(b.age_hyd) + (leave ? i0 : q.base), a.ah);
+    }
+  };
+  auto gather = [&](const SubRef& q, const Sub& a, double (&rr)[NL][NQ]) {
+    const GLOBAL_AS double* __restrict__ rec = gp(runs[q.r].rec);
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      // slots past the count hold no particle: their position may be anything
+      const uint32_t pj = off(j) < q.cnt ? a.pos[j] : 0u;
+      const GLOBAL_AS double* rp = rec + (size_t)pj * REC;
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) rr[j][f] = rp[f];
     }
   };
 
+  // per-wave counters of the run cnt_run, flushed when the wave changes runs
   TileCounts cnt_acc;
+  uint32_t cnt_run = 0xFFFFFFFFu;
   uint32_t acc_run = 0xFFFFFFFFu;
   GLOBAL_AS double* acc_S = nullptr;
   uint32_t acc_len = 0;
 
-  TileRef cur = tile_ref(t_begin);
-  Slots A;
-  load(cur, A);
-  for (uint32_t t = t_begin; t < t_end; ++t) {
-    const uint32_t par = t & 1u;
-    const uint32_t r = cur.r;
-    const RunDev& rd = runs[r];
-    if (r != acc_run) {  // block moves to another run: flush the last one's sums
-      if (PRIV && acc_run != 0xFFFFFFFFu) {
+  SubRef cur = sub_ref(grp * WPB + wave);
+  Sub A;
+  double rr[NL][NQ];
+  load_sub(cur, A);
+  gather(cur, A, rr);
+  // Later sub-chunks enter with their slots copied from the prefetch and
+  // their records in flight; taking the first one's registers through empty
+  // asm statements makes the entry path match.
+  asm volatile("" : "+v"(A.st));
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    asm volatile("" : "+v"(A.ad[j]), "+v"(A.ah[j]), "+v"(A.pos[j]));
+#pragma unroll
+    for (int v = 0; v < NV; ++v) asm volatile("" : "+v"(A.x[v][j]));
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) asm volatile("" : "+v"(rr[j][f]));
+  }
+
+  for (;;) {
+    // ---- block-uniform: the run whose contributions the LDS accumulates
+    if (PRIV) {
+      const uint32_t r0 = find_run(tile_off, R, (grp * WPB) / NSUB);
+      if (r0 != acc_run) {
         __syncthreads();
         for (uint32_t i = tid; i < acc_len; i += BLK) {
           const double v = s_acc[i];
           if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
         }
-      }
-      if (tid == 0 && acc_run != 0xFFFFFFFFu) flush_counts(b.rs + acc_run, cnt_acc);
-      acc_run = r;
-      if (PRIV) {
-        acc_S = gp(rd.S);
-        acc_len = rd.n_s * rd.n_c;
+        acc_run = r0;
+        acc_S = gp(runs[r0].S);
+        acc_len = runs[r0].n_s * runs[r0].n_c;
         __syncthreads();
         for (uint32_t i = tid; i < acc_len; i += BLK) s_acc[i] = 0.0;
         __syncthreads();
       }
     }
-    run_state(r);
-    const uint32_t step = c_step;
-    const uint32_t budget = c_budget;
-    const GLOBAL_AS double* __restrict__ rec = gp(rd.rec);
-
-    // ---- compartment records of the current tile (needs its positions)
-    double rr[NPT][NS + 2];
-#pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      const GLOBAL_AS double* rp = rec + (size_t)A.pos[k] * REC;
-#pragma unroll
-      for (int q = 0; q < NS + 2; ++q) rr[k][q] = rp[q];
+    // ---- the wave's sub-chunk of this group
+    const uint32_t sc = cur.sc;
+    const uint32_t r = cur.r;
+    const RunDev& rd = runs[r];
+    if (cur.cnt && r != cnt_run) {  // wave moves to another run
+      if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
+      cnt_acc = TileCounts{};
+      cnt_run = r;
     }
-    // ---- prefetch the next tile's slots
-    Slots B;
-    TileRef nxt = cur;
-    if (t + 1 < t_end) {
-      nxt = tile_ref(t + 1);
-      load(nxt, B);
-      run_state(r);
-    }
-
-    // ---- per-particle work on the current tile
+    const uint32_t step = rsr[r].step;
     const double dt = rd.dt;
     const double w = rd.weight;
     const uint32_t n_c = rd.n_c;
     const uint64_t seed = rd.seed;
     GLOBAL_AS double* __restrict__ S = gp(rd.S);
+    const bool to_lds = PRIV && r == acc_run;
     double p[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) p[q] = rd.params[q];
     const bool do_move = rd.enable_move && (stages & STAGE_MOVE);
     const bool do_leave = rd.enable_leave && (stages & STAGE_LEAVE);
-    const uint64_t base = cur.base;
-    const uint32_t slot0 = cur.slot0;
-    GLOBAL_AS uint8_t* __restrict__ st_b = gp(b.status) + base;
-    GLOBAL_AS double* __restrict__ ah_b = gp(b.age_hyd) + base;
-    GLOBAL_AS double* __restrict__ ad_b = gp(b.age_div) + base;
-    bool idle[NPT], want[NPT], mover[NPT];
-    uint32_t n_exit = 0, n_dead = 0, n_move = 0, n_proc = 0;
+    const uint32_t g_next = next_group(grp);
+
+    // ---- prefetch: the wave's sub-chunk of the next group (a phantom past
+    // the last group)
+    const SubRef nxt = sub_ref(g_next < n_groups ? g_next * WPB + wave : n_sc);
+    __builtin_amdgcn_sched_barrier(0);
+    Sub B;
+    load_sub(nxt, B);
+    __builtin_amdgcn_sched_barrier(0);
+
+    // ---- per-particle work: arithmetic only
+    const uint32_t i0 = NL * lane;  // contiguous layout: sub-chunk-relative slot of j = 0
+    const bool pair_valid = i0 < cur.cnt;
+    bool valid[NL], idle[NL], want[NL], mover[NL], exited[NL], dead[NL];
+    double x[NL][NV], ad[NL], ah[NL];
+    double contrib[NL][NS];
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      idle[k] = A.st[k] == IDLE;
-      want[k] = false;
-      mover[k] = false;
-      if (!idle[k]) continue;
-      ++n_proc;
-      const uint32_t i = k * BLK + tid;
-      double contrib[NS];
+    for (int j = 0; j < NL; ++j) {
+      const uint32_t stj = (A.st >> (8 * j)) & 0xFFu;
+      valid[j] = off(j) < cur.cnt;
+      idle[j] = valid[j] && stj == IDLE;
+      want[j] = mover[j] = exited[j] = dead[j] = false;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) contrib[s] = 0.0;
+      for (int v = 0; v < NV; ++v) x[j][v] = A.x[v][j];
+      ad[j] = A.ad[j];
+      ah[j] = A.ah[j];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) contrib[j][q] = 0.0;
+      if (!idle[j]) continue;
       int code = IDLE;
       if (do_model) {
-        A.ad[k] += dt;
-        code = M::update(p, dt, A.x[k], rr[k], contrib);
+        ad[j] += dt;
+        code = M::update(p, dt, x[j], rr[j], contrib[j]);
       }
-      if (code == DEAD) {  // a dead cell stops contributing, moving and leaving
-        st_b[i] = DEAD;
-        idle[k] = false;
-        ++n_dead;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) (gp(b.state) + (uint64_t)v * P + base)[i] = A.x[k][v];
-        ad_b[i] = A.ad[k];
-        continue;
-      }
-      want[k] = code == DIVISION;
-      if (do_contrib) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          const double c = w * contrib[s];
-          if (PRIV)
-            atomicAdd(s_acc + s * n_c + A.pos[k], c);
-          else
-            unsafeAtomicAdd((double*)(S + (size_t)s * n_c + A.pos[k]), c);
+      if constexpr (M::CAN_DIE) {
+        if (code == DEAD) {  // a dead cell stops contributing, moving and leaving
+          dead[j] = true;
+          continue;
         }
       }
-      const U4 u = draw4(seed, slot0 + i, step, TAG_TRANSPORT, 0u);
+      want[j] = code == DIVISION;
+      const U4 u = draw4(seed, cur.slot0 + off(j), step, TAG_TRANSPORT, 0u);
       // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:313-325)
-      // is queued; k_move finds its neighbour and runs its exit test
-      mover[k] = do_move && rr[k][NS] > u01(u.x);
-      n_move += mover[k] ? 1u : 0u;
+      // is listed; k_resolve finds its neighbour and runs its exit test
+      mover[j] = do_move && rr[j][NS] > u01(u.x);
       if (do_leave) {
-        A.ah[k] += dt;
-        const double q = rr[k][NS + 1];
-        if (!mover[k] && q < 1.0 && u01(u.z) > q) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
-          if (rd.probe_exit) {
-            const unsigned long long e = atomicAdd(&b.rs[r].probe_exit_n, 1ull);
-            if (e < rd.probe_cap) rd.probe_exit[e] = A.ah[k];
+        ah[j] += dt;
+        const double qx = rr[j][NS + 1];
+        if (!mover[j] && qx < 1.0 && u01(u.z) > qx) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
+          if constexpr (PROBE) {
+            if (rd.probe_exit) {
+              const unsigned long long e = atomicAdd(&b.rs[r].probe_exit_n, 1ull);
+              if (e < rd.probe_cap) rd.probe_exit[e] = ah[j];
+            }
           }
-          A.ah[k] = 0.0;
-          st_b[i] = EXIT;
-          ++n_exit;
+          ah[j] = 0.0;
+          exited[j] = true;
         }
-        st_stream(ah_b + i, A.ah[k]);
       }
-      if (!want[k]) {  // final state now; dividers are written after ranking
+    }
+    // contributions of the Idle particles (a dead cell contributes nothing)
+    if (do_contrib) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) st_stream(gp(b.state) + (uint64_t)v * P + base + i, A.x[k][v]);
-        if (do_model) st_stream(ad_b + i, A.ad[k]);
+      for (int j = 0; j < NL; ++j) {
+        if (!idle[j] || dead[j]) continue;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const double c = w * contrib[j][q];
+          if (to_lds)
+            atomicAdd(s_acc + q * n_c + A.pos[j], c);
+          else
+            unsafeAtomicAdd((double*)(S + (size_t)q * n_c + A.pos[j]), c);
+        }
       }
     }
 
-    // ---- rank the division requests of the tile in slot order
-    uint64_t mk[NPT], mv[NPT];
+    // ---- the records of the next sub-chunk (its positions have arrived)
+    double rn[NL][NQ];
+    gather(nxt, B, rn);
+
+    // ---- stores: the lane's final state, status changes, and the
+    // sub-chunk's movers and division requests in slot order
+    const uint64_t gi = cur.base + i0;
+    if constexpr (ILV) {
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      mk[k] = __ballot(want[k]);
-      mv[k] = __ballot(mover[k]);
-      if (lane == 0) {
-        s_div[par][k][wave] = (uint32_t)__popcll(mk[k]);
-        s_mov[par][k][wave] = (uint32_t)__popcll(mv[k]);
+      for (int j = 0; j < NL; ++j) {
+        if (!valid[j]) continue;
+        const uint64_t gj = cur.base + off(j);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) st_stream(gp(b.state) + (uint64_t)v * P + gj, x[j][v]);
+        st_stream(gp(b.age_div) + gj, ad[j]);
+        if (do_leave) st_stream(gp(b.age_hyd) + gj, ah[j]);
+        if (exited[j] || dead[j]) gp(b.status)[gj] = dead[j] ? (uint8_t)DEAD : (uint8_t)EXIT;
       }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double xv[NL];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) xv[j] = x[j][v];
+        if (pair_valid) st_lane<NL>(gp(b.state) + (uint64_t)v * P + gi, xv);
+      }
+      if (pair_valid) st_lane<NL>(gp(b.age_div) + gi, ad);
+      if (do_leave && pair_valid) st_lane<NL>(gp(b.age_hyd) + gi, ah);
+      uint32_t stn = A.st;
+      bool changed = false;
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        const uint32_t nv = dead[j] ? (uint32_t)DEAD : exited[j] ? (uint32_t)EXIT : ((A.st >> (8 * j)) & 0xFFu);
+        stn = (stn & ~(0xFFu << (8 * j))) | (nv << (8 * j));
+        changed = changed || exited[j] || dead[j];
+      }
+      if (changed && pair_valid) st_status<NL>(gp(b.status) + gi, stn);
     }
-    n_exit = wave_sum(n_exit);
-    n_dead = wave_sum(n_dead);
-    n_move = wave_sum(n_move);
-    n_proc = wave_sum(n_proc);
-    if (lane == 0) {
-      s_red[par][0][wave] = n_exit;
-      s_red[par][1][wave] = n_dead;
-      s_red[par][2][wave] = n_move;
-      s_red[par][3][wave] = n_proc;
-    }
-    __syncthreads();
-    uint32_t total = 0;
+    const uint64_t lt = lanemask_lt(lane);
+    const uint64_t seg = (uint64_t)sc * SUB;  // the sub-chunk's list segment
+    const uint32_t slot_t = cur.slot0 % TILE;  // tile-relative slot of the sub-chunk's first slot
+    uint32_t n_mov = 0, n_div = 0;
+    {
+      uint64_t mv[NL], wd[NL];
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k)
+      for (int j = 0; j < NL; ++j) {
+        mv[j] = __ballot(mover[j]);
+        wd[j] = __ballot(want[j]);
+      }
+      // rank of slot off(j) among the flagged slots of the sub-chunk
+      uint32_t rm[NL], rd_[NL];
+      if constexpr (ILV) {
+        uint32_t am = 0, adv = 0;
 #pragma unroll
-      for (uint32_t wv = 0; wv < NW; ++wv) total += s_div[par][k][wv];
-    const uint32_t births = total < budget ? total : budget;
-    if (total) {
-      uint32_t before = 0;  // requests in earlier k-rows
-#pragma unroll
-      for (uint32_t k = 0; k < NPT; ++k) {
-        uint32_t rank = before + (uint32_t)__popcll(mk[k] & lanemask_lt(lane));
-#pragma unroll
-        for (uint32_t wv = 0; wv < NW; ++wv) {
-          rank += (wv < wave) ? s_div[par][k][wv] : 0u;
-          before += s_div[par][k][wv];
+        for (int j = 0; j < NL; ++j) {
+          rm[j] = am + (uint32_t)__popcll(mv[j] & lt);
+          rd_[j] = adv + (uint32_t)__popcll(wd[j] & lt);
+          am += (uint32_t)__popcll(mv[j]);
+          adv += (uint32_t)__popcll(wd[j]);
         }
-        if (!want[k]) continue;
-        const uint32_t i = k * BLK + tid;
-        if (rank < budget) {
-          if (rd.probe_div) {
-            const unsigned long long e = atomicAdd(&b.rs[r].probe_div_n, 1ull);
-            if (e < rd.probe_cap) rd.probe_div[e] = A.ad[k];
-          }
-          double xd[NV];
-          M::division(p, A.x[k], xd, RngKey{seed, slot0 + i, step});
-          const uint64_t sidx = (uint64_t)t * stride + rank;
+        n_mov = am;
+        n_div = adv;
+      } else {
+        uint32_t bm = 0, bd = 0;
 #pragma unroll
-          for (int v = 0; v < NV; ++v) gp(b.st_state)[(uint64_t)v * ST + sidx] = xd[v];
-          gp(b.st_pos)[sidx] = A.pos[k];
-          A.ad[k] = 0.0;
+        for (int j = 0; j < NL; ++j) {
+          bm += (uint32_t)__popcll(mv[j] & lt);
+          bd += (uint32_t)__popcll(wd[j] & lt);
+          n_mov += (uint32_t)__popcll(mv[j]);
+          n_div += (uint32_t)__popcll(wd[j]);
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) (gp(b.state) + (uint64_t)v * P + base)[i] = A.x[k][v];
-        if (do_model) ad_b[i] = A.ad[k];
+        for (int j = 0; j < NL; ++j) {
+          rm[j] = bm;
+          rd_[j] = bd;
+          bm += mover[j] ? 1u : 0u;
+          bd += want[j] ? 1u : 0u;
+        }
       }
-    }
-    // ---- the tile's movers, in slot order, for k_move
-    uint32_t n_moves = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      uint32_t rank = n_moves + (uint32_t)__popcll(mv[k] & lanemask_lt(lane));
+      for (int j = 0; j < NL; ++j) {
+        if (mover[j]) {
+          gp(b.move_slot)[seg + rm[j]] = (uint16_t)(slot_t + off(j));
+          gp(b.move_pos)[seg + rm[j]] = A.pos[j];
+        }
+        if (n_div && want[j]) {  // rare
+          gp(b.div_slot)[seg + rd_[j]] = (uint16_t)(slot_t + off(j));
+          gp(b.div_pos)[seg + rd_[j]] = A.pos[j];
+        }
+      }
+    }
+    if (lane == 0 && sc < n_sc) {
+      gp(b.sub_moves)[sc] = n_mov;
+      gp(b.sub_divs)[sc] = n_div;
+    }
+    // ---- counters of the sub-chunk (wave-uniform)
 #pragma unroll
-      for (uint32_t wv = 0; wv < NW; ++wv) {
-        rank += (wv < wave) ? s_mov[par][k][wv] : 0u;
-        n_moves += s_mov[par][k][wv];
-      }
-      if (mover[k]) {
-        b.move_slot[(uint64_t)t * TILE + rank] = (uint16_t)(k * BLK + tid);
-        b.move_pos[(uint64_t)t * TILE + rank] = A.pos[k];
-      }
+    for (int j = 0; j < NL; ++j) {
+      cnt_acc.processed += (uint32_t)__popcll(__ballot(idle[j]));
+      cnt_acc.exits += (uint32_t)__popcll(__ballot(exited[j]));
+      cnt_acc.deaths += (uint32_t)__popcll(__ballot(dead[j]));
     }
-    if (tid == 0) {
-      b.tile_moves[t] = n_moves;
-      b.tile_births[t] = births;
-      for (uint32_t wv = 0; wv < NW; ++wv) {
-        cnt_acc.exits += s_red[par][0][wv];
-        cnt_acc.deaths += s_red[par][1][wv];
-        cnt_acc.moves += s_red[par][2][wv];
-        cnt_acc.processed += s_red[par][3][wv];
-      }
-      cnt_acc.overflow += total - births;
-    }
+    cnt_acc.moves += n_mov;
+
+    __builtin_amdgcn_sched_barrier(0);
     cur = nxt;
+    A = B;
 #pragma unroll
-    for (uint32_t k = 0; k < NPT; ++k) {
-      A.st[k] = B.st[k];
-      A.pos[k] = B.pos[k];
+    for (int j = 0; j < NL; ++j)
 #pragma unroll
-      for (int v = 0; v < NV; ++v) A.x[k][v] = B.x[k][v];
-      A.ad[k] = B.ad[k];
-      A.ah[k] = B.ah[k];
-    }
+      for (int f = 0; f < NQ; ++f) rr[j][f] = rn[j][f];
+    grp = g_next;
+    if (grp >= n_groups) break;
   }
   if (PRIV) {
     __syncthreads();
@@ -541,49 +664,94 @@ __global__ void __launch_bounds__(BLK)
       if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
     }
   }
-  if (tid == 0) flush_counts(b.rs + acc_run, cnt_acc);
+  if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
 }
 
-// ------------------------------------------------------------------ moves
-// Neighbour pick for the particles the pass listed: the first CDF column at or
-// above u, then the exit test in the new compartment (move_kernel.hpp:392-437,
-// 586-648).  A block takes MOVE_GROUP tiles and spreads their movers over its
-// threads; every mover is independent, so the processing order is free.
+// After the particle pass, per group of RES_SC sub-chunks (one per lane of
+// the first wave): (1) the movers the pass listed take their neighbour, the
+// first CDF column at or above u, and run the exit test in the new
+// compartment (move_kernel.hpp:392-437, 586-648); (2) each tile accepts the
+// first `budget` division requests in slot order (the reference's division
+// buffer, model_kernel.hpp:163-268), halves the mother and stages the
+// newborn at its rank.  Every mover and every request is independent, so
+// the processing order is free.
+constexpr uint32_t RES_SC = 64;
 
-__global__ void __launch_bounds__(256) k_move(BatchDev b, const RunDev* __restrict__ runs,
-                                              const MapDev* __restrict__ maps,
-                                              const uint32_t* __restrict__ tile_off) {
-  __shared__ uint32_t s_pre[MOVE_GROUP + 1];
-  __shared__ uint32_t s_run[MOVE_GROUP];
-  const uint32_t T = tile_off[b.R];
-  const uint32_t n_groups = (T + MOVE_GROUP - 1) / MOVE_GROUP;
+template <class M>
+__global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
+                                                 const MapDev* __restrict__ maps,
+                                                 const uint32_t* __restrict__ tile_off) {
+  constexpr int NV = M::NV;
+  constexpr int NP = M::NP > 0 ? M::NP : 1;
+  __shared__ uint32_t s_pre[RES_SC + 1];  // movers: prefix over the group's sub-chunks
+  __shared__ uint32_t s_run[RES_SC];
+  __shared__ uint32_t s_dcnt[RES_SC];     // division requests per sub-chunk
+  __shared__ uint32_t s_dpre[RES_SC];     // ... their prefix within the tile
+  __shared__ uint32_t s_births[RES_SC];   // accepted requests per tile of the group
+  __shared__ uint32_t s_any;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t R = b.R;
+  const uint32_t nsub = b.nsub;          // sub-chunks per tile (divides RES_SC)
+  const uint32_t sub = TILE / nsub;      // slots per sub-chunk
+  const uint32_t T = tile_off[R];
+  const uint32_t n_sc = T * nsub;
+  const uint32_t tpg = RES_SC / nsub;    // tiles per group
+  const uint32_t n_groups = (n_sc + RES_SC - 1) / RES_SC;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
   for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
-    const uint32_t t0 = grp * MOVE_GROUP;
-    if (threadIdx.x < 64) {  // the group's tiles, one lane each
-      const uint32_t lane = threadIdx.x, t = t0 + lane;
-      uint32_t cnt = 0, r = 0;
-      if (lane < MOVE_GROUP && t < T) {
-        cnt = b.tile_moves[t];
-        r = find_run(tile_off, b.R, t);
+    const uint32_t sc0 = grp * RES_SC;
+    const uint32_t t0 = sc0 / nsub;
+    if (tid < 64) {  // the group's sub-chunks, one lane each
+      const uint32_t lane = tid, sc = sc0 + lane;
+      uint32_t cm = 0, cd = 0, r = 0;
+      if (sc < n_sc) {
+        cm = b.sub_moves[sc];
+        cd = b.sub_divs[sc];
+        r = find_run(tile_off, R, sc / nsub);
       }
-      const uint32_t inc = wave_incl_scan(cnt, lane);
-      if (lane < MOVE_GROUP) {
-        s_pre[lane] = inc - cnt;
-        s_run[lane] = r;
+      const uint32_t inc = wave_incl_scan(cm, lane);
+      s_pre[lane] = inc - cm;
+      if (lane == RES_SC - 1) s_pre[RES_SC] = inc;
+      s_run[lane] = r;
+      s_dcnt[lane] = cd;
+      // requests: prefix within each tile (segments of nsub lanes)
+      const uint32_t incd = wave_incl_scan(cd, lane);
+      const uint32_t first = lane - lane % nsub;
+      const uint32_t before = __shfl(incd, first ? first - 1 : 0, 64);
+      const uint32_t seg_base = first ? before : 0u;
+      s_dpre[lane] = incd - cd - seg_base;
+      if (lane % nsub == nsub - 1) {  // the tile's births: the first `budget` requests
+        const uint32_t t = t0 + lane / nsub;
+        const uint32_t tot = incd - seg_base;
+        uint32_t births = 0;
+        if (t < T) {
+          RunState* rs = b.rs + r;
+          const uint32_t budget = rs->budget;
+          births = tot < budget ? tot : budget;
+          b.tile_births[t] = births;
+          if (tot > births) atomicAdd(&rs->step_overflow, (unsigned long long)(tot - births));
+        }
+        s_births[lane / nsub] = births;
       }
-      if (lane == MOVE_GROUP - 1) s_pre[MOVE_GROUP] = inc;
+      const uint64_t any = __ballot(cd != 0);
+      if (lane == 0) s_any = any != 0 ? 1u : 0u;
     }
     __syncthreads();
-    const uint32_t total = s_pre[MOVE_GROUP];
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-      uint32_t j = 0;
-      while (s_pre[j + 1] <= e) ++j;
-      const uint32_t t = t0 + j;
+    // ---- movers
+    const uint32_t total = s_pre[RES_SC];
+    for (uint32_t e = tid; e < total; e += blockDim.x) {
+      uint32_t lo = 0, hi = RES_SC;  // the sub-chunk j with s_pre[j] <= e < s_pre[j+1]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= e) lo = mid; else hi = mid;
+      }
+      const uint32_t j = lo;
+      const uint32_t sc = sc0 + j;
+      const uint32_t t = sc / nsub;
       const uint32_t r = s_run[j];
       const RunDev& rd = runs[r];
       RunState* rs = b.rs + r;
-      const uint64_t q = (uint64_t)t * TILE + (e - s_pre[j]);
+      const uint64_t q = (uint64_t)sc * sub + (e - s_pre[j]);
       const uint32_t slot = (t - tile_off[r]) * TILE + b.move_slot[q];
       uint32_t pos = b.move_pos[q];
       const uint64_t g = rd.p_off + slot;
@@ -605,6 +773,44 @@ __global__ void __launch_bounds__(256) k_move(BatchDev b, const RunDev* __restri
           b.age_hyd[g] = 0.0;
           b.status[g] = EXIT;
           atomicAdd(&rs->step_exits, 1ull);
+        }
+      }
+    }
+    // ---- divisions (rare)
+    if (s_any) {
+      for (uint32_t k = 0; k < tpg; ++k) {
+        const uint32_t t = t0 + k;
+        const uint32_t births = s_births[k];
+        if (t >= T || births == 0) continue;
+        const uint32_t r = s_run[k * nsub];
+        const RunDev& rd = runs[r];
+        RunState* rs = b.rs + r;
+        const uint32_t step = rs->step;
+        double p[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) p[i] = rd.params[i];
+        for (uint32_t e = tid; e < births; e += blockDim.x) {
+          uint32_t j = k * nsub;  // the tile's sub-chunk holding request e
+          while (j + 1 < (k + 1) * nsub && s_dpre[j + 1] <= e) ++j;
+          const uint64_t q = (uint64_t)(sc0 + j) * sub + (e - s_dpre[j]);
+          const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
+          const uint64_t g = rd.p_off + slot;
+          double x[NV], xd[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
+          if (rd.probe_div) {
+            const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
+            if (i < rd.probe_cap) rd.probe_div[i] = b.age_div[g];
+          }
+          M::division(p, x, xd, RngKey{rd.seed, slot, step});
+          const uint64_t sidx = (uint64_t)t * b.stage_stride + e;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            b.state[(uint64_t)v * b.P + g] = x[v];
+            b.st_state[(uint64_t)v * b.ST + sidx] = xd[v];
+          }
+          b.age_div[g] = 0.0;
+          b.st_pos[sidx] = b.div_pos[q];
         }
       }
     }
@@ -892,32 +1098,43 @@ __global__ void __launch_bounds__(256) k_repartition(BatchDev b, uint32_t r, uns
 }
 
 // ------------------------------------------------------------------ launchers
-template <class M, int BLK, bool PRIV>
+template <class M, int BLK, bool PRIV, bool PROBE>
 static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_particle<M, BLK, PRIV>), dim3(grid), dim3(BLK), lds, s, b, b.runs, b.maps,
-                     (const uint32_t*)b.tile_off);
+  hipLaunchKernelGGL((k_particle<M, BLK, LANE_SLOTS, LANE_INTERLEAVE, PRIV, PROBE>), dim3(grid), dim3(BLK), lds, s, b, b.runs,
+                     (const RunState*)b.rs, (const uint32_t*)b.tile_off);
 }
 
 template <class M>
-static const void* kernel_ptr_t(int blk, bool priv) {
+static const void* kernel_ptr_t(int blk, bool priv, bool probe) {
+  if (probe) return priv ? (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, true, true> : (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, false, true>;
   if (priv) {
-    if (blk == 256) return (const void*)&k_particle<M, 256, true>;
-    if (blk == 512) return (const void*)&k_particle<M, 512, true>;
-    return (const void*)&k_particle<M, 1024, true>;
+    if (blk == 256) return (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, true, false>;
+    if (blk == 512) return (const void*)&k_particle<M, 512, LANE_SLOTS, LANE_INTERLEAVE, true, false>;
+    return (const void*)&k_particle<M, 1024, LANE_SLOTS, LANE_INTERLEAVE, true, false>;
   }
-  return (const void*)&k_particle<M, 256, false>;
+  return (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, false, false>;
 }
 
+// Runs with probes take a 256-thread variant that records exit and division
+// ages; the atomics it needs (with returned values) are kept out of the
+// production variants, where they would make every wait for a load in the
+// tile loop a wait for all outstanding memory operations.
 template <class M>
 static void launch_t(const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
-  if (!L.priv)
-    launch_particle_t<M, 256, false>(b, L.grid, 0, s);
-  else if (L.block == 256)
-    launch_particle_t<M, 256, true>(b, L.grid, L.lds, s);
-  else if (L.block == 512)
-    launch_particle_t<M, 512, true>(b, L.grid, L.lds, s);
-  else
-    launch_particle_t<M, 1024, true>(b, L.grid, L.lds, s);
+  if (L.probe) {
+    if (L.priv)
+      launch_particle_t<M, 256, true, true>(b, L.grid, L.lds, s);
+    else
+      launch_particle_t<M, 256, false, true>(b, L.grid, 0, s);
+  } else if (!L.priv) {
+    launch_particle_t<M, 256, false, false>(b, L.grid, 0, s);
+  } else if (L.block == 256) {
+    launch_particle_t<M, 256, true, false>(b, L.grid, L.lds, s);
+  } else if (L.block == 512) {
+    launch_particle_t<M, 512, true, false>(b, L.grid, L.lds, s);
+  } else {
+    launch_particle_t<M, 1024, true, false>(b, L.grid, L.lds, s);
+  }
 }
 
 void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
@@ -931,13 +1148,13 @@ void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, h
   }
 }
 
-const void* particle_kernel_ptr(int model_id, int block, bool priv) {
+const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe) {
   switch (model_id) {
-    case MonodMass::ID: return kernel_ptr_t<MonodMass>(block, priv);
-    case DualMonod::ID: return kernel_ptr_t<DualMonod>(block, priv);
-    case Monod::ID: return kernel_ptr_t<Monod>(block, priv);
-    case FixedLength::ID: return kernel_ptr_t<FixedLength>(block, priv);
-    case SimpleAcetate::ID: return kernel_ptr_t<SimpleAcetate>(block, priv);
+    case MonodMass::ID: return kernel_ptr_t<MonodMass>(block, priv, probe);
+    case DualMonod::ID: return kernel_ptr_t<DualMonod>(block, priv, probe);
+    case Monod::ID: return kernel_ptr_t<Monod>(block, priv, probe);
+    case FixedLength::ID: return kernel_ptr_t<FixedLength>(block, priv, probe);
+    case SimpleAcetate::ID: return kernel_ptr_t<SimpleAcetate>(block, priv, probe);
     default: return nullptr;
   }
 }
@@ -957,8 +1174,16 @@ void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) 
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_merge, dim3(grid), dim3(BLOCK), 0, s, b, n_var);
 }
-void launch_move(const BatchDev& b, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_move, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, (const uint32_t*)b.tile_off);
+void launch_resolve(int model_id, const BatchDev& b, int grid, hipStream_t s) {
+  const uint32_t* to = b.tile_off;
+  switch (model_id) {
+    case MonodMass::ID: hipLaunchKernelGGL(k_resolve<MonodMass>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
+    case DualMonod::ID: hipLaunchKernelGGL(k_resolve<DualMonod>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
+    case Monod::ID: hipLaunchKernelGGL(k_resolve<Monod>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
+    case FixedLength::ID: hipLaunchKernelGGL(k_resolve<FixedLength>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
+    case SimpleAcetate::ID: hipLaunchKernelGGL(k_resolve<SimpleAcetate>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
+    default: break;
+  }
 }
 void launch_layout(const BatchDev& b, hipStream_t s) {
   hipLaunchKernelGGL(k_layout, dim3(1), dim3(SCAN_THREADS), 0, s, b);

@@ -9,20 +9,22 @@ namespace mcst {
 
 void launch_transport(const BatchDev& b, int grid, hipStream_t s);
 // Launch shape of the particle pass: block size (256/512/1024 threads per
-// 1024-slot tile), grid, and the LDS bytes of the per-block contribution
-// accumulator (priv) or none (direct global atomics).
+// 1024-slot tile), grid, the LDS bytes of the per-block contribution
+// accumulator (priv) or none (direct global atomics), and whether some run
+// records exit and division ages (probe: a 256-thread variant).
 struct ParticleLaunch {
   int block = 256;
   int grid = 1;
   bool priv = false;
+  bool probe = false;
   size_t lds = 0;
 };
 
 void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
-const void* particle_kernel_ptr(int model_id, int block, bool priv);
+const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe);
 void launch_bookkeep(const BatchDev& b, hipStream_t s);
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
-void launch_move(const BatchDev& b, int grid, hipStream_t s);
+void launch_resolve(int model_id, const BatchDev& b, int grid, hipStream_t s);
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_layout(const BatchDev& b, hipStream_t s);
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s);

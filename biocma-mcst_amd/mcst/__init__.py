@@ -170,7 +170,7 @@ class Engine:
         _ck(lib().mcst_engine_synchronize(self.h), "synchronize")
 
     def profile(self, n_steps):
-        """(total ms, per-kernel mean ms: transport+layout, particle, move, bookkeep, removal, merge)."""
+        """(total ms, per-kernel mean ms: transport+layout, particle, resolve, bookkeep, removal, merge)."""
         tot = C.c_double()
         per = np.zeros(6)
         _ck(lib().mcst_engine_profile(self.h, n_steps, C.byref(tot), per), "profile")

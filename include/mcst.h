@@ -149,7 +149,7 @@ int mcst_engine_replay(mcst_engine* e, uint32_t times);
 int mcst_engine_synchronize(mcst_engine* e);
 /* Time n steps with HIP events on the engine stream: total ms and the mean
  * duration of each step kernel, in the order transport (with the tile
- * layout), particle pass, mover resolution, bookkeeping, removal (4 kernels
+ * layout), particle pass, mover and division resolution, bookkeeping, removal (4 kernels
  * together, skipped when no particle can become inactive), merge
  * -> per_kernel_ms[6]. */
 int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, double* per_kernel_ms);
