@@ -211,8 +211,11 @@ struct HostRun {
   std::vector<double> ah, ad;
   double weight = 0.0;
   // device-side placement
-  uint64_t p_off = 0;
-  uint32_t comp_off = 0, tile_cap_off = 0, map_base = 0;
+  int phase = 0;                 // 0 defined, 1 on the device, 2 retired
+  uint64_t p_off = 0;            // first slot (a multiple of TILE)
+  uint32_t tiles = 0;            // slot range in tiles: [p_off/TILE, p_off/TILE + tiles)
+  uint32_t map_base = 0;
+  std::vector<void*> dmem;       // the run's own device arrays
 };
 
 }  // namespace
@@ -224,12 +227,23 @@ struct mcst_engine {
   hipStream_t stream = nullptr;
   std::vector<mcst::HostRun> runs;
   bool committed = false;
+  // capacity reserved for runs admitted after commit (0: sized by the runs at commit)
+  uint64_t reserve_slots = 0;
+  uint32_t reserve_runs = 0;
   // device
-  std::vector<void*> allocs;
+  std::vector<void*> allocs;        // engine-wide arrays, freed at destroy
   mcst::BatchDev bd{};
   mcst::RunDev* d_runs = nullptr;
   mcst::RunState* d_rs = nullptr;
+  uint32_t runs_cap = 0;            // entries of d_runs / d_rs / comp_off / tile_off
+  uint32_t admitted = 0;            // runs [0, admitted) have been admitted
   std::vector<mcst::RunDev> h_runs;
+  std::vector<std::pair<uint32_t, uint32_t>> free_tiles;  // free slot ranges of the pool, in tiles
+  std::vector<mcst::MapDev> h_maps;
+  mcst::MapDev* d_maps = nullptr;
+  uint32_t maps_cap = 0;
+  std::vector<uint32_t> comp_off;
+  uint32_t* d_comp = nullptr;
   int grid_particle = 1, grid_transport = 1;
   mcst::ParticleLaunch plaunch;
   int grid_move = 1;
@@ -237,7 +251,6 @@ struct mcst_engine {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_steps = 0;
-  std::vector<uint32_t> host_step;  // steps launched per run (host mirror)
 };
 
 namespace mcst {
@@ -259,13 +272,76 @@ int dev_upload(mcst_engine* e, T** p, const T* src, size_t count) {
   return MCST_OK;
 }
 
-void release(mcst_engine* e) {
+// Allocation of one run's own arrays (freed when the run is retired).
+int run_alloc(HostRun& h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t err = hipMalloc(p, bytes);
+  if (err != hipSuccess) return fail(MCST_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(err));
+  h.dmem.push_back(*p);
+  return MCST_OK;
+}
+
+template <class T>
+int run_upload(HostRun& h, T** p, const T* src, size_t count) {
+  int rc = run_alloc(h, (void**)p, count * sizeof(T));
+  if (rc) return rc;
+  if (count) HIP_TRY(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return MCST_OK;
+}
+
+void free_run_memory(HostRun& h) {
+  for (void* p : h.dmem) (void)hipFree(p);
+  h.dmem.clear();
+}
+
+void drop_graph(mcst_engine* e) {
   if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   e->graph_exec = nullptr;
   e->graph = nullptr;
+  e->graph_steps = 0;
+}
+
+void release(mcst_engine* e) {
+  drop_graph(e);
+  for (auto& h : e->runs) free_run_memory(h);
   for (void* p : e->allocs) (void)hipFree(p);
   e->allocs.clear();
+  e->d_maps = nullptr;
+  e->maps_cap = 0;
+}
+
+// First-fit range of `n` tiles from the pool's free list; false if none.
+bool take_tiles(mcst_engine* e, uint32_t n, uint32_t& start) {
+  if (n == 0) {
+    start = 0;
+    return true;
+  }
+  for (size_t i = 0; i < e->free_tiles.size(); ++i) {
+    auto& f = e->free_tiles[i];
+    if (f.second < n) continue;
+    start = f.first;
+    f.first += n;
+    f.second -= n;
+    if (f.second == 0) e->free_tiles.erase(e->free_tiles.begin() + (long)i);
+    return true;
+  }
+  return false;
+}
+
+void give_tiles(mcst_engine* e, uint32_t start, uint32_t n) {
+  if (n == 0) return;
+  auto& fl = e->free_tiles;
+  fl.push_back({start, n});
+  std::sort(fl.begin(), fl.end());
+  std::vector<std::pair<uint32_t, uint32_t>> merged;
+  for (const auto& f : fl) {
+    if (!merged.empty() && merged.back().first + merged.back().second == f.first)
+      merged.back().second += f.second;
+    else
+      merged.push_back(f);
+  }
+  fl.swap(merged);
 }
 
 HostRun* get_run(mcst_engine* e, int32_t run) {
@@ -298,185 +374,35 @@ void exit_thresholds(const HostRun& r, HostMap& mp) {
   }
 }
 
-int commit(mcst_engine* e) {
+int check_definable(const mcst_engine* e, const HostRun& h, uint32_t r) {
+  if (h.maps.empty()) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no flowmap");
+  if (!h.has_particles) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no particles");
+  for (const auto& fd : h.feeds)
+    if (fd.kind != MCST_FEED_CONSTANT)
+      return fail(MCST_ERR_ARG, "time-dependent feeds are not supported on the device path");
+  if (h.stage_cap > e->bd.stage_stride)
+    return fail(MCST_ERR_ARG, "division cap above the engine's staging rows");
+  return MCST_OK;
+}
+
+// Launch shapes for the runs on the device: LDS contribution accumulator
+// when the largest live run's [n_s][n_c] fits, then the block size with the
+// most resident waves; a grid of resident blocks.  MCST_PARTICLE_BLOCK
+// (256/512/1024), MCST_CONTRIB_LDS (0/1) and MCST_TILE_GROUP override.
+int choose_launch(mcst_engine* e) {
   const ModelEntry& M = *e->model;
-  const uint32_t R = (uint32_t)e->runs.size();
-  if (R == 0) return fail(MCST_ERR_STATE, "no run defined");
-  uint64_t P = 0;
-  uint64_t T_max = 0;
-  uint32_t stride = 1, n_comp = 0, n_maps = 0;
-  for (uint32_t r = 0; r < R; ++r) {
-    HostRun& h = e->runs[r];
-    if (h.maps.empty()) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no flowmap");
-    if (!h.has_particles) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no particles");
-    for (const auto& fd : h.feeds)
-      if (fd.kind != MCST_FEED_CONSTANT)
-        return fail(MCST_ERR_ARG, "time-dependent feeds are not supported on the device path");
-    h.p_off = P;
-    h.tile_cap_off = (uint32_t)T_max;
-    h.comp_off = n_comp;
-    h.map_base = n_maps;
-    // each run's slots start on a tile boundary: the particle pass moves
-    // whole tiles, so the padding after cap is addressable but never used
-    P += (h.cap + TILE - 1) / TILE * TILE;
-    T_max += (h.cap + TILE - 1) / TILE;
-    stride = std::max(stride, h.stage_cap);
-    n_comp += h.n_c;
-    n_maps += (uint32_t)h.maps.size();
-  }
-  if (P >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "more than 2^32-1 particle slots in one engine");
-  if (T_max == 0) T_max = 1;
-  const uint32_t nv = (uint32_t)M.nv;
   BatchDev& b = e->bd;
-  b = BatchDev{};
-  b.P = P;
-  b.T_max = (uint32_t)T_max;
-  b.stage_stride = stride;
-  b.ST = T_max * stride;
-  b.R = R;
-  b.n_comp_total = n_comp;
-  b.nsub = NSUB;
-  b.ns_max = std::max<uint32_t>(1, (uint32_t)M.ns);  // one model per engine; thread s == 0 also writes the map fields
-  b.stages = STAGE_ALL;
-  int rc;
-  if ((rc = dev_alloc(e, (void**)&b.pos, P * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.status, P))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.state, P * nv * sizeof(double)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.age_hyd, P * sizeof(double)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.age_div, P * sizeof(double)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.st_state, b.ST * nv * sizeof(double)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.st_pos, b.ST * sizeof(uint32_t)))) return rc;
-  uint32_t** tile_arrays[] = {&b.tile_births, &b.tile_birth_off, &b.tile_gap, &b.tile_fill, &b.tile_gap_off,
-                              &b.tile_fill_off};
-  for (uint32_t** a : tile_arrays)
-    if ((rc = dev_alloc(e, (void**)a, T_max * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.tile_off, (R + 1) * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.move_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * NSUB * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * NSUB * sizeof(uint32_t)))) return rc;
-  HIP_TRY(hipMemsetAsync(b.tile_births, 0, T_max * sizeof(uint32_t), e->stream));
-  HIP_TRY(hipMemsetAsync(b.tile_off, 0, (R + 1) * sizeof(uint32_t), e->stream));
-  HIP_TRY(hipMemsetAsync(b.sub_moves, 0, T_max * NSUB * sizeof(uint32_t), e->stream));
-  HIP_TRY(hipMemsetAsync(b.sub_divs, 0, T_max * NSUB * sizeof(uint32_t), e->stream));
-
-  // particles
-  for (uint32_t r = 0; r < R; ++r) {
-    const HostRun& h = e->runs[r];
-    if (h.n == 0) continue;
-    HIP_TRY(hipMemcpy(b.pos + h.p_off, h.pos.data(), h.n * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b.status + h.p_off, h.status.data(), h.n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b.age_hyd + h.p_off, h.ah.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b.age_div + h.p_off, h.ad.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
-    for (uint32_t v = 0; v < nv; ++v)
-      HIP_TRY(hipMemcpy(b.state + (uint64_t)v * P + h.p_off, h.state.data() + (size_t)v * h.n,
-                        h.n * sizeof(double), hipMemcpyHostToDevice));
-  }
-
-  // maps, liquid state, run descriptors
-  std::vector<MapDev> maps;
-  std::vector<uint32_t> comp_off;
-  e->h_runs.assign(R, RunDev{});
-  std::vector<RunState> rs(R);
-  for (uint32_t r = 0; r < R; ++r) {
-    HostRun& h = e->runs[r];
-    RunDev& d = e->h_runs[r];
-    comp_off.push_back(h.comp_off);
-    for (auto& mp : h.maps) {
-      exit_thresholds(h, mp);
-      MapDev md{};
-      md.m = mp.m;
-      if ((rc = dev_upload(e, (double**)&md.p_move, mp.p_move.data(), mp.p_move.size()))) return rc;
-      if ((rc = dev_upload(e, (double**)&md.q_exit, mp.q_exit.data(), mp.q_exit.size()))) return rc;
-      if ((rc = dev_upload(e, (double**)&md.cdf, mp.cdf.data(), mp.cdf.size()))) return rc;
-      if ((rc = dev_upload(e, (uint32_t**)&md.nbr, mp.nbr.data(), mp.nbr.size()))) return rc;
-      if ((rc = dev_upload(e, (uint32_t**)&md.col_ptr, mp.col_ptr.data(), mp.col_ptr.size()))) return rc;
-      if ((rc = dev_upload(e, (uint32_t**)&md.src, mp.src.data(), mp.src.size()))) return rc;
-      if ((rc = dev_upload(e, (double**)&md.val, mp.val.data(), mp.val.size()))) return rc;
-      if ((rc = dev_upload(e, (double**)&md.inv_v, mp.inv_v.data(), mp.inv_v.size()))) return rc;
-      maps.push_back(md);
-    }
-    const size_t nsc = (size_t)h.n_s * h.n_c;
-    std::vector<double> C0 = h.C0.empty() ? std::vector<double>(nsc, 0.0) : h.C0;
-    std::vector<double> m0(nsc);
-    for (uint32_t s = 0; s < h.n_s; ++s)
-      for (uint32_t j = 0; j < h.n_c; ++j) m0[(size_t)s * h.n_c + j] = C0[(size_t)s * h.n_c + j] * h.maps[0].V[j];
-    std::vector<double> F, sink;
-    feed_tables(h, F, sink);
-    const uint32_t rw = (uint32_t)rec_stride((int)h.n_s);
-    std::vector<double> rec((size_t)h.n_c * rw, 0.0);
-    if ((rc = dev_upload(e, &d.C[0], C0.data(), nsc))) return rc;
-    if ((rc = dev_upload(e, &d.C[1], C0.data(), nsc))) return rc;
-    if ((rc = dev_upload(e, &d.m, m0.data(), nsc))) return rc;
-    std::vector<double> zeros(nsc, 0.0);
-    if ((rc = dev_upload(e, &d.S, zeros.data(), nsc))) return rc;
-    if ((rc = dev_upload(e, (double**)&d.F, F.data(), nsc))) return rc;
-    if ((rc = dev_upload(e, (double**)&d.sink, sink.data(), sink.size()))) return rc;
-    if ((rc = dev_upload(e, &d.rec, rec.data(), rec.size()))) return rc;
-    if (h.probe_cap) {
-      if ((rc = dev_alloc(e, (void**)&d.probe_exit, (size_t)h.probe_cap * sizeof(double)))) return rc;
-      if ((rc = dev_alloc(e, (void**)&d.probe_div, (size_t)h.probe_cap * sizeof(double)))) return rc;
-    }
-    d.p_off = h.p_off;
-    d.seed = h.seed;
-    d.dt = h.dt;
-    d.weight = h.weight;
-    d.ratio = h.ratio;
-    d.min_removal = h.min_removal;
-    std::memcpy(d.params, h.params, sizeof(d.params));
-    d.cap = (uint32_t)h.cap;
-    d.n_c = h.n_c;
-    d.n_s = h.n_s;
-    d.rec_w = rw;
-    d.enable_move = h.n_c > 1 ? 1u : 0u;
-    d.enable_leave = 0;
-    for (const auto& fd : h.feeds)
-      if (fd.out >= 0) d.enable_leave = 1;
-    d.n_maps = (uint32_t)h.maps.size();
-    d.steps_per_map = h.steps_per_map;
-    d.map_base = h.map_base;
-    d.stage_cap = h.stage_cap;
-    d.comp_off = h.comp_off;
-    d.tile_cap_off = h.tile_cap_off;
-    d.probe_cap = h.probe_cap;
-    d.step_target = h.step_target;
-    RunState& s = rs[r];
-    std::memset(&s, 0, sizeof(s));
-    s.n = (uint32_t)h.n;
-    uint64_t inactive = 0;
-    for (uint64_t p = 0; p < h.n; ++p) inactive += h.status[p] != IDLE;
-    s.inactive = (uint32_t)inactive;
-    s.counters[C_N] = h.n;
-    s.counters[C_INACTIVE] = inactive;
-  }
-  comp_off.push_back(n_comp);
-  // Removal can only ever trigger if some particle can become inactive: a
-  // leaving flow, a model that can return Dead, or inactive initial slots.
-  e->removal_possible = e->model->can_die;
-  for (uint32_t r = 0; r < R; ++r)
-    if (e->h_runs[r].enable_leave || rs[r].inactive) e->removal_possible = true;
-  MapDev* d_maps = nullptr;
-  uint32_t* d_comp = nullptr;
-  if ((rc = dev_upload(e, &d_maps, maps.data(), maps.size()))) return rc;
-  if ((rc = dev_upload(e, &d_comp, comp_off.data(), comp_off.size()))) return rc;
-  if ((rc = dev_upload(e, &e->d_runs, e->h_runs.data(), R))) return rc;
-  if ((rc = dev_upload(e, &e->d_rs, rs.data(), R))) return rc;
-  b.maps = d_maps;
-  b.comp_off = d_comp;
-  b.runs = e->d_runs;
-  b.rs = e->d_rs;
-
-  // launch geometry of the particle pass: LDS contribution accumulator when
-  // the largest run's [n_s][n_c] fits, then the block size with the most
-  // resident waves; a grid of resident blocks, each taking a tile range.
-  // MCST_PARTICLE_BLOCK (256/512/1024) and MCST_CONTRIB_LDS (0/1) override.
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, e->device));
   size_t lds_need = 0;
-  for (const auto& h : e->runs) lds_need = std::max(lds_need, (size_t)h.n_s * h.n_c * sizeof(double));
+  uint32_t live = 0;
+  bool probe = false;
+  for (const auto& h : e->runs)
+    if (h.phase == 1) {
+      lds_need = std::max(lds_need, (size_t)h.n_s * h.n_c * sizeof(double));
+      probe = probe || h.probe_cap > 0;
+      ++live;
+    }
   const size_t lds_cap = std::min<size_t>(std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin),
                                           96 * 1024);
   bool priv = lds_need <= lds_cap;
@@ -515,44 +441,295 @@ int commit(mcst_engine* e) {
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false, false), 256, 0));
     best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
   }
-  best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, T_max));
-  for (const auto& h : e->runs) best.probe = best.probe || h.probe_cap > 0;
-  // one run: blocks take groups in turn, which keeps the wave front of the
-  // whole grid inside a narrow window of the particle arrays; a batch: each
-  // block takes one contiguous range, so it meets few runs
-  b.tile_group = R == 1 ? 1u : 0u;
-  if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
+  best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, b.T_max));
+  best.probe = probe;
   if (best.probe && best.lds > 64 * 1024)
     (void)hipFuncSetAttribute(particle_kernel_ptr(M.id, 256, best.priv, true),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)best.lds);
+  // one run: blocks take groups in turn, which keeps the wave front of the
+  // whole grid inside a narrow window of the particle arrays; a batch: each
+  // block takes one contiguous range, so it meets few runs
+  b.tile_group = live <= 1 ? 1u : 0u;
+  if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
   e->plaunch = best;
-  e->grid_particle = (int)std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, T_max);
-  e->grid_particle = std::max(e->grid_particle, 1);
+  e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
   // one thread per (compartment, species), plus the block that lays out the tiles
-  e->grid_transport = (int)std::min<uint64_t>((n_comp * b.ns_max + 255) / 256, 8192) + 1;
+  e->grid_transport = (int)std::min<uint64_t>(((uint64_t)b.n_comp_total * b.ns_max + 255) / 256, 8192) + 1;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
-                                                              (T_max * NSUB + 63) / 64));
+                                                              ((uint64_t)b.T_max * NSUB + 63) / 64));
   if (std::getenv("MCST_VERBOSE"))
-    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d (lds cap %zu, per-block %zu, optin %zu)\n",
-                 best.block, best.grid, best.lds, (int)best.priv, lds_cap, (size_t)prop.sharedMemPerBlock,
-                 (size_t)prop.sharedMemPerBlockOptin);
-  e->host_step.assign(R, 0);
+    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u (lds cap %zu)\n",
+                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, lds_cap);
+  return MCST_OK;
+}
+
+// Compartment offsets of the transport space (a run that is not on the
+// device has none) and the batch-wide counts the kernels read.
+int refresh_tables(mcst_engine* e) {
+  BatchDev& b = e->bd;
+  const uint32_t R = e->admitted;
+  e->comp_off.assign(R + 1, 0);
+  uint32_t n_comp = 0;
+  for (uint32_t r = 0; r < R; ++r) {
+    e->comp_off[r] = n_comp;
+    if (e->runs[r].phase == 1) n_comp += e->runs[r].n_c;
+  }
+  e->comp_off[R] = n_comp;
+  HIP_TRY(hipMemcpy(e->d_comp, e->comp_off.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (e->h_maps.size() > e->maps_cap) {  // the flowmap descriptor table grows
+    const uint32_t cap = (uint32_t)std::max<size_t>(e->h_maps.size(), 2 * (size_t)e->maps_cap);
+    MapDev* d = nullptr;
+    int rc = dev_alloc(e, (void**)&d, cap * sizeof(MapDev));
+    if (rc) return rc;
+    e->d_maps = d;
+    e->maps_cap = cap;
+  }
+  if (!e->h_maps.empty())
+    HIP_TRY(hipMemcpy(e->d_maps, e->h_maps.data(), e->h_maps.size() * sizeof(MapDev), hipMemcpyHostToDevice));
+  b.R = R;
+  b.n_comp_total = n_comp;
+  b.maps = e->d_maps;
+  e->removal_possible = e->model->can_die;
+  for (uint32_t r = 0; r < R; ++r)
+    if (e->runs[r].phase == 1 && e->h_runs[r].enable_leave) e->removal_possible = true;
+  return choose_launch(e);
+}
+
+// Upload run r into its slot range and its own arrays.
+int upload_run(mcst_engine* e, uint32_t r) {
+  HostRun& h = e->runs[r];
+  BatchDev& b = e->bd;
+  const uint32_t nv = (uint32_t)e->model->nv;
+  int rc;
+  if (h.n) {
+    HIP_TRY(hipMemcpy(b.pos + h.p_off, h.pos.data(), h.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.status + h.p_off, h.status.data(), h.n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.age_hyd + h.p_off, h.ah.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b.age_div + h.p_off, h.ad.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
+    for (uint32_t v = 0; v < nv; ++v)
+      HIP_TRY(hipMemcpy(b.state + (uint64_t)v * b.P + h.p_off, h.state.data() + (size_t)v * h.n,
+                        h.n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  RunDev& d = e->h_runs[r];
+  d = RunDev{};
+  h.map_base = (uint32_t)e->h_maps.size();
+  for (auto& mp : h.maps) {
+    exit_thresholds(h, mp);
+    MapDev md{};
+    md.m = mp.m;
+    if ((rc = run_upload(h, (double**)&md.p_move, mp.p_move.data(), mp.p_move.size()))) return rc;
+    if ((rc = run_upload(h, (double**)&md.q_exit, mp.q_exit.data(), mp.q_exit.size()))) return rc;
+    if ((rc = run_upload(h, (double**)&md.cdf, mp.cdf.data(), mp.cdf.size()))) return rc;
+    if ((rc = run_upload(h, (uint32_t**)&md.nbr, mp.nbr.data(), mp.nbr.size()))) return rc;
+    if ((rc = run_upload(h, (uint32_t**)&md.col_ptr, mp.col_ptr.data(), mp.col_ptr.size()))) return rc;
+    if ((rc = run_upload(h, (uint32_t**)&md.src, mp.src.data(), mp.src.size()))) return rc;
+    if ((rc = run_upload(h, (double**)&md.val, mp.val.data(), mp.val.size()))) return rc;
+    if ((rc = run_upload(h, (double**)&md.inv_v, mp.inv_v.data(), mp.inv_v.size()))) return rc;
+    e->h_maps.push_back(md);
+  }
+  const size_t nsc = (size_t)h.n_s * h.n_c;
+  std::vector<double> C0 = h.C0.empty() ? std::vector<double>(nsc, 0.0) : h.C0;
+  std::vector<double> m0(nsc);
+  for (uint32_t sp = 0; sp < h.n_s; ++sp)
+    for (uint32_t j = 0; j < h.n_c; ++j) m0[(size_t)sp * h.n_c + j] = C0[(size_t)sp * h.n_c + j] * h.maps[0].V[j];
+  std::vector<double> F, sink;
+  feed_tables(h, F, sink);
+  const uint32_t rw = (uint32_t)rec_stride((int)h.n_s);
+  std::vector<double> rec((size_t)h.n_c * rw, 0.0);
+  std::vector<double> zeros(nsc, 0.0);
+  if ((rc = run_upload(h, &d.C[0], C0.data(), nsc))) return rc;
+  if ((rc = run_upload(h, &d.C[1], C0.data(), nsc))) return rc;
+  if ((rc = run_upload(h, &d.m, m0.data(), nsc))) return rc;
+  if ((rc = run_upload(h, &d.S, zeros.data(), nsc))) return rc;
+  if ((rc = run_upload(h, (double**)&d.F, F.data(), nsc))) return rc;
+  if ((rc = run_upload(h, (double**)&d.sink, sink.data(), sink.size()))) return rc;
+  if ((rc = run_upload(h, &d.rec, rec.data(), rec.size()))) return rc;
+  if (h.probe_cap) {
+    if ((rc = run_alloc(h, (void**)&d.probe_exit, (size_t)h.probe_cap * sizeof(double)))) return rc;
+    if ((rc = run_alloc(h, (void**)&d.probe_div, (size_t)h.probe_cap * sizeof(double)))) return rc;
+  }
+  d.p_off = h.p_off;
+  d.seed = h.seed;
+  d.dt = h.dt;
+  d.weight = h.weight;
+  d.ratio = h.ratio;
+  d.min_removal = h.min_removal;
+  std::memcpy(d.params, h.params, sizeof(d.params));
+  d.cap = (uint32_t)h.cap;
+  d.n_c = h.n_c;
+  d.n_s = h.n_s;
+  d.rec_w = rw;
+  d.enable_move = h.n_c > 1 ? 1u : 0u;
+  d.enable_leave = 0;
+  for (const auto& fd : h.feeds)
+    if (fd.out >= 0) d.enable_leave = 1;
+  d.n_maps = (uint32_t)h.maps.size();
+  d.steps_per_map = h.steps_per_map;
+  d.map_base = h.map_base;
+  d.stage_cap = h.stage_cap;
+  d.comp_off = 0;
+  d.tile_cap_off = (uint32_t)(h.p_off / TILE);
+  d.probe_cap = h.probe_cap;
+  d.step_target = h.step_target;
+  RunState st;
+  std::memset(&st, 0, sizeof(st));
+  st.n = (uint32_t)h.n;
+  uint64_t inactive = 0;
+  for (uint64_t q = 0; q < h.n; ++q) inactive += h.status[q] != IDLE;
+  st.inactive = (uint32_t)inactive;
+  st.counters[C_N] = h.n;
+  st.counters[C_INACTIVE] = inactive;
+  HIP_TRY(hipMemcpy(e->d_runs + r, &d, sizeof(RunDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->d_rs + r, &st, sizeof(RunState), hipMemcpyHostToDevice));
+  // host copies of the initial particles are no longer needed
+  std::vector<uint32_t>().swap(h.pos);
+  std::vector<uint8_t>().swap(h.status);
+  std::vector<double>().swap(h.state);
+  std::vector<double>().swap(h.ah);
+  std::vector<double>().swap(h.ad);
+  h.phase = 1;
+  return MCST_OK;
+}
+
+// Admit the defined runs in id order while the pool has room: they take
+// part from the next step on.  The stream is idle during the update.
+int admit(mcst_engine* e, uint32_t* n_admitted) {
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  uint32_t k = 0;
+  int rc = MCST_OK;
+  while (e->admitted < e->runs.size()) {
+    const uint32_t r = e->admitted;
+    HostRun& h = e->runs[r];
+    if ((rc = check_definable(e, h, r))) break;
+    if (r >= e->runs_cap) {
+      rc = fail(MCST_ERR_MEMORY, "run table full");
+      break;
+    }
+    const uint32_t tiles = (uint32_t)((h.cap + TILE - 1) / TILE);
+    uint32_t start = 0;
+    if (!take_tiles(e, tiles, start)) {
+      rc = fail(MCST_ERR_MEMORY, "particle pool full");
+      break;
+    }
+    h.p_off = (uint64_t)start * TILE;
+    h.tiles = tiles;
+    if ((rc = upload_run(e, r))) {
+      give_tiles(e, start, tiles);
+      free_run_memory(h);
+      break;
+    }
+    ++e->admitted;
+    ++k;
+  }
+  if (n_admitted) *n_admitted = k;
+  if (k) {
+    drop_graph(e);
+    const int rc2 = refresh_tables(e);
+    if (rc2) return rc2;
+  }
+  return rc;
+}
+
+int commit(mcst_engine* e) {
+  const ModelEntry& M = *e->model;
+  const uint32_t R0 = (uint32_t)e->runs.size();
+  if (R0 == 0 && e->reserve_runs == 0) return fail(MCST_ERR_STATE, "no run defined");
+  uint64_t tiles = 0;
+  uint32_t stride = e->reserve_runs ? 64u : 1u;
+  for (uint32_t r = 0; r < R0; ++r) {
+    const HostRun& h = e->runs[r];
+    tiles += (h.cap + TILE - 1) / TILE;
+    stride = std::max(stride, h.stage_cap);
+  }
+  tiles = std::max<uint64_t>(tiles, (e->reserve_slots + TILE - 1) / TILE);
+  if (tiles == 0) tiles = 1;
+  const uint64_t P = tiles * TILE;
+  if (P >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "more than 2^32-1 particle slots in one engine");
+  const uint64_t T_max = tiles;
+  const uint32_t nv = (uint32_t)M.nv;
+  BatchDev& b = e->bd;
+  b = BatchDev{};
+  b.P = P;
+  b.T_max = (uint32_t)T_max;
+  b.stage_stride = stride;
+  b.ST = T_max * stride;
+  b.R = 0;
+  b.n_comp_total = 0;
+  b.nsub = NSUB;
+  b.ns_max = std::max<uint32_t>(1, (uint32_t)M.ns);  // one model per engine; thread s == 0 also writes the map fields
+  b.stages = STAGE_ALL;
+  e->runs_cap = std::max(R0, e->reserve_runs);
+  const uint32_t RC = e->runs_cap;
+  int rc;
+  if ((rc = dev_alloc(e, (void**)&b.pos, P * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.status, P))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.state, P * nv * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.age_hyd, P * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.age_div, P * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.st_state, b.ST * nv * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.st_pos, b.ST * sizeof(uint32_t)))) return rc;
+  uint32_t** tile_arrays[] = {&b.tile_births, &b.tile_birth_off, &b.tile_gap, &b.tile_fill, &b.tile_gap_off,
+                              &b.tile_fill_off};
+  for (uint32_t** a : tile_arrays)
+    if ((rc = dev_alloc(e, (void**)a, T_max * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.tile_off, (RC + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.move_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * NSUB * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * NSUB * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&e->d_runs, (size_t)RC * sizeof(RunDev)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&e->d_rs, (size_t)RC * sizeof(RunState)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&e->d_comp, (size_t)(RC + 1) * sizeof(uint32_t)))) return rc;
+  HIP_TRY(hipMemset(b.tile_births, 0, T_max * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.tile_off, 0, (RC + 1) * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.sub_moves, 0, T_max * NSUB * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.sub_divs, 0, T_max * NSUB * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(e->d_rs, 0, (size_t)RC * sizeof(RunState)));
+  b.runs = e->d_runs;
+  b.rs = e->d_rs;
+  b.comp_off = e->d_comp;
+  e->h_runs.assign(RC, RunDev{});
+  e->free_tiles.assign(1, {0u, (uint32_t)T_max});
+  e->admitted = 0;
+  // with reserved capacity, runs that do not fit yet stay defined (admit them later)
+  rc = admit(e, nullptr);
+  if (rc && !(rc == MCST_ERR_MEMORY && e->reserve_runs)) return rc;
+  if (e->admitted < R0 && !e->reserve_runs) return fail(MCST_ERR_MEMORY, "the runs do not fit the engine");
+  if (R0 == 0 && (rc = refresh_tables(e))) return rc;
   launch_layout(b, e->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(e->stream));
   e->committed = true;
-  // host copies of the initial particles are no longer needed
-  for (auto& h : e->runs) {
-    std::vector<uint32_t>().swap(h.pos);
-    std::vector<uint8_t>().swap(h.status);
-    std::vector<double>().swap(h.state);
-    std::vector<double>().swap(h.ah);
-    std::vector<double>().swap(h.ad);
-  }
   return MCST_OK;
 }
 
+// Take a finished run off the device: its slots and arrays return to the
+// pool, and it stops taking part in the batch.  The id is not reused.
+int retire(mcst_engine* e, uint32_t r) {
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HostRun& h = e->runs[r];
+  RunDev& d = e->h_runs[r];
+  d.step_target = 0;  // inactive from now on: no tiles, no transport
+  HIP_TRY(hipMemcpy(e->d_runs + r, &d, sizeof(RunDev), hipMemcpyHostToDevice));
+  RunState st;
+  HIP_TRY(hipMemcpy(&st, e->d_rs + r, sizeof(RunState), hipMemcpyDeviceToHost));
+  st.n = 0;
+  st.active = 0;
+  st.compact = 0;
+  HIP_TRY(hipMemcpy(e->d_rs + r, &st, sizeof(RunState), hipMemcpyHostToDevice));
+  give_tiles(e, (uint32_t)(h.p_off / TILE), h.tiles);
+  h.phase = 2;
+  drop_graph(e);
+  int rc = refresh_tables(e);
+  free_run_memory(h);
+  return rc;
+}
+
 void launch_step(mcst_engine* e, uint32_t stages) {
+  if (e->bd.R == 0) return;  // no run admitted yet
   BatchDev b = e->bd;
   b.stages = stages;
   const uint32_t nv = (uint32_t)e->model->nv;
@@ -581,13 +758,14 @@ using namespace mcst;
   } while (0)
 #define CHECK_DEFINE(e, r, h)                                               \
   CHECK_ENGINE(e);                                                          \
-  if ((e)->committed) return fail(MCST_ERR_STATE, "engine already committed"); \
   HostRun* h = get_run(e, r);                                               \
-  if (!h) return fail(MCST_ERR_ARG, "bad run id")
+  if (!h) return fail(MCST_ERR_ARG, "bad run id");                          \
+  if (h->phase != 0) return fail(MCST_ERR_STATE, "run already on the device")
 #define CHECK_READY(e, r)                                                   \
   CHECK_ENGINE(e);                                                          \
   if (!(e)->committed) return fail(MCST_ERR_STATE, "engine not committed"); \
-  if (!get_run(e, r)) return fail(MCST_ERR_ARG, "bad run id")
+  if (!get_run(e, r)) return fail(MCST_ERR_ARG, "bad run id");              \
+  if (get_run(e, r)->phase != 1) return fail(MCST_ERR_STATE, "run not on the device")
 
 extern "C" {
 
@@ -656,7 +834,8 @@ int mcst_engine_destroy(mcst_engine** pe) {
 int mcst_run_create(mcst_engine* e, uint32_t n_compartments, uint32_t n_species, double dt, uint64_t seed,
                     int32_t* run_id) {
   CHECK_ENGINE(e);
-  if (e->committed) return fail(MCST_ERR_STATE, "engine already committed");
+  if (e->committed && e->reserve_runs == 0)
+    return fail(MCST_ERR_STATE, "engine already committed without reserved capacity");
   if (n_compartments == 0) return fail(MCST_ERR_ARG, "a run needs at least one compartment");
   if ((int)n_species != e->model->ns)
     return fail(MCST_ERR_ARG, "model " + std::string(e->model->name) + " needs " + std::to_string(e->model->ns) +
@@ -831,6 +1010,33 @@ int mcst_engine_commit(mcst_engine* e) {
   return rc;
 }
 
+int mcst_engine_reserve(mcst_engine* e, uint64_t slots, uint32_t runs) {
+  CHECK_ENGINE(e);
+  if (e->committed) return fail(MCST_ERR_STATE, "engine already committed");
+  if (slots >= 0xFFFFFFFFull || runs > (1u << 20)) return fail(MCST_ERR_ARG, "reservation too large");
+  e->reserve_slots = slots;
+  e->reserve_runs = runs;
+  return MCST_OK;
+}
+
+int mcst_engine_admit(mcst_engine* e, uint32_t* n_admitted) {
+  CHECK_ENGINE(e);
+  if (n_admitted) *n_admitted = 0;
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  HIP_TRY(hipSetDevice(e->device));
+  return admit(e, n_admitted);
+}
+
+int mcst_engine_retire(mcst_engine* e, int32_t run) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  HostRun* h = get_run(e, run);
+  if (!h) return fail(MCST_ERR_ARG, "bad run id");
+  if (h->phase != 1) return fail(MCST_ERR_STATE, "run not on the device");
+  HIP_TRY(hipSetDevice(e->device));
+  return retire(e, (uint32_t)run);
+}
+
 int mcst_engine_step(mcst_engine* e, uint32_t n_steps) {
   CHECK_ENGINE(e);
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
@@ -896,7 +1102,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipEventRecord(t0, e->stream));
   const uint32_t nv = (uint32_t)e->model->nv;
-  for (uint32_t s = 0; s < n_steps; ++s) {
+  for (uint32_t s = 0; s < n_steps && e->bd.R; ++s) {
     hipEvent_t* x = &ev[(size_t)s * (K + 1)];
     BatchDev b = e->bd;
     HIP_TRY(hipEventRecord(x[0], e->stream));

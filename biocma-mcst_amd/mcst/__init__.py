@@ -32,6 +32,7 @@ EXPORTS = [
     "mcst_run_get_n", "mcst_run_get_weight", "mcst_run_get_particles", "mcst_run_get_concentrations",
     "mcst_run_get_mass", "mcst_run_get_contributions", "mcst_run_get_repartition",
     "mcst_run_get_exit_ages", "mcst_run_get_division_ages",
+    "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -75,6 +76,9 @@ def lib():
             "mcst_run_init_particles": [_vp, i32, u64, u64, dbl],
             "mcst_run_set_particles": [_vp, i32, u64, u64, _u32p, _u8p, _f64p, _f64p, _f64p, dbl],
             "mcst_engine_commit": [_vp],
+            "mcst_engine_reserve": [_vp, u64, u32],
+            "mcst_engine_admit": [_vp, C.POINTER(u32)],
+            "mcst_engine_retire": [_vp, i32],
             "mcst_engine_step": [_vp, u32],
             "mcst_engine_step_stages": [_vp, u32],
             "mcst_engine_capture": [_vp, u32],
@@ -153,6 +157,23 @@ class Engine:
 
     def commit(self):
         _ck(lib().mcst_engine_commit(self.h), "commit")
+
+    def reserve(self, slots, runs):
+        """Before commit: a particle-slot pool and run entries for runs added later."""
+        _ck(lib().mcst_engine_reserve(self.h, int(slots), int(runs)), "reserve")
+
+    def admit(self):
+        """After commit: put the runs defined since on the device, in id order.
+        Returns how many were admitted; stops quietly at the first that does
+        not fit (it stays defined)."""
+        n = C.c_uint32()
+        rc = lib().mcst_engine_admit(self.h, C.byref(n))
+        if rc not in (0, -4):  # MCST_ERR_MEMORY: no room for the next run yet
+            _ck(rc, "admit")
+        return n.value
+
+    def retire(self, run):
+        _ck(lib().mcst_engine_retire(self.h, run.id), "retire")
 
     def step(self, n=1):
         _ck(lib().mcst_engine_step(self.h, n), "step")

@@ -160,3 +160,43 @@ def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, s
         p = None
     return Case(f"small_{model}", model, n_c, n_s, 0.01, seed, maps, steps_per_map=steps_per_map, feeds=fd,
                 C0=C0, params=p, n_particles=n_particles, x0=1.0)
+
+
+MONOD_MASS_CONFIG0 = np.array([0.5 / 3600.0, 0.01, 0.5, 2.0, 1e-15, 0.1])
+
+
+def ensemble_run(i, seed=11, n_steps=500, n_c_range=(32, 4000), n_range=(1e4, 2e6)):
+    """Run i of the batched ensemble: a geometric-graph network as in config1
+    with a log-uniform compartment count, a log-uniform particle count, its own
+    seed and Monod parameters (mu_max, K_s, yield) drawn uniform +-30 % around
+    config0's.  One species (glucose 5 g/L), one static flowmap, no feed."""
+    rng = np.random.default_rng([seed, i])
+    n_c = int(round(math.exp(rng.uniform(math.log(n_c_range[0]), math.log(n_c_range[1])))))
+    n_p = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+    edges, _ = geometric_graph_cylinder(n_c, 6.0, rng)
+    V = rng.uniform(0.1, 1.0, n_c) * L
+    q = rng.lognormal(math.log(1e-4), 1.0, len(edges))
+    rows, cols, flows = _symmetric(edges, q)
+    p = MONOD_MASS_CONFIG0.copy()
+    p[:3] *= rng.uniform(0.7, 1.3, 3)
+    C0 = np.full((1, n_c), 5.0)
+    return Case(f"ensemble_{i}", "monod_mass", n_c, 1, 0.01, int(seed) * 1_000_003 + i, [(V, rows, cols, flows)],
+                C0=C0, params=p, n_particles=n_p, x0=1.0, n_steps=n_steps)
+
+
+def config2(n_runs=512, seed=11, n_steps=500, **kw):
+    """512 independent runs of uneven size (the batched ensemble)."""
+    return [ensemble_run(i, seed, n_steps, **kw) for i in range(n_runs)]
+
+
+def config3_requests(n_requests, rate=20.0, seed=13, steps_range=(200, 5000), **kw):
+    """The served stream: Poisson arrivals at `rate` per second, each request
+    an ensemble run with a log-uniform step count.  Returns
+    [(arrival_seconds, case)] in arrival order."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    out = []
+    for i in range(n_requests):
+        steps = int(round(math.exp(rng.uniform(math.log(steps_range[0]), math.log(steps_range[1])))))
+        out.append((float(t[i]), ensemble_run(i, seed, steps, **kw)))
+    return out

@@ -136,6 +136,23 @@ int mcst_run_set_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t cap
 /* Allocate device memory, upload every run; the engine can then step. */
 int mcst_engine_commit(mcst_engine* e);
 
+/* ---- serving: runs that join and leave a stepping engine ----
+ * No counterpart in the reference (its SimulationUnit holds one run for the
+ * whole program); it serves the request loop that admits simulations by
+ * device-memory footprint and keeps each run resident until it completes.
+ * Before commit: reserve a particle-slot pool and a number of run entries
+ * for runs defined later.  The engine may then be committed with no run. */
+int mcst_engine_reserve(mcst_engine* e, uint64_t slots, uint32_t runs);
+/* After commit: put the runs defined since (in id order) on the device; they
+ * take part from the next step on, starting at their own step 0.  Stops at
+ * the first run that does not fit (MCST_ERR_MEMORY; it and later runs stay
+ * defined and can be admitted once room is free).  *n_admitted counts the
+ * runs admitted by this call. */
+int mcst_engine_admit(mcst_engine* e, uint32_t* n_admitted);
+/* Take a run off the device (read its results first): its slots and arrays
+ * return to the pool and its id becomes invalid for result calls. */
+int mcst_engine_retire(mcst_engine* e, int32_t run);
+
 /* ---- stepping ---- */
 /* n steps of every active run: feed, transport, particle cycle, removal,
  * merge (the body of main_loop, apps/core/src/host_specific.cpp:280-291). */

@@ -53,3 +53,30 @@ def assert_parity(g, o, rtol=RTOL, probes=True):
     if probes:
         close(np.sort(g.exit_ages()), np.sort(o.exit_ages()), rtol, "exit ages")
         close(np.sort(g.division_ages()), np.sort(o.division_ages()), rtol, "division ages")
+
+
+def snapshot(run, probes=False):
+    """Everything assert_parity compares, read from a run now (a served run
+    leaves the device once it completes)."""
+    s = {"counters": run.counters(), "particles": run.particles(), "C": run.concentrations(),
+         "m": run.mass(), "S": run.contributions(), "rep": run.repartition()}
+    if probes:
+        s["exit_ages"] = run.exit_ages()
+        s["division_ages"] = run.division_ages()
+    return s
+
+
+def assert_snapshot(s, o, rtol=RTOL):
+    """A snapshot of a HIP run against an oracle run at the same step."""
+    co = o.counters()
+    assert s["counters"] == co, (s["counters"], co)
+    pg, po = s["particles"], o.particles()
+    np.testing.assert_array_equal(pg["pos"], po["pos"])
+    np.testing.assert_array_equal(pg["status"], po["status"])
+    close(pg["state"], po["state"], rtol, "state")
+    close(pg["age_hyd"], po["age_hyd"], rtol, "age_hyd")
+    close(pg["age_div"], po["age_div"], rtol, "age_div")
+    close(s["C"], o.concentrations(), rtol, "concentrations")
+    close(s["m"], o.mass(), rtol, "liquid mass")
+    close(s["S"], o.contributions(), 1e-11, "contributions")
+    np.testing.assert_array_equal(s["rep"], o.repartition())
