@@ -63,6 +63,87 @@ def cpu_baseline(case_fn, seconds=12.0, max_steps=20000, n_particles=1_000_000):
                       f"{n_particles:.0e} particles, {done} steps, {el:.1f} s"}
 
 
+def served_measurement(n_requests=40, rate=20.0, seed=13, chunk=20):
+    """The served stream (Poisson arrivals at `rate`/s, ensemble-sized runs with
+    200..5000 steps) through the request loop, and the same requests one at a
+    time on the same GPU (each on its own engine, started when it has arrived
+    and the previous one is done)."""
+    from mcst import cases
+    from mcst.serve import Request, Server, percentile, run_alone
+
+    reqs = [Request(i, t, c, c.n_steps) for i, (t, c) in enumerate(cases.config3_requests(n_requests, rate, seed))]
+    slots = 2 * sum(r.case.n_particles for r in reqs) + 1024 * n_requests
+
+    def serve(rs):
+        srv = Server("monod_mass", slots=slots, max_runs=n_requests, budget_bytes=200 * 10**9, chunk_steps=chunk)
+        out = srv.serve(rs)
+        srv.close()
+        return out
+
+    res = serve(reqs)
+    lat = [r.latency for r in res]
+    ps = sum(r.particle_steps for r in res)
+    t_first = min(r.arrival for r in reqs)
+    served_span = max(r.finished for r in res) - t_first
+    # the same requests all at once: the batched engine's capacity
+    burst = serve([Request(r.rid, 0.0, r.case, r.steps) for r in reqs])
+    burst_span = max(r.finished for r in burst)
+    # one at a time: the measured durations, scheduled against the same arrivals
+    t_free, lat_seq = 0.0, []
+    ps_seq = 0
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        dur, p_steps = run_alone(r, "monod_mass")
+        start = max(r.arrival, t_free)
+        t_free = start + dur
+        lat_seq.append(t_free - r.arrival)
+        ps_seq += p_steps
+    seq_span = t_free - t_first
+    return {
+        "requests": n_requests, "arrival_rate_per_s": rate, "chunk_steps": chunk,
+        "particle_steps": ps, "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99),
+        "aggregate_particle_steps_per_s": ps / served_span,
+        "sequential": {"p50_s": percentile(lat_seq, 50), "p99_s": percentile(lat_seq, 99),
+                       "aggregate_particle_steps_per_s": ps_seq / seq_span},
+        "speedup_vs_sequential": (ps / served_span) / (ps_seq / seq_span),
+        "burst": {"p50_s": percentile([r.latency for r in burst], 50),
+                  "p99_s": percentile([r.latency for r in burst], 99),
+                  "aggregate_particle_steps_per_s": sum(r.particle_steps for r in burst) / burst_span,
+                  "speedup_vs_sequential": (sum(r.particle_steps for r in burst) / burst_span) / (ps_seq / seq_span)},
+        "workload": "config3 stream: monod_mass runs, 32..4000 compartments, 1e4..2e6 particles, "
+                    "200..5000 steps, Poisson arrivals",
+    }
+
+
+def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
+    """The batched ensemble: n_runs uneven runs in one engine, steps captured
+    in a graph of `graph` steps."""
+    import mcst
+    from mcst import cases
+
+    cs = cases.config2(n_runs, seed, steps)
+    eng = mcst.Engine("monod_mass")
+    runs = []
+    for c in cs:
+        r = eng.add_run(c.n_c, c.dt, c.seed)
+        c.apply(r)
+        runs.append(r)
+    eng.commit()
+    eng.capture(graph)
+    eng.replay(1)
+    eng.synchronize()
+    p0 = sum(r.counters()["processed"] for r in runs)
+    reps = max(1, (steps - graph) // graph)
+    t0 = time.perf_counter()
+    eng.replay(reps)
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    ps = sum(r.counters()["processed"] for r in runs) - p0
+    eng.close()
+    return {"runs": n_runs, "particles": int(sum(c.n_particles for c in cs)),
+            "compartments": int(sum(c.n_c for c in cs)), "steps_timed": reps * graph,
+            "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / (reps * graph)}
+
+
 def copy_calibration(mib, reps=5):
     """Plain device-to-device copies of a known size: what FETCH_SIZE and
     WRITE_SIZE read for them calibrates the particle kernel's counters."""
@@ -90,6 +171,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=0, help="replay a captured graph of this many steps")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--served-requests", type=int, default=40)
+    ap.add_argument("--ensemble", action="store_true", help="also measure the 512-run batched ensemble")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this many MiB (PMC counter calibration)")
     a = ap.parse_args()
@@ -171,6 +255,16 @@ def main():
         log("timing the CPU baseline")
         cpu = cpu_baseline(cases.config1)
         log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
+    served = None
+    if not a.no_served:
+        log(f"serving {a.served_requests} requests of the served stream, then one at a time")
+        served = served_measurement(a.served_requests)
+        log(f"served p50 {served['p50_s']:.3f} s p99 {served['p99_s']:.3f} s, "
+            f"{served['speedup_vs_sequential']:.2f}x the one-at-a-time aggregate")
+    ensemble = None
+    if a.ensemble:
+        log("batched ensemble of 512 runs")
+        ensemble = ensemble_measurement()
     nnz = case.nnz + case.n_c
     out = {
         "metric": METRIC,
@@ -200,7 +294,10 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "served": served,
     }
+    if ensemble is not None:
+        out["ensemble"] = ensemble
     eng.close()
     print(json.dumps(out), flush=True)
 

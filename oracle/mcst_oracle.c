@@ -852,6 +852,7 @@ typedef struct {
   uint32_t stage_n;
   double* stage_state; /* [stage_cap][n_var] */
   uint32_t* stage_pos;
+  dvec exit_ages, div_ages; /* probes of this tile (threaded path), appended in tile order */
 } tile_out;
 
 static void particle_tile(orc_run* r, const flowmap* f, uint64_t step, uint32_t tile, uint32_t budget,
@@ -1048,11 +1049,18 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
         double* S_loc = Sp + nsc * (size_t)tid;
 #pragma omp for schedule(static)
         for (int64_t tl = 0; tl < (int64_t)n_tiles; ++tl)
-          particle_tile(r, f, step, (uint32_t)tl, budget, stages, enable_move, enable_leave, S_loc, &outs[tl], NULL, NULL);
+          particle_tile(r, f, step, (uint32_t)tl, budget, stages, enable_move, enable_leave, S_loc, &outs[tl],
+                        &outs[tl].exit_ages, &outs[tl].div_ages);
       }
       for (int th = 0; th < n_threads; ++th)
         for (size_t o = 0; o < nsc; ++o) r->S[o] += Sp[nsc * (size_t)th + o];
       free(Sp);
+      for (uint32_t tl = 0; tl < n_tiles; ++tl) {
+        for (uint64_t k = 0; k < outs[tl].exit_ages.n; ++k) dvec_push(&r->exit_ages, outs[tl].exit_ages.v[k]);
+        for (uint64_t k = 0; k < outs[tl].div_ages.n; ++k) dvec_push(&r->div_ages, outs[tl].div_ages.v[k]);
+        free(outs[tl].exit_ages.v);
+        free(outs[tl].div_ages.v);
+      }
     }
     uint64_t births = 0, exits = 0, deaths = 0;
     for (uint32_t tl = 0; tl < n_tiles; ++tl) {

@@ -249,3 +249,7 @@ def test_threads_do_not_change_events(oracle_lib, model):
     np.testing.assert_array_equal(a["status"], b["status"])
     np.testing.assert_allclose(a["state"], b["state"], rtol=1e-12)
     np.testing.assert_allclose(runs[0].concentrations(), runs[1].concentrations(), rtol=1e-12)
+    # the probes are recorded in tile order whatever the thread count
+    np.testing.assert_array_equal(runs[0].exit_ages(), runs[1].exit_ages())
+    np.testing.assert_array_equal(runs[0].division_ages(), runs[1].division_ages())
+    assert len(runs[0].division_ages()) > 0
