@@ -126,7 +126,9 @@ int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap);
 int mcst_run_set_step_target(mcst_engine* e, int32_t run, uint32_t steps);
 /* Initial particles: n particles uniformly over compartments with
  * Model::init, capacity slots reserved, weight = x0 * V_total / sum(mass)
- * (MC::init + post_init_weight).  Needs a flowmap. */
+ * (MC::init + post_init_weight).  Particle i is drawn from Philox stream i;
+ * the drawn particles are then stored in compartment order (stable in i).
+ * Needs a flowmap. */
 int mcst_run_init_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, double x0);
 /* Or explicit particles: pos[n], status[n], state[n_var][n], ages[n]. */
 int mcst_run_set_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, const uint32_t* pos,

@@ -812,6 +812,30 @@ int orc_init_particles(orc_run* r, uint64_t n, uint64_t capacity, double x0) {
   double vt = 0.0;
   for (uint32_t j = 0; j < r->n_c; ++j) vt += r->maps[0].V[j];
   r->weight = n ? (x0 * vt) / total_mass : 0.0;
+  /* the drawn particles are stored in compartment order, draw order among
+     equal positions (a stable counting sort; the weight above is summed in
+     draw order) */
+  if (n > 1 && r->n_c > 1) {
+    uint64_t* next = (uint64_t*)calloc((size_t)r->n_c + 1, sizeof(uint64_t));
+    uint64_t* dst = (uint64_t*)malloc(sizeof(uint64_t) * n);
+    uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    double* col = (double*)malloc(sizeof(double) * n);
+    if (!next || !dst || !pos || !col) {
+      free(next), free(dst), free(pos), free(col);
+      return -2;
+    }
+    for (uint64_t p = 0; p < n; ++p) next[r->pos[p] + 1]++;
+    for (uint32_t c = 0; c < r->n_c; ++c) next[c + 1] += next[c];
+    for (uint64_t p = 0; p < n; ++p) dst[p] = next[r->pos[p]]++;
+    for (uint64_t p = 0; p < n; ++p) pos[dst[p]] = r->pos[p];
+    memcpy(r->pos, pos, sizeof(uint32_t) * n);
+    for (int v = 0; v < r->n_var; ++v) {
+      double* x = r->state + (size_t)v * r->cap;
+      for (uint64_t p = 0; p < n; ++p) col[dst[p]] = x[p];
+      memcpy(x, col, sizeof(double) * n);
+    }
+    free(next), free(dst), free(pos), free(col);
+  }
   return 0;
 }
 
