@@ -453,8 +453,8 @@ int choose_launch(mcst_engine* e) {
   if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
   e->plaunch = best;
   e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
-  // one thread per (compartment, species), plus the block that lays out the tiles
-  e->grid_transport = (int)std::min<uint64_t>(((uint64_t)b.n_comp_total * b.ns_max + 255) / 256, 8192) + 1;
+  // one thread per compartment, plus the block that lays out the tiles
+  e->grid_transport = (int)std::min<uint64_t>(((uint64_t)b.n_comp_total + 255) / 256, 8192) + 1;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
                                                               ((uint64_t)b.T_max * NSUB + 63) / 64));
   if (std::getenv("MCST_VERBOSE"))
@@ -733,7 +733,7 @@ void launch_step(mcst_engine* e, uint32_t stages) {
   BatchDev b = e->bd;
   b.stages = stages;
   const uint32_t nv = (uint32_t)e->model->nv;
-  launch_transport(b, e->grid_transport, e->stream);
+  launch_transport(e->model->id, b, e->grid_transport, e->stream);
   launch_particle(e->model->id, b, e->plaunch, e->stream);
   launch_resolve(e->model->id, b, e->grid_move, e->stream);
   launch_bookkeep(b, e->stream);
@@ -1131,7 +1131,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
     hipEvent_t* x = &ev[(size_t)s * (K + 1)];
     BatchDev b = e->bd;
     HIP_TRY(hipEventRecord(x[0], e->stream));
-    launch_transport(b, e->grid_transport, e->stream);
+    launch_transport(e->model->id, b, e->grid_transport, e->stream);
     HIP_TRY(hipEventRecord(x[1], e->stream));
     launch_particle(e->model->id, b, e->plaunch, e->stream);
     HIP_TRY(hipEventRecord(x[2], e->stream));

@@ -129,58 +129,91 @@ __device__ uint32_t block_scan(const uint32_t* in, uint32_t* out, uint32_t n, bo
 
 // ------------------------------------------------------------------ transport
 // m += dt*(C*M - C*diag(sink) + S + F);  C = m*diag(1/V)  for every
-// (compartment, species) of every active run, one thread each; also refreshes
-// the per-compartment record {C_0..C_{n_s-1}, p_move, q_exit} the particle
-// pass gathers, and clears S.  The extra last block lays out the step's
-// tiles (layout_pass), which needs only the counts of the previous step.
+// compartment of every active run, one thread each, all species at once (the
+// flow matrix entries are read once for all of them); also refreshes the
+// per-compartment record {M::record(C), p_move, q_exit} the particle pass
+// gathers, and clears S.  The extra last block lays out the step's tiles
+// (layout_pass), which needs only the counts of the previous step.
 template <uint32_t NT>
 __device__ void layout_pass(const BatchDev& b);
 
+template <class M>
 __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
   if (blockIdx.x == gridDim.x - 1) {
     layout_pass<256>(b);
     return;
   }
+  constexpr int NS = M::NS;
   const uint32_t total = b.n_comp_total;
   const uint32_t stride = (gridDim.x - 1) * blockDim.x;
-  const uint32_t pairs = total * b.ns_max;
-  for (uint32_t gs = blockIdx.x * blockDim.x + threadIdx.x; gs < pairs; gs += stride) {
-    // gs enumerates (compartment g, species s) pairs, species-major
-    const uint32_t g = gs % total;
-    const uint32_t s = gs / total;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
     const uint32_t r = find_run(b.comp_off, b.R, g);
     const RunDev& rd = b.runs[r];
-    if (s >= rd.n_s && s != 0) continue;
     const RunState* rs = b.rs + r;
     // the run's activity is recomputed here rather than read from the
     // layout block, which writes it concurrently
     if (rs->step >= rd.step_target) continue;
     const uint32_t j = g - b.comp_off[r];
-    const uint32_t n_c = rd.n_c, n_s = rd.n_s;
+    const uint32_t n_c = rd.n_c;
     const uint32_t step = rs->step;
     const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
     double* rec = rd.rec + (size_t)j * rd.rec_w;
-    if (s == 0) {
-      rec[n_s] = mp.p_move[j];
-      rec[n_s + 1] = mp.q_exit[j];
-    }
-    if (s >= n_s) continue;
-    const double* c = rd.C[step & 1u] + (size_t)s * n_c;
+    rec[NS] = mp.p_move[j];
+    rec[NS + 1] = mp.q_exit[j];
+    const double* c = rd.C[step & 1u];
     double* Cn = rd.C[(step + 1u) & 1u];
-    const size_t o = (size_t)s * n_c + j;
-    double cn = c[j];
+    double cn[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) cn[s] = c[(size_t)s * n_c + j];
     if (b.stages & STAGE_TRANSPORT) {
       const uint32_t e0 = mp.col_ptr[j], e1 = mp.col_ptr[j + 1];
-      double acc = 0.0;
-      for (uint32_t e = e0; e < e1; ++e) acc += c[mp.src[e]] * mp.val[e];
-      const double dmdt = (acc - c[j] * rd.sink[j]) + (rd.S[o] + rd.F[o]);
-      const double mm = rd.m[o] + rd.dt * dmdt;
-      rd.m[o] = mm;
-      cn = mm * mp.inv_v[j];
+      const uint32_t* __restrict__ src = mp.src;
+      const double* __restrict__ val = mp.val;
+      double acc[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+      // entries in groups of 8 whose loads are all in flight together (a
+      // column holds the degree plus the diagonal, rarely more than 8);
+      // the sums still run in entry order
+      constexpr uint32_t EG = 8;
+      for (uint32_t e = e0; e < e1; e += EG) {
+        uint32_t i[EG];
+        double v[EG], cv[EG][NS];
+#pragma unroll
+        for (uint32_t k = 0; k < EG; ++k) {
+          const uint32_t ek = e + k < e1 ? e + k : e1 - 1u;
+          i[k] = src[ek];
+          v[k] = val[ek];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < EG; ++k)
+#pragma unroll
+          for (int s = 0; s < NS; ++s) cv[k][s] = c[(size_t)s * n_c + i[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < EG; ++k)
+          if (e + k < e1)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc[s] += cv[k][s] * v[k];
+      }
+      const double sink = rd.sink[j], inv_v = mp.inv_v[j];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const size_t o = (size_t)s * n_c + j;
+        const double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + rd.F[o]);
+        const double mm = rd.m[o] + rd.dt * dmdt;
+        rd.m[o] = mm;
+        cn[s] = mm * inv_v;
+      }
     }
-    Cn[o] = cn;
-    rec[s] = cn;
-    rd.S[o] = 0.0;
+    double rt[NS];
+    M::record(rd.params, cn, rt);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const size_t o = (size_t)s * n_c + j;
+      Cn[o] = cn[s];
+      rec[s] = rt[s];
+      rd.S[o] = 0.0;
+    }
   }
 }
 
@@ -313,7 +346,7 @@ __global__ void __launch_bounds__(BLK)
     k_particle(BatchDev b, const RunDev* __restrict__ runs, const RunState* __restrict__ rsr,
                const uint32_t* __restrict__ tile_off) {
   constexpr int NV = M::NV, NS = M::NS, REC = rec_stride(M::NS);
-  constexpr int NQ = NS + 2;             // record fields: C_0..C_{n_s-1}, p_move, q_exit
+  constexpr int NQ = NS + 2;             // record fields: M::record terms r_0..r_{n_s-1}, p_move, q_exit
   constexpr int NP = M::NP > 0 ? M::NP : 1;
   constexpr uint32_t WPB = BLK / 64;     // waves per block = sub-chunks per group
   constexpr uint32_t SUB = 64 * NL;       // slots per sub-chunk
@@ -1159,8 +1192,15 @@ const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe) 
   }
 }
 
-void launch_transport(const BatchDev& b, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_transport, dim3(grid), dim3(256), 0, s, b);
+void launch_transport(int model_id, const BatchDev& b, int grid, hipStream_t s) {
+  switch (model_id) {
+    case MonodMass::ID: hipLaunchKernelGGL(k_transport<MonodMass>, dim3(grid), dim3(256), 0, s, b); break;
+    case DualMonod::ID: hipLaunchKernelGGL(k_transport<DualMonod>, dim3(grid), dim3(256), 0, s, b); break;
+    case Monod::ID: hipLaunchKernelGGL(k_transport<Monod>, dim3(grid), dim3(256), 0, s, b); break;
+    case FixedLength::ID: hipLaunchKernelGGL(k_transport<FixedLength>, dim3(grid), dim3(256), 0, s, b); break;
+    case SimpleAcetate::ID: hipLaunchKernelGGL(k_transport<SimpleAcetate>, dim3(grid), dim3(256), 0, s, b); break;
+    default: break;
+  }
 }
 void launch_bookkeep(const BatchDev& b, hipStream_t s) {
   hipLaunchKernelGGL(k_bookkeep, dim3(b.R), dim3(SCAN_THREADS), 0, s, b);

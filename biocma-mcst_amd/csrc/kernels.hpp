@@ -7,7 +7,7 @@
 
 namespace mcst {
 
-void launch_transport(const BatchDev& b, int grid, hipStream_t s);
+void launch_transport(int model_id, const BatchDev& b, int grid, hipStream_t s);
 // Launch shape of the particle pass: block size (256/512/1024 threads per
 // 1024-slot tile), grid, the LDS bytes of the per-block contribution
 // accumulator (priv) or none (direct global atomics), and whether some run

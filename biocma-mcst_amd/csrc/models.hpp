@@ -4,14 +4,20 @@
 // ModelType concept, apps/libs/mc/public/mc/traits.hpp:66-153):
 //   NV, NS, NP           state variables, liquid species, parameters
 //   init(p, x, seed, slot)                 host, initial state      (Model::init)
-//   update(p, dt, x, c, contrib) -> Status device+host               (Model::update)
+//   record(p, c, r)                        device+host, per compartment
+//   update(p, dt, x, r, contrib) -> Status device+host               (Model::update)
 //   division(p, x, xd, key)                device+host              (Model::division)
 //   mass(p, x)                             device+host              (Model::mass)
 //   CAN_DIE                                whether update() may return DEAD
 // State is FP64; contributions (kg/s per particle) are returned in
 // registers instead of a per-particle contribs row, because the step kernel
-// scatters them immediately.  All code here is built with -ffp-contract=off,
-// so each expression rounds exactly as written.
+// scatters them immediately.  record() maps a compartment's concentrations
+// c[0..NS) to the NS terms r[] that update() reads: the parts of the
+// reference's update that depend on the liquid alone (e.g. the Monod factor
+// mu_max*S/(Ks+S)), evaluated once per compartment and step by the transport
+// kernel instead of once per particle, with the same operations in the same
+// order, so the results are identical.  All code here is built with
+// -ffp-contract=off, so each expression rounds exactly as written.
 #pragma once
 
 #include <math.h>
@@ -144,9 +150,12 @@ struct MonodMass {
     x[1] = m / (1.0 + u01(w.z));
   }
   MCST_HD static double mass(const double*, const double* x) { return x[0]; }
-  MCST_HD static int update(const double* p, double dt, double* x, const double* c, double* contrib) {
+  MCST_HD static void record(const double* p, const double* c, double* r) {
     const double s = c[0] > 0.0 ? c[0] : 0.0;
-    const double mu = p[0] * s / (p[1] + s);
+    r[0] = p[0] * s / (p[1] + s);  // mu
+  }
+  MCST_HD static int update(const double* p, double dt, double* x, const double* r, double* contrib) {
+    const double mu = r[0];
     const double g = mu * x[0];
     contrib[0] = -(g / p[2]);
     x[0] = x[0] + dt * g;
@@ -180,12 +189,16 @@ struct DualMonod {
     x[2] = 0.0;
   }
   MCST_HD static double mass(const double*, const double* x) { return x[0]; }
-  MCST_HD static int update(const double* p, double dt, double* x, const double* c, double* contrib) {
+  MCST_HD static void record(const double* p, const double* c, double* r) {
     const double S = c[0] > 0.0 ? c[0] : 0.0;
     const double O = c[1] > 0.0 ? c[1] : 0.0;
-    const double qt = p[0] * S / (p[1] + S);
+    r[0] = p[0] * S / (p[1] + S);  // qt: target uptake rate
+    r[1] = O / (p[2] + O);         // fo: oxygen saturation
+  }
+  MCST_HD static int update(const double* p, double dt, double* x, const double* r, double* contrib) {
+    const double qt = r[0];
     const double U = x[1] * x[0];
-    const double fo = O / (p[2] + O);
+    const double fo = r[1];
     const double G = p[3] * U * fo;
     const double Fst = U * (1.0 - fo);
     const double R = p[6] * x[2] * fo;
@@ -229,9 +242,12 @@ struct Monod {
     x[5] = 0.0;
   }
   MCST_HD static double mass(const double* p, const double* x) { return x[0] * lin_density(p[4]); }
-  MCST_HD static int update(const double* p, double dt, double* x, const double* c, double* contrib) {
+  MCST_HD static void record(const double* p, const double* c, double* r) {
     const double s = c[0] > 0.0 ? c[0] : 0.0;  // monod.hpp:130-160
-    const double mu = p[0] * s / (p[1] + s);
+    r[0] = p[0] * s / (p[1] + s);  // mu
+  }
+  MCST_HD static int update(const double* p, double dt, double* x, const double* r, double* contrib) {
+    const double mu = r[0];
     const double mu_eff = x[2] < mu ? x[2] : mu;
     x[0] = x[0] + dt * (mu_eff * x[4]);
     x[2] = x[2] + dt * (1.0 / p[5]) * (mu - x[2]);
@@ -268,11 +284,14 @@ struct FixedLength {
     x[1] = p[1];
   }
   MCST_HD static double mass(const double* p, const double* x) { return x[0] * lin_density(p[3]); }
-  MCST_HD static int update(const double* p, double dt, double* x, const double* c, double* contrib) {
+  MCST_HD static void record(const double* p, const double* c, double* r) {
+    const double s = c[0];
+    r[0] = s / (p[2] + s);  // g: uptake saturation
+  }
+  MCST_HD static int update(const double* p, double dt, double* x, const double* r, double* contrib) {
     const double ld = lin_density(p[3]);
     const double phi_s_max = (ld * p[0]) / 0.5;
-    const double s = c[0];
-    const double g = s / (p[2] + s);
+    const double g = r[0];
     const double phi = phi_s_max * g;
     const double ldot = p[0] * g;
     x[0] = x[0] + dt * ldot;
@@ -312,6 +331,13 @@ struct SimpleAcetate {
     for (int i = 4; i < 9; ++i) x[i] = 0.0;
   }
   MCST_HD static double mass(const double*, const double* x) { return x[0] * lin_density(D_M); }
+  // D0 and D1 multiply the concentration before the saturation factor, so
+  // no liquid-only term can be split off without changing their rounding:
+  // the record is the concentrations themselves.
+  MCST_HD static void record(const double*, const double* c, double* r) {
+    r[0] = c[0];
+    r[1] = c[1];
+  }
   MCST_HD static int update(const double*, double dt, double* x, const double* c, double* contrib) {
     const double ld = lin_density(D_M);
     const double adm0 = x[3], adm1 = x[3] / 3.0;
