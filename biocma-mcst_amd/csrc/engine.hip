@@ -16,6 +16,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <thread>
 #include <vector>
 
@@ -210,6 +211,12 @@ struct HostRun {
   std::vector<double> state;  // [nv][n]
   std::vector<double> ah, ad;
   double weight = 0.0;
+  // state restored from a checkpoint (mcst_run_restore): liquid mass and
+  // pending contributions, step, counters and probes
+  bool restored = false;
+  std::vector<double> r_m, r_S, r_exit, r_div;
+  uint32_t r_step = 0;
+  unsigned long long r_counters[C_COUNT] = {};
   // device-side placement
   int phase = 0;                 // 0 defined, 1 on the device, 2 retired
   uint64_t p_off = 0;            // first slot (a multiple of TILE)
@@ -539,14 +546,17 @@ int upload_run(mcst_engine* e, uint32_t r) {
   std::vector<double> zeros(nsc, 0.0);
   if ((rc = run_upload(h, &d.C[0], C0.data(), nsc))) return rc;
   if ((rc = run_upload(h, &d.C[1], C0.data(), nsc))) return rc;
-  if ((rc = run_upload(h, &d.m, m0.data(), nsc))) return rc;
-  if ((rc = run_upload(h, &d.S, zeros.data(), nsc))) return rc;
+  if ((rc = run_upload(h, &d.m, h.restored ? h.r_m.data() : m0.data(), nsc))) return rc;
+  if ((rc = run_upload(h, &d.S, h.restored ? h.r_S.data() : zeros.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.F, F.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.sink, sink.data(), sink.size()))) return rc;
   if ((rc = run_upload(h, &d.rec, rec.data(), rec.size()))) return rc;
   if (h.probe_cap) {
     if ((rc = run_alloc(h, (void**)&d.probe_exit, (size_t)h.probe_cap * sizeof(double)))) return rc;
     if ((rc = run_alloc(h, (void**)&d.probe_div, (size_t)h.probe_cap * sizeof(double)))) return rc;
+    const size_t ke = std::min<size_t>(h.r_exit.size(), h.probe_cap), kd = std::min<size_t>(h.r_div.size(), h.probe_cap);
+    if (ke) HIP_TRY(hipMemcpy(d.probe_exit, h.r_exit.data(), ke * sizeof(double), hipMemcpyHostToDevice));
+    if (kd) HIP_TRY(hipMemcpy(d.probe_div, h.r_div.data(), kd * sizeof(double), hipMemcpyHostToDevice));
   }
   d.p_off = h.p_off;
   d.seed = h.seed;
@@ -577,6 +587,12 @@ int upload_run(mcst_engine* e, uint32_t r) {
   uint64_t inactive = 0;
   for (uint64_t q = 0; q < h.n; ++q) inactive += h.status[q] != IDLE;
   st.inactive = (uint32_t)inactive;
+  if (h.restored) {
+    std::memcpy(st.counters, h.r_counters, sizeof(st.counters));
+    st.step = h.r_step;
+    st.probe_exit_n = std::min<uint64_t>(h.r_exit.size(), h.probe_cap);
+    st.probe_div_n = std::min<uint64_t>(h.r_div.size(), h.probe_cap);
+  }
   st.counters[C_N] = h.n;
   st.counters[C_INACTIVE] = inactive;
   HIP_TRY(hipMemcpy(e->d_runs + r, &d, sizeof(RunDev), hipMemcpyHostToDevice));
@@ -587,6 +603,10 @@ int upload_run(mcst_engine* e, uint32_t r) {
   std::vector<double>().swap(h.state);
   std::vector<double>().swap(h.ah);
   std::vector<double>().swap(h.ad);
+  std::vector<double>().swap(h.r_m);
+  std::vector<double>().swap(h.r_S);
+  std::vector<double>().swap(h.r_exit);
+  std::vector<double>().swap(h.r_div);
   h.phase = 1;
   return MCST_OK;
 }
@@ -897,6 +917,7 @@ int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double p
 
 int mcst_run_set_concentrations(mcst_engine* e, int32_t run, const double* C) {
   CHECK_DEFINE(e, run, h);
+  h->restored = false;  // a checkpoint restored earlier no longer applies
   if (!C) return fail(MCST_ERR_ARG, "null concentrations");
   const size_t nsc = (size_t)h->n_s * h->n_c;
   for (size_t i = 0; i < nsc; ++i)
@@ -958,6 +979,7 @@ static void order_by_compartment(HostRun& h, uint32_t nv) {
 
 int mcst_run_init_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t capacity, double x0) {
   CHECK_DEFINE(e, run, h);
+  h->restored = false;  // a checkpoint restored earlier no longer applies
   if (h->maps.empty()) return fail(MCST_ERR_STATE, "add a flowmap before the particles");
   if (capacity < n || capacity >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "capacity must be in [n, 2^32)");
   const ModelEntry& M = *e->model;
@@ -1007,6 +1029,7 @@ int mcst_run_set_particles(mcst_engine* e, int32_t run, uint64_t n, uint64_t cap
                            const uint8_t* status, const double* state, const double* age_hyd,
                            const double* age_div, double weight) {
   CHECK_DEFINE(e, run, h);
+  h->restored = false;  // a checkpoint restored earlier no longer applies
   if (capacity < n || capacity >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "capacity must be in [n, 2^32)");
   if (n && (!pos || !status || !state || !age_hyd || !age_div)) return fail(MCST_ERR_ARG, "null particle array");
   const uint32_t nv = (uint32_t)e->model->nv;
@@ -1253,6 +1276,32 @@ int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist) {
   return MCST_OK;
 }
 
+int mcst_run_get_properties(mcst_engine* e, int32_t run, double* values, double* spatial) {
+  CHECK_READY(e, run);
+  if (!values && !spatial) return fail(MCST_ERR_ARG, "null outputs");
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  const RunDev& d = e->h_runs[run];
+  const uint64_t rows = (uint64_t)e->model->nv + 1, n = s.n;
+  const size_t vb = rows * n * sizeof(double), sb = rows * d.n_c * sizeof(double);
+  char* buf = nullptr;
+  HIP_TRY(hipMalloc((void**)&buf, vb + sb + 8));
+  double* dv = (double*)buf;
+  double* ds = (double*)(buf + vb);
+  hipError_t err = hipMemsetAsync(ds, 0, sb, e->stream);
+  if (err == hipSuccess && n) {
+    launch_properties(e->model->id, e->bd, (uint32_t)run, n, dv, ds, e->stream);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err == hipSuccess && values && vb) err = hipMemcpy(values, dv, vb, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && spatial) err = hipMemcpy(spatial, ds, sb, hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (err != hipSuccess) return fail(MCST_ERR_DEVICE, std::string("properties: ") + hipGetErrorString(err));
+  return MCST_OK;
+}
+
 static int get_probe(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n, bool exit_ages) {
   CHECK_READY(e, run);
   RunState s;
@@ -1271,6 +1320,134 @@ int mcst_run_get_exit_ages(mcst_engine* e, int32_t run, double* out, uint64_t ma
 }
 int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n) {
   return get_probe(e, run, out, max, n, false);
+}
+
+// ---------------------------------------------------------------- checkpoints
+// A checkpoint is one flat little-endian buffer: the header below, then
+// pos u32[n], status u8[n] padded to 8 bytes, state f64[n_var][n],
+// age_hyd f64[n], age_div f64[n], C, m, S f64[n_s][n_c] each, and the
+// recorded exit and division ages.  It holds everything a step carries to
+// the next (random draws are keyed by step, so none is stored); the network,
+// feeds and model parameters are the run's definition and are not stored.
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, model, n_c, n_s, nv, step;
+  uint64_t n, cap, seed, n_exit, n_div;
+  double dt, weight;
+  unsigned long long counters[C_COUNT];
+};
+constexpr char CKPT_MAGIC[8] = {'M', 'C', 'S', 'T', 'C', 'K', 'P', 'T'};
+constexpr uint32_t CKPT_VERSION = 1;
+
+uint64_t ckpt_bytes(uint64_t n, uint32_t nv, uint64_t nsc, uint64_t n_exit, uint64_t n_div) {
+  return sizeof(CkptHeader) + 4 * n + ((n + 7) / 8) * 8 + 8 * ((uint64_t)nv * n + 2 * n + 3 * nsc + n_exit + n_div);
+}
+}  // namespace
+
+int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, uint64_t* needed) {
+  CHECK_READY(e, run);
+  RunState s;
+  int rc = read_state(e, run, s);
+  if (rc) return rc;
+  const RunDev& d = e->h_runs[run];
+  const uint32_t nv = (uint32_t)e->model->nv;
+  const uint64_t n = s.n, nsc = (uint64_t)d.n_s * d.n_c;
+  const uint64_t n_exit = std::min<uint64_t>(s.probe_exit_n, d.probe_cap);
+  const uint64_t n_div = std::min<uint64_t>(s.probe_div_n, d.probe_cap);
+  const uint64_t bytes = ckpt_bytes(n, nv, nsc, n_exit, n_div);
+  if (needed) *needed = bytes;
+  if (!buf) return MCST_OK;
+  if (size < bytes) return fail(MCST_ERR_ARG, "checkpoint buffer too small (" + std::to_string(bytes) + " bytes needed)");
+  CkptHeader hd{};
+  std::memcpy(hd.magic, CKPT_MAGIC, 8);
+  hd.version = CKPT_VERSION;
+  hd.model = (uint32_t)e->model->id;
+  hd.n_c = d.n_c;
+  hd.n_s = d.n_s;
+  hd.nv = nv;
+  hd.step = s.step;
+  hd.n = n;
+  hd.cap = d.cap;
+  hd.seed = d.seed;
+  hd.n_exit = n_exit;
+  hd.n_div = n_div;
+  hd.dt = d.dt;
+  hd.weight = d.weight;
+  std::memcpy(hd.counters, s.counters, sizeof(hd.counters));
+  char* p = (char*)buf;
+  std::memcpy(p, &hd, sizeof(hd));
+  p += sizeof(hd);
+  const BatchDev& b = e->bd;
+  const uint64_t o = d.p_off;
+  auto down = [&](const void* src, uint64_t nb) -> hipError_t {
+    hipError_t err = nb ? hipMemcpy(p, src, nb, hipMemcpyDeviceToHost) : hipSuccess;
+    p += nb;
+    return err;
+  };
+  HIP_TRY(down(b.pos + o, 4 * n));
+  HIP_TRY(down(b.status + o, n));
+  std::memset(p, 0, ((n + 7) / 8) * 8 - n);
+  p += ((n + 7) / 8) * 8 - n;
+  for (uint32_t v = 0; v < nv; ++v) HIP_TRY(down(b.state + (uint64_t)v * b.P + o, 8 * n));
+  HIP_TRY(down(b.age_hyd + o, 8 * n));
+  HIP_TRY(down(b.age_div + o, 8 * n));
+  HIP_TRY(down(d.C[s.step & 1u], 8 * nsc));
+  HIP_TRY(down(d.m, 8 * nsc));
+  HIP_TRY(down(d.S, 8 * nsc));
+  HIP_TRY(down(d.probe_exit, 8 * n_exit));
+  HIP_TRY(down(d.probe_div, 8 * n_div));
+  return MCST_OK;
+}
+
+int mcst_run_restore(mcst_engine* e, int32_t run, const void* buf, uint64_t size) {
+  CHECK_DEFINE(e, run, h);
+  if (!buf || size < sizeof(CkptHeader)) return fail(MCST_ERR_ARG, "checkpoint buffer too small");
+  CkptHeader hd;
+  std::memcpy(&hd, buf, sizeof(hd));
+  if (std::memcmp(hd.magic, CKPT_MAGIC, 8) != 0 || hd.version != CKPT_VERSION)
+    return fail(MCST_ERR_ARG, "not a checkpoint of this library version");
+  const uint32_t nv = (uint32_t)e->model->nv;
+  if (hd.model != (uint32_t)e->model->id || hd.nv != nv) return fail(MCST_ERR_ARG, "checkpoint of another model");
+  if (hd.n_c != h->n_c || hd.n_s != h->n_s || hd.dt != h->dt || hd.seed != h->seed)
+    return fail(MCST_ERR_ARG, "checkpoint of a run with another compartment count, species count, dt or seed");
+  const uint64_t n = hd.n, nsc = (uint64_t)hd.n_s * hd.n_c;
+  if (hd.cap < n || hd.cap >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "bad particle counts in the checkpoint");
+  if (size < ckpt_bytes(n, nv, nsc, hd.n_exit, hd.n_div)) return fail(MCST_ERR_ARG, "truncated checkpoint");
+  const char* p = (const char*)buf + sizeof(hd);
+  auto take = [&](auto& vec, uint64_t count) {
+    using T = typename std::decay_t<decltype(vec)>::value_type;
+    vec.resize(count);
+    if (count) std::memcpy(vec.data(), p, count * sizeof(T));
+    p += count * sizeof(T);
+  };
+  std::vector<uint32_t> pos;
+  std::vector<uint8_t> status;
+  take(pos, n);
+  take(status, n);
+  p += ((n + 7) / 8) * 8 - n;
+  for (uint64_t q = 0; q < n; ++q) {
+    if (pos[q] >= h->n_c) return fail(MCST_ERR_ARG, "particle position out of range in the checkpoint");
+    if (status[q] > DEAD) return fail(MCST_ERR_ARG, "bad particle status in the checkpoint");
+  }
+  h->pos.swap(pos);
+  h->status.swap(status);
+  take(h->state, (uint64_t)nv * n);
+  take(h->ah, n);
+  take(h->ad, n);
+  take(h->C0, nsc);
+  take(h->r_m, nsc);
+  take(h->r_S, nsc);
+  take(h->r_exit, hd.n_exit);
+  take(h->r_div, hd.n_div);
+  h->n = n;
+  h->cap = hd.cap;
+  h->weight = hd.weight;
+  h->has_particles = true;
+  h->restored = true;
+  h->r_step = hd.step;
+  std::memcpy(h->r_counters, hd.counters, sizeof(hd.counters));
+  return MCST_OK;
 }
 
 }  // extern "C"

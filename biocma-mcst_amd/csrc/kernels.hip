@@ -1130,6 +1130,36 @@ __global__ void __launch_bounds__(256) k_repartition(BatchDev b, uint32_t r, uns
   }
 }
 
+// ------------------------------------------------------------------ properties
+// Per-particle export values of run r (get_properties, post_process.hpp:54-98):
+// for every Idle slot the model state and M::mass, row-major [n_var+1][n]
+// (zero for inactive slots), and their sums per compartment [n_var+1][n_c].
+template <class M>
+__global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint64_t n, double* values,
+                                                    double* spatial) {
+  constexpr int NV = M::NV;
+  const RunDev& rd = b.runs[r];
+  const uint32_t n_c = rd.n_c;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = rd.p_off + s;
+    if (b.status[g] != IDLE) {
+      for (int v = 0; v <= NV; ++v) values[(uint64_t)v * n + s] = 0.0;
+      continue;
+    }
+    const uint32_t c = b.pos[g];
+    double x[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      x[v] = b.state[(uint64_t)v * b.P + g];
+      values[(uint64_t)v * n + s] = x[v];
+      unsafeAtomicAdd(spatial + (uint64_t)v * n_c + c, x[v]);
+    }
+    const double m = M::mass(rd.params, x);
+    values[(uint64_t)NV * n + s] = m;
+    unsafeAtomicAdd(spatial + (uint64_t)NV * n_c + c, m);
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <class M, int BLK, bool PRIV, bool PROBE>
 static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
@@ -1230,6 +1260,17 @@ void launch_layout(const BatchDev& b, hipStream_t s) {
 }
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s) {
   hipLaunchKernelGGL(k_repartition, dim3(1024), dim3(256), 0, s, b, run, hist);
+}
+void launch_properties(int model_id, const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial,
+                       hipStream_t s) {
+  switch (model_id) {
+    case MonodMass::ID: hipLaunchKernelGGL(k_properties<MonodMass>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
+    case DualMonod::ID: hipLaunchKernelGGL(k_properties<DualMonod>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
+    case Monod::ID: hipLaunchKernelGGL(k_properties<Monod>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
+    case FixedLength::ID: hipLaunchKernelGGL(k_properties<FixedLength>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
+    case SimpleAcetate::ID: hipLaunchKernelGGL(k_properties<SimpleAcetate>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
+    default: break;
+  }
 }
 
 }  // namespace mcst

@@ -28,5 +28,7 @@ void launch_resolve(int model_id, const BatchDev& b, int grid, hipStream_t s);
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_layout(const BatchDev& b, hipStream_t s);
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s);
+void launch_properties(int model_id, const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial,
+                       hipStream_t s);
 
 }  // namespace mcst

@@ -33,6 +33,7 @@ EXPORTS = [
     "mcst_run_get_mass", "mcst_run_get_contributions", "mcst_run_get_repartition",
     "mcst_run_get_exit_ages", "mcst_run_get_division_ages",
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
+    "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -95,6 +96,9 @@ def lib():
             "mcst_run_get_repartition": [_vp, i32, _u64p],
             "mcst_run_get_exit_ages": [_vp, i32, _vp, u64, C.POINTER(u64)],
             "mcst_run_get_division_ages": [_vp, i32, _vp, u64, C.POINTER(u64)],
+            "mcst_run_checkpoint": [_vp, i32, _vp, u64, C.POINTER(u64)],
+            "mcst_run_restore": [_vp, i32, _vp, u64],
+            "mcst_run_get_properties": [_vp, i32, _vp, _vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -152,6 +156,7 @@ class Engine:
         rid = C.c_int32()
         _ck(lib().mcst_run_create(self.h, n_c, self.n_s, dt, seed, C.byref(rid)), "run_create")
         run = Run(self, rid.value, n_c)
+        run.dt = float(dt)
         self.runs.append(run)
         return run
 
@@ -314,3 +319,26 @@ class Run:
 
     def division_ages(self):
         return self._probe("mcst_run_get_division_ages")
+
+    def properties(self):
+        """(values [n_var+1][n], spatial [n_var+1][n_c]): each Idle particle's
+        state and mass, and their per-compartment sums (mcst_run_get_properties)."""
+        n = self.n
+        values = np.zeros((self.n_var + 1, n))
+        spatial = np.zeros((self.n_var + 1, self.n_c))
+        _ck(lib().mcst_run_get_properties(self._h, self.id, values.ctypes.data if n else None,
+                                          spatial.ctypes.data), "get_properties")
+        return values, spatial
+
+    def checkpoint(self):
+        """Everything the next step needs, as bytes (mcst_run_checkpoint)."""
+        n = C.c_uint64()
+        _ck(lib().mcst_run_checkpoint(self._h, self.id, None, 0, C.byref(n)), "checkpoint")
+        buf = np.zeros(n.value, np.uint8)
+        _ck(lib().mcst_run_checkpoint(self._h, self.id, buf.ctypes.data, n.value, C.byref(n)), "checkpoint")
+        return buf.tobytes()
+
+    def restore(self, blob):
+        """Load a checkpoint into this defined run (same n_c, n_s, dt, seed)."""
+        buf = np.frombuffer(blob, np.uint8)
+        _ck(lib().mcst_run_restore(self._h, self.id, buf.ctypes.data, len(buf)), "restore")

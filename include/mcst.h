@@ -186,8 +186,30 @@ int mcst_run_get_mass(mcst_engine* e, int32_t run, double* m);
 int mcst_run_get_contributions(mcst_engine* e, int32_t run, double* S);
 int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c] */);
 /* Exit / division ages recorded so far (up to the probe capacity). */
+/* Export values (PostProcessing::get_properties, apps/core/public/core/post_process.hpp:54-98,173-190):
+ * values[n_var+1][n]: each Idle slot's model state then Model::mass (zero
+ * for inactive slots); spatial[n_var+1][n_c]: their sums per compartment
+ * over Idle particles.  Either output may be NULL. */
+int mcst_run_get_properties(mcst_engine* e, int32_t run, double* values, double* spatial);
 int mcst_run_get_exit_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n);
 int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n);
+
+/* ---- checkpoint / resume (replaces the reference's serde of a run:
+ * apps/core/src/serde.cpp:65-104 and the container's save/load,
+ * apps/libs/mc/public/mc/particles_container.hpp:491-537).
+ * mcst_run_checkpoint writes everything a step carries to the next into a
+ * caller buffer: particles (positions, status, state, both ages), liquid
+ * concentrations, liquid mass and the pending contributions, the step count,
+ * the counters and the recorded probes.  With buf == NULL (or size too small:
+ * then MCST_ERR_ARG) it only reports the size in *needed.
+ * mcst_run_restore loads such a buffer into a defined, not yet committed or
+ * admitted run created with the same compartment count, species count, dt
+ * and seed; the caller defines its flowmaps, feeds and parameters as for the
+ * original run.  Stepping it then continues the original run exactly.
+ * A later set_concentrations / init_particles / set_particles on that run
+ * discards the restored state. */
+int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, uint64_t* needed);
+int mcst_run_restore(mcst_engine* e, int32_t run, const void* buf, uint64_t size);
 
 #ifdef __cplusplus
 }

@@ -1164,6 +1164,26 @@ int orc_get_repartition(const orc_run* r, uint64_t* hist) {
     if (r->status[p] == ORC_IDLE) hist[r->pos[p]]++;
   return 0;
 }
+/* post_process.hpp:54-98: per Idle particle its state and Model::mass
+   (values[n_var+1][n], zero for inactive slots), summed per compartment in
+   slot order (spatial[n_var+1][n_c]). */
+int orc_get_properties(const orc_run* r, double* values, double* spatial) {
+  const uint64_t n = r->n;
+  const int nv = r->n_var;
+  memset(spatial, 0, sizeof(double) * (size_t)(nv + 1) * r->n_c);
+  double x[MAX_VAR];
+  for (uint64_t p = 0; p < n; ++p) {
+    const int idle = r->status[p] == ORC_IDLE;
+    for (int v = 0; v < nv; ++v) x[v] = r->state[(size_t)v * r->cap + p];
+    const double m = model_mass(r->model, r->params, x);
+    for (int v = 0; v <= nv; ++v) {
+      const double val = idle ? (v < nv ? x[v] : m) : 0.0;
+      values[(size_t)v * n + p] = val;
+      if (idle) spatial[(size_t)v * r->n_c + r->pos[p]] += val;
+    }
+  }
+  return 0;
+}
 uint64_t orc_get_exit_ages(const orc_run* r, double* out, uint64_t max) {
   const uint64_t n = r->exit_ages.n < max ? r->exit_ages.n : max;
   if (out && n) memcpy(out, r->exit_ages.v, sizeof(double) * n);

@@ -115,6 +115,7 @@ int orc_get_mass(const orc_run* r, double* m);
 int orc_get_contributions(const orc_run* r, double* S);
 int orc_get_counters(const orc_run* r, uint64_t* c);
 int orc_get_repartition(const orc_run* r, uint64_t* hist);
+int orc_get_properties(const orc_run* r, double* values, double* spatial);
 uint64_t orc_get_exit_ages(const orc_run* r, double* out, uint64_t max);
 uint64_t orc_get_division_ages(const orc_run* r, double* out, uint64_t max);
 

@@ -89,6 +89,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, _f64p]
         L.orc_get_counters.argtypes = [C.c_void_p, _u64p]
         L.orc_get_repartition.argtypes = [C.c_void_p, _u64p]
+        L.orc_get_properties.argtypes = [C.c_void_p, _f64p, _f64p]
         for name in ("orc_get_exit_ages", "orc_get_division_ages"):
             getattr(L, name).restype = C.c_uint64
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
@@ -265,6 +266,13 @@ class OracleRun:
         h = np.zeros(self.n_c, np.uint64)
         _check(lib().orc_get_repartition(self.h, h), "repartition")
         return h
+
+    def properties(self):
+        n = self.n
+        values = np.zeros((self.n_var + 1, max(n, 1)))
+        spatial = np.zeros((self.n_var + 1, self.n_c))
+        _check(lib().orc_get_properties(self.h, values, spatial), "properties")
+        return values[:, :n], spatial
 
     def exit_ages(self):
         n = lib().orc_get_exit_ages(self.h, None, 0)
