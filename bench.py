@@ -115,8 +115,9 @@ def served_measurement(n_requests=40, rate=20.0, seed=13, chunk=20):
 
 
 def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
-    """The batched ensemble: n_runs uneven runs in one engine, steps captured
-    in a graph of `graph` steps."""
+    """The batched ensemble: n_runs uneven runs in one engine; the particle
+    pass's launch shape is chosen by the autotuner over the first steps, the
+    rest are captured in a graph of `graph` steps and replayed."""
     import mcst
     from mcst import cases
 
@@ -128,11 +129,13 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
         c.apply(r)
         runs.append(r)
     eng.commit()
+    tuned = eng.autotune(2)  # its trial steps are steps of the runs
+    done = runs[0].counters()["steps"]
     eng.capture(graph)
     eng.replay(1)
     eng.synchronize()
     p0 = sum(r.counters()["processed"] for r in runs)
-    reps = max(1, (steps - graph) // graph)
+    reps = max(1, (steps - done - graph) // graph)
     t0 = time.perf_counter()
     eng.replay(reps)
     eng.synchronize()
@@ -141,7 +144,8 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
     eng.close()
     return {"runs": n_runs, "particles": int(sum(c.n_particles for c in cs)),
             "compartments": int(sum(c.n_c for c in cs)), "steps_timed": reps * graph,
-            "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / (reps * graph)}
+            "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / (reps * graph),
+            "autotuned": {"block": tuned["block"], "tile_group": tuned["group"], "trial_steps": done}}
 
 
 def copy_calibration(mib, reps=5):

@@ -253,6 +253,8 @@ struct mcst_engine {
   uint32_t* d_comp = nullptr;
   int grid_particle = 1, grid_transport = 1;
   mcst::ParticleLaunch plaunch;
+  int tuned_block = 0;              // particle-pass shape chosen by mcst_engine_autotune (0: none)
+  int tuned_group = -1;
   int grid_move = 1;
   bool removal_possible = true;
   hipGraph_t graph = nullptr;
@@ -414,7 +416,7 @@ int choose_launch(mcst_engine* e) {
                                           96 * 1024);
   bool priv = lds_need <= lds_cap;
   if (const char* v = std::getenv("MCST_CONTRIB_LDS")) priv = priv && std::atoi(v) != 0;
-  int forced_block = 0;
+  int forced_block = e->tuned_block;
   if (const char* v = std::getenv("MCST_PARTICLE_BLOCK")) forced_block = std::atoi(v);
   ParticleLaunch best;
   int best_waves = -1;
@@ -456,7 +458,7 @@ int choose_launch(mcst_engine* e) {
   // one run: blocks take groups in turn, which keeps the wave front of the
   // whole grid inside a narrow window of the particle arrays; a batch: each
   // block takes one contiguous range, so it meets few runs
-  b.tile_group = live <= 1 ? 1u : 0u;
+  b.tile_group = e->tuned_group >= 0 ? (uint32_t)e->tuned_group : (live <= 1 ? 1u : 0u);
   if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
   e->plaunch = best;
   e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
@@ -1127,6 +1129,49 @@ int mcst_engine_replay(mcst_engine* e, uint32_t times) {
   if (!e->graph_exec) return fail(MCST_ERR_STATE, "no captured graph");
   HIP_TRY(hipSetDevice(e->device));
   for (uint32_t k = 0; k < times; ++k) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+  return MCST_OK;
+}
+
+// Times the particle pass over `steps` real steps for each launch shape
+// (block size x tile grouping) and keeps the fastest.  Results do not depend
+// on the shape (every event is decided per slot and per tile; only the order
+// of the FP64 contribution atomics changes), so the trial steps are ordinary
+// steps of the runs.
+int mcst_engine_autotune(mcst_engine* e, uint32_t steps, int32_t* block, int32_t* group, double* particle_ms) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  if (steps == 0) return fail(MCST_ERR_ARG, "tune over at least one step");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  drop_graph(e);
+  double best = 1e300;
+  int best_block = 0, best_group = -1;
+  int rc = MCST_OK;
+  for (int blk : {256, 512, 1024}) {
+    e->tuned_block = blk;
+    e->tuned_group = -1;
+    if ((rc = choose_launch(e))) break;
+    if (e->plaunch.block != blk || (e->plaunch.probe && blk != 256)) continue;  // not resident / same kernel
+    for (int g : {0, 1, 2, 4, 8}) {
+      e->bd.tile_group = (uint32_t)g;
+      double tot = 0.0, per[6] = {};
+      if ((rc = mcst_engine_profile(e, steps, &tot, per))) break;
+      if (per[1] < best) {
+        best = per[1];
+        best_block = blk;
+        best_group = g;
+      }
+    }
+    if (rc) break;
+  }
+  e->tuned_block = best_block;
+  e->tuned_group = best_group;
+  const int rc2 = choose_launch(e);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (block) *block = best_block;
+  if (group) *group = best_group;
+  if (particle_ms) *particle_ms = best;
   return MCST_OK;
 }
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "mcst_run_get_mass", "mcst_run_get_contributions", "mcst_run_get_repartition",
     "mcst_run_get_exit_ages", "mcst_run_get_division_ages",
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
-    "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties",
+    "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -99,6 +99,7 @@ def lib():
             "mcst_run_checkpoint": [_vp, i32, _vp, u64, C.POINTER(u64)],
             "mcst_run_restore": [_vp, i32, _vp, u64],
             "mcst_run_get_properties": [_vp, i32, _vp, _vp],
+            "mcst_engine_autotune": [_vp, u32, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -194,6 +195,13 @@ class Engine:
 
     def synchronize(self):
         _ck(lib().mcst_engine_synchronize(self.h), "synchronize")
+
+    def autotune(self, steps=3):
+        """Pick the particle pass's launch shape by timing each candidate over
+        `steps` real steps (mcst_engine_autotune); returns the choice."""
+        blk, grp, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _ck(lib().mcst_engine_autotune(self.h, steps, C.byref(blk), C.byref(grp), C.byref(ms)), "autotune")
+        return {"block": blk.value, "group": grp.value, "particle_ms": ms.value}
 
     def profile(self, n_steps):
         """(total ms, per-kernel mean ms: transport+layout, particle, resolve, bookkeep, removal, merge)."""

@@ -172,6 +172,14 @@ int mcst_engine_synchronize(mcst_engine* e);
  * together, skipped when no particle can become inactive), merge
  * -> per_kernel_ms[6]. */
 int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, double* per_kernel_ms);
+/* Autotuner of the particle pass (no counterpart in the reference; the
+ * launch shapes of its Kokkos kernels are fixed).  Runs `steps` real steps
+ * with each candidate shape (threads per block 256/512/1024 x consecutive
+ * tiles per block turn 0/1/2/4/8), times the particle pass with HIP events,
+ * and keeps the fastest for the engine's later steps (also across admit and
+ * retire).  The runs advance by the trial steps; their results do not depend
+ * on the shape.  Drops a captured graph. */
+int mcst_engine_autotune(mcst_engine* e, uint32_t steps, int32_t* block, int32_t* group, double* particle_ms);
 
 /* ---- results ---- */
 int mcst_run_get_counters(mcst_engine* e, int32_t run, uint64_t* counters /* MCST_C_COUNT */);
