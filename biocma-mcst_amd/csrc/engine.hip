@@ -62,7 +62,7 @@ ModelEntry entry() {
 
 const std::vector<ModelEntry>& registry() {
   static const std::vector<ModelEntry> r = {entry<MonodMass>(), entry<DualMonod>(), entry<Monod>(),
-                                            entry<FixedLength>(), entry<SimpleAcetate>()};
+                                            entry<FixedLength>(), entry<SimpleAcetate>(), entry<QuadMonod>()};
   return r;
 }
 
@@ -198,6 +198,7 @@ struct HostRun {
   uint32_t steps_per_map = 1;
   std::vector<HostFeed> feeds;
   std::vector<double> C0;
+  std::vector<double> kla, cstar;  // gas-liquid transfer, [n_s][n_c] (empty: none)
   double ratio = 0.01;
   uint64_t min_removal = 0;
   uint32_t stage_cap = 64;
@@ -534,6 +535,7 @@ int upload_run(mcst_engine* e, uint32_t r) {
     if ((rc = run_upload(h, (uint32_t**)&md.src, mp.src.data(), mp.src.size()))) return rc;
     if ((rc = run_upload(h, (double**)&md.val, mp.val.data(), mp.val.size()))) return rc;
     if ((rc = run_upload(h, (double**)&md.inv_v, mp.inv_v.data(), mp.inv_v.size()))) return rc;
+    if ((rc = run_upload(h, (double**)&md.vol, mp.V.data(), mp.V.size()))) return rc;
     e->h_maps.push_back(md);
   }
   const size_t nsc = (size_t)h.n_s * h.n_c;
@@ -552,6 +554,10 @@ int upload_run(mcst_engine* e, uint32_t r) {
   if ((rc = run_upload(h, &d.S, h.restored ? h.r_S.data() : zeros.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.F, F.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.sink, sink.data(), sink.size()))) return rc;
+  if (!h.kla.empty()) {
+    if ((rc = run_upload(h, (double**)&d.kla, h.kla.data(), nsc))) return rc;
+    if ((rc = run_upload(h, (double**)&d.cstar, h.cstar.data(), nsc))) return rc;
+  }
   if ((rc = run_upload(h, &d.rec, rec.data(), rec.size()))) return rc;
   if (h.probe_cap) {
     if ((rc = run_alloc(h, (void**)&d.probe_exit, (size_t)h.probe_cap * sizeof(double)))) return rc;
@@ -933,6 +939,18 @@ int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min
   if (!(ratio >= 0.0)) return fail(MCST_ERR_ARG, "removal ratio must be >= 0");
   h->ratio = ratio;
   h->min_removal = min_removal;
+  return MCST_OK;
+}
+
+int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, const double* c_star) {
+  CHECK_DEFINE(e, run, h);
+  if (!kla || !c_star) return fail(MCST_ERR_ARG, "null gas-liquid transfer arrays");
+  const size_t nsc = (size_t)h->n_s * h->n_c;
+  for (size_t i = 0; i < nsc; ++i)
+    if (!(kla[i] >= 0.0) || !std::isfinite(kla[i]) || !std::isfinite(c_star[i]))
+      return fail(MCST_ERR_ARG, "gas-liquid transfer coefficients must be finite and non-negative");
+  h->kla.assign(kla, kla + nsc);
+  h->cstar.assign(c_star, c_star + nsc);
   return MCST_OK;
 }
 

@@ -199,7 +199,10 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         const size_t o = (size_t)s * n_c + j;
-        const double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + rd.F[o]);
+        double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + rd.F[o]);
+        // gas-liquid transfer from a fixed gas phase (performStepGL,
+        // implScalar.cpp:228-248): kla*(C* - C)*V
+        if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];
         const double mm = rd.m[o] + rd.dt * dmdt;
         rd.m[o] = mm;
         cn[s] = mm * inv_v;
@@ -1207,6 +1210,7 @@ void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, h
     case Monod::ID: launch_t<Monod>(b, L, s); break;
     case FixedLength::ID: launch_t<FixedLength>(b, L, s); break;
     case SimpleAcetate::ID: launch_t<SimpleAcetate>(b, L, s); break;
+    case QuadMonod::ID: launch_t<QuadMonod>(b, L, s); break;
     default: break;
   }
 }
@@ -1218,6 +1222,7 @@ const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe) 
     case Monod::ID: return kernel_ptr_t<Monod>(block, priv, probe);
     case FixedLength::ID: return kernel_ptr_t<FixedLength>(block, priv, probe);
     case SimpleAcetate::ID: return kernel_ptr_t<SimpleAcetate>(block, priv, probe);
+    case QuadMonod::ID: return kernel_ptr_t<QuadMonod>(block, priv, probe);
     default: return nullptr;
   }
 }
@@ -1229,6 +1234,7 @@ void launch_transport(int model_id, const BatchDev& b, int grid, hipStream_t s) 
     case Monod::ID: hipLaunchKernelGGL(k_transport<Monod>, dim3(grid), dim3(256), 0, s, b); break;
     case FixedLength::ID: hipLaunchKernelGGL(k_transport<FixedLength>, dim3(grid), dim3(256), 0, s, b); break;
     case SimpleAcetate::ID: hipLaunchKernelGGL(k_transport<SimpleAcetate>, dim3(grid), dim3(256), 0, s, b); break;
+    case QuadMonod::ID: hipLaunchKernelGGL(k_transport<QuadMonod>, dim3(grid), dim3(256), 0, s, b); break;
     default: break;
   }
 }
@@ -1252,6 +1258,7 @@ void launch_resolve(int model_id, const BatchDev& b, int grid, hipStream_t s) {
     case Monod::ID: hipLaunchKernelGGL(k_resolve<Monod>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
     case FixedLength::ID: hipLaunchKernelGGL(k_resolve<FixedLength>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
     case SimpleAcetate::ID: hipLaunchKernelGGL(k_resolve<SimpleAcetate>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
+    case QuadMonod::ID: hipLaunchKernelGGL(k_resolve<QuadMonod>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, to); break;
     default: break;
   }
 }
@@ -1269,6 +1276,7 @@ void launch_properties(int model_id, const BatchDev& b, uint32_t run, uint64_t n
     case Monod::ID: hipLaunchKernelGGL(k_properties<Monod>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
     case FixedLength::ID: hipLaunchKernelGGL(k_properties<FixedLength>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
     case SimpleAcetate::ID: hipLaunchKernelGGL(k_properties<SimpleAcetate>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
+    case QuadMonod::ID: hipLaunchKernelGGL(k_properties<QuadMonod>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial); break;
     default: break;
   }
 }

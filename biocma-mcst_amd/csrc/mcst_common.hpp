@@ -66,6 +66,7 @@ struct MapDev {
   const uint32_t* src;      // [nnz]   source compartment of each column entry
   const double* val;        // [nnz]   F(i,j) off-diagonal, -Q_out(j) on the diagonal
   const double* inv_v;      // [n_c]
+  const double* vol;        // [n_c]   liquid volumes V
   uint32_t m;
   uint32_t pad;
 };
@@ -84,6 +85,8 @@ struct RunDev {
   double* S;                // [n_s][n_c] particle sources of the current step
   const double* F;          // [n_s][n_c] constant feed sources
   const double* sink;       // [n_c]      outlet flow per compartment
+  const double* kla;        // [n_s][n_c] gas-liquid transfer coefficient (null: no transfer)
+  const double* cstar;      // [n_s][n_c] saturation concentration set by the gas phase
   double* rec;              // [n_c][rec] per-compartment record read by the particle pass
   double* probe_exit;       // exit ages (residence times), may be null
   double* probe_div;        // division ages, may be null

@@ -381,4 +381,71 @@ struct SimpleAcetate {
   }
 };
 
+// ---------------------------------------------------------------- quad_monod
+// Four-species uptake model of the 50 000-compartment lattice case: glucose
+// S, oxygen O, acetate A and CO2.  Lagged glucose and acetate uptake rates;
+// the glucose taken up is respired in proportion to the oxygen saturation and
+// the rest overflows to acetate (a fraction Yas is excreted, the rest stored);
+// acetate is taken up aerobically, repressed by glucose.
+// x = {m, qs, qa, X, o2_rate, overflow, m_birth, generation};
+// p = {qs_max, Ks, Ko, Ka, qa_max, Yxs, Yxa, Yos, Yoa, Yas, tau, Yco2, m_div, m0_mean, m0_sd}.
+struct QuadMonod {
+  static constexpr int ID = 5, NV = 8, NS = 4, NP = 15;
+  static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr const char* NAME = "quad_monod";
+  static void defaults(double* p) {
+    p[0] = 2.0; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.01; p[4] = 0.5; p[5] = 0.5; p[6] = 0.3; p[7] = 0.3;
+    p[8] = 0.4; p[9] = 0.3; p[10] = 0.2; p[11] = 0.4; p[12] = 2e-15; p[13] = 1.4e-15; p[14] = 0.2e-15;
+  }
+  static void init(const double* p, double* x, uint64_t seed, uint32_t slot) {
+    const U4 w = draw4(seed, slot, 0u, TAG_INIT, 0u);
+    x[0] = trunc_normal(p[13], p[14], 0.5 * p[12], p[12], u01_52(w.x, w.y));
+    x[1] = 0.5 * p[0];
+    x[2] = 0.0;
+    x[3] = 0.0;
+    x[4] = 0.0;
+    x[5] = 0.0;
+    x[6] = x[0];
+    x[7] = 0.0;
+  }
+  MCST_HD static double mass(const double*, const double* x) { return x[0]; }
+  MCST_HD static void record(const double* p, const double* c, double* r) {
+    const double S = c[0] > 0.0 ? c[0] : 0.0;
+    const double O = c[1] > 0.0 ? c[1] : 0.0;
+    const double A = c[2] > 0.0 ? c[2] : 0.0;
+    r[0] = S / (p[1] + S);     // glucose saturation
+    r[1] = O / (p[2] + O);     // oxygen saturation
+    r[2] = A / (p[3] + A);     // acetate saturation
+    r[3] = p[1] / (p[1] + S);  // glucose repression of acetate uptake
+  }
+  MCST_HD static int update(const double* p, double dt, double* x, const double* r, double* contrib) {
+    const double qs_t = p[0] * r[0];
+    const double qa_t = p[4] * r[2] * r[3];
+    const double Us = x[1] * x[0];
+    const double Ua = x[2] * x[0] * r[1];
+    const double Resp = Us * r[1];
+    const double Ovf = Us - Resp;
+    const double G = p[5] * Resp + p[6] * Ua;
+    const double O2 = p[7] * Resp + p[8] * Ua;
+    contrib[0] = -Us;
+    contrib[1] = -O2;
+    contrib[2] = p[9] * Ovf - Ua;
+    contrib[3] = p[11] * (Resp + Ua);
+    x[0] = x[0] + dt * G;
+    x[3] = x[3] + dt * (Ovf * (1.0 - p[9]));
+    x[1] = x[1] + (dt / p[10]) * (qs_t - x[1]);
+    x[2] = x[2] + (dt / p[10]) * (qa_t - x[2]);
+    x[4] = O2;
+    x[5] = Ovf;
+    return (x[0] >= p[12]) ? DIVISION : IDLE;
+  }
+  MCST_HD static void division(const double*, double* x, double* xd, const RngKey&) {
+    x[0] = x[0] * 0.5;
+    x[3] = x[3] * 0.5;
+    x[6] = x[0];
+    x[7] = x[7] + 1.0;
+    for (int i = 0; i < NV; ++i) xd[i] = x[i];
+  }
+};
+
 }  // namespace mcst

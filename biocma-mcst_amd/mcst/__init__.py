@@ -34,6 +34,7 @@ EXPORTS = [
     "mcst_run_get_exit_ages", "mcst_run_get_division_ages",
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
+    "mcst_run_set_gas_transfer",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -100,6 +101,7 @@ def lib():
             "mcst_run_restore": [_vp, i32, _vp, u64],
             "mcst_run_get_properties": [_vp, i32, _vp, _vp],
             "mcst_engine_autotune": [_vp, u32, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)],
+            "mcst_run_set_gas_transfer": [_vp, i32, _f64p, _f64p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -249,6 +251,15 @@ class Run:
 
     def set_stage_cap(self, cap):
         _ck(lib().mcst_run_set_division_cap(self._h, self.id, cap), "set_division_cap")
+
+    def set_gas_transfer(self, kla, cstar):
+        """kla [1/s] and saturation C* [kg/m3], per species (broadcast over
+        compartments) or [n_s][n_c]."""
+        kla = np.ascontiguousarray(np.broadcast_to(np.asarray(kla, float).reshape(self.n_s, -1),
+                                                   (self.n_s, self.n_c)), np.float64)
+        cstar = np.ascontiguousarray(np.broadcast_to(np.asarray(cstar, float).reshape(self.n_s, -1),
+                                                     (self.n_s, self.n_c)), np.float64)
+        _ck(lib().mcst_run_set_gas_transfer(self._h, self.id, kla, cstar), "set_gas_transfer")
 
     def set_probes(self, cap):
         _ck(lib().mcst_run_set_probes(self._h, self.id, cap), "set_probes")

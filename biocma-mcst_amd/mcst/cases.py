@@ -33,6 +33,7 @@ class Case:
     capacity: int = 0
     x0: float = 1.0
     n_steps: int = 1
+    gas: tuple = None               # (kla[n_s], C*[n_s]) gas-liquid transfer, or None
 
     def apply(self, run, particles=True):
         if self.params is not None:
@@ -43,6 +44,8 @@ class Case:
         for f in self.feeds:
             run.add_feed(*f)
         run.set_concentrations(self.C0)
+        if self.gas is not None:
+            run.set_gas_transfer(*self.gas)
         if particles:
             run.init_particles(self.n_particles, self.capacity or 2 * self.n_particles, self.x0)
         return run
@@ -124,12 +127,112 @@ def config1(n_particles=10_000_000, n_steps=1000, seed=7, n_c=5000, n_maps=20, s
                 C0=C0, params=p, n_particles=n_particles, x0=1.0, n_steps=n_steps)
 
 
+def lattice_tank(dims=(50, 50, 20), loop_flow=1e-3, exchange=(1e-6, 1e-4), rng=None):
+    """A stirred-tank-like lattice: compartment (x, y, z) is z + NZ*(y + NY*x)
+    (vertical index fastest, so vertical neighbours are adjacent ids).
+    Recirculation: every column and its outward neighbour column (one cell
+    further from the axis) carry two closed loops of `loop_flow` m3/s: out
+    at impeller height, down the outer column, in at the bottom, up the
+    inner column (lower loop), and out at impeller height, up the outer
+    column, in at the top, down the inner column (upper loop).  Loops are
+    closed, so every compartment's inflow equals its outflow.  On top, every
+    lattice edge exchanges a log-uniform flow in both directions.
+    Returns (rows, cols, flows) summed per directed pair."""
+    rng = rng or np.random.default_rng(0)
+    NX, NY, NZ = dims
+    n = NX * NY * NZ
+
+    def cid(x, y, z):
+        return z + NZ * (y + NY * x)
+
+    pairs = {}
+
+    def add(a, b, q):
+        pairs[(a, b)] = pairs.get((a, b), 0.0) + q
+
+    cx, cy, zm = (NX - 1) / 2.0, (NY - 1) / 2.0, NZ // 2
+    for x in range(NX):
+        for y in range(NY):
+            dx, dy = x - cx, y - cy
+            if abs(dx) >= abs(dy):
+                ox, oy = x + (1 if dx > 0 else -1), y
+            else:
+                ox, oy = x, y + (1 if dy > 0 else -1)
+            if not (0 <= ox < NX and 0 <= oy < NY):
+                continue
+            inner = [cid(x, y, z) for z in range(NZ)]
+            outer = [cid(ox, oy, z) for z in range(NZ)]
+            # lower loop: out at zm-1, down the outer column, in at 0, up the inner column
+            add(inner[zm - 1], outer[zm - 1], loop_flow)
+            for z in range(zm - 1, 0, -1):
+                add(outer[z], outer[z - 1], loop_flow)
+            add(outer[0], inner[0], loop_flow)
+            for z in range(0, zm - 1):
+                add(inner[z], inner[z + 1], loop_flow)
+            # upper loop: out at zm, up the outer column, in at the top, down the inner column
+            add(inner[zm], outer[zm], loop_flow)
+            for z in range(zm, NZ - 1):
+                add(outer[z], outer[z + 1], loop_flow)
+            add(outer[NZ - 1], inner[NZ - 1], loop_flow)
+            for z in range(NZ - 1, zm, -1):
+                add(inner[z], inner[z - 1], loop_flow)
+    keys = np.array(list(pairs.keys()), dtype=np.int64).reshape(-1, 2)
+    vals = np.array(list(pairs.values()))
+    # exchanges on every lattice edge
+    g = np.arange(n).reshape(NX, NY, NZ)
+    e = np.concatenate([np.stack([g[:-1].ravel(), g[1:].ravel()], 1), np.stack([g[:, :-1].ravel(), g[:, 1:].ravel()], 1),
+                        np.stack([g[:, :, :-1].ravel(), g[:, :, 1:].ravel()], 1)])
+    q = np.exp(rng.uniform(math.log(exchange[0]), math.log(exchange[1]), len(e)))
+    rows = np.concatenate([keys[:, 0], e[:, 0], e[:, 1]])
+    cols = np.concatenate([keys[:, 1], e[:, 1], e[:, 0]])
+    flows = np.concatenate([vals, q, q])
+    # sum duplicates (a loop pair that is also an exchange pair)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    key, rows, cols, flows = key[order], rows[order], cols[order], flows[order]
+    first = np.concatenate([[True], key[1:] != key[:-1]])
+    idx = np.cumsum(first) - 1
+    summed = np.zeros(first.sum())
+    np.add.at(summed, idx, flows)
+    return rows[first].astype(np.uint32), cols[first].astype(np.uint32), summed
+
+
+QUAD_MONOD_CONFIG4 = np.array([2.0, 0.01, 1e-4, 0.01, 0.5, 0.5, 0.3, 0.3, 0.4, 0.3, 0.2, 0.4, 2e-15, 1.5e-15, 0.2e-15])
+
+
+def config4(n_particles=100_000_000, n_steps=200, seed=17, dims=(50, 50, 20), feed_flow=2e-3, kla_o2=1.0):
+    """50x50x20 lattice tank (50 000 compartments, 1 L each +-20 %), loop flows
+    1e-3 m3/s plus log-uniform 1e-6..1e-4 m3/s exchanges; feeds enter through
+    the bottom layer and leave through the top layer (one feed per column,
+    `feed_flow` m3/s each); 4 species (glucose 5 g/L, O2 8 mg/L with a fixed
+    gas phase supplying it at kla = `kla_o2` 1/s, acetate, CO2); quad_monod
+    (8 FP64 states) growing about two-fold over the 200 steps of 5 ms while
+    the outlet washes particles out."""
+    rng = np.random.default_rng(seed)
+    NX, NY, NZ = dims
+    n = NX * NY * NZ
+    rows, cols, flows = lattice_tank(dims, rng=rng)
+    V = rng.uniform(0.8, 1.2, n) * L
+    feeds = []
+    for x in range(NX):
+        for y in range(NY):
+            bottom = 0 + NZ * (y + NY * x)
+            top = NZ - 1 + NZ * (y + NY * x)
+            feeds.append(("constant", feed_flow, 0.0, bottom, top, [0, 1], [10.0, 8e-3]))
+    C0 = np.stack([np.full(n, 5.0), np.full(n, 8e-3), np.zeros(n), np.zeros(n)])
+    gas = (np.array([0.0, kla_o2, 0.0, 0.0]), np.array([0.0, 8e-3, 0.0, 0.0]))
+    return Case("config4_lattice50k", "quad_monod", n, 4, 0.005, seed, [(V, rows, cols, flows)], feeds=feeds,
+                C0=C0, params=QUAD_MONOD_CONFIG4.copy(), n_particles=n_particles, capacity=int(2.5 * n_particles),
+                x0=0.005, n_steps=n_steps, gas=gas)
+
+
 def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, steps_per_map=3,
                feeds=True, fast=True):
     """A small network for parity tests: chords + feed/outlet, fast growth so
     that divisions, moves and exits all happen within a few steps."""
     rng = np.random.default_rng(seed)
-    n_s = {"monod_mass": 1, "dual_monod": 2, "monod": 1, "fixed_length": 1, "simple_acetate": 2}[model]
+    n_s = {"monod_mass": 1, "dual_monod": 2, "monod": 1, "fixed_length": 1, "simple_acetate": 2,
+           "quad_monod": 4}[model]
     if n_c > 1:
         edges = ring_with_chords(n_c, min(4, max(0, n_c * (n_c - 3) // 2)), rng) if n_c > 3 else \
             np.array([[i, (i + 1) % n_c] for i in range(n_c)])
@@ -156,6 +259,9 @@ def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, s
         p = np.array([0.3, 1e-3, 2.0, 2e-6, 0.6e-6, 3.0]) if fast else None
     elif model == "fixed_length":
         p = np.array([2e-7, 2e-6, 1e-3, 0.6e-6]) if fast else None
+    elif model == "quad_monod":
+        p = np.array([20.0, 0.01, 1e-4, 0.01, 5.0, 0.5, 0.3, 0.3, 0.4, 0.3, 0.05, 0.4, 2e-15, 1.4e-15,
+                      0.2e-15]) if fast else None
     else:
         p = None
     return Case(f"small_{model}", model, n_c, n_s, 0.01, seed, maps, steps_per_map=steps_per_map, feeds=fd,

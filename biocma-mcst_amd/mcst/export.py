@@ -33,6 +33,8 @@ EXPORTED = {
     "fixed_length": [(0, "length")],                                         # fixed_length.hpp:85-98
     "simple_acetate": [(i, n) for i, n in enumerate(                         # simple_acetate.hpp:115-121
         ["length", "l_max", "a_p", "a_max", "a_e", "a_e_s", "a_e_a", "phi_s", "phi_a"])],
+    "quad_monod": [(i, n) for i, n in enumerate(
+        ["m", "qs", "qa", "storage", "o2_rate", "overflow", "m_birth", "generation"])],
 }
 
 COUNTER_NAMES = ["steps", "births", "exits", "deaths", "moves", "overflow", "compactions", "n", "inactive",

@@ -119,6 +119,12 @@ int mcst_run_set_concentrations(mcst_engine* e, int32_t run, const double* C);
  * BIOMC_MC_MINIMUM_REMOVAL). */
 int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min_removal);
 /* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
+/* Gas-liquid mass transfer with a fixed gas phase (the liquid side of
+ * ScalarSimulation::performStepGL, apps/libs/simulation/src/implScalar.cpp:228-248,
+ * with mtr = kla*(Henry*C_gas - C)*V as in mass_transfer.cpp:140-157): the
+ * liquid mass of species s in compartment j gains dt*kla[s][j]*(c_star[s][j] - C)*V_j
+ * each step; c_star = Henry*C_gas is held constant (no gas transport). */
+int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, const double* c_star);
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
 /* Keep up to `cap` exit ages and division ages (the reference's probes). */
 int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap);

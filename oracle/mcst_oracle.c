@@ -160,9 +160,9 @@ static double trunc_normal(double mu, double sd, double lo, double hi, double u)
 /* Models.  State is an FP64 vector x[n_var]; c[] the local concentrations;
  * contrib[] the per-particle source term (kg/s) for each species. */
 
-static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9};
-static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2};
-static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0};
+static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9, 8};
+static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2, 4};
+static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0, 15};
 
 int orc_model_info(int model, int* n_var, int* n_species, int* n_params) {
   if (model < 0 || model >= ORC_MODEL_COUNT) return -1;
@@ -187,6 +187,10 @@ int orc_model_default_params(int model, double* p, int n) {
       break;
     case ORC_MODEL_FIXED_LENGTH: /* l_dot_max, l_max, k, d_m */
       p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6;
+      break;
+    case ORC_MODEL_QUAD_MONOD: /* qs_max, Ks, Ko, Ka, qa_max, Yxs, Yxa, Yos, Yoa, Yas, tau, Yco2, m_div, m0 mean, m0 sd */
+      p[0] = 2.0; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.01; p[4] = 0.5; p[5] = 0.5; p[6] = 0.3; p[7] = 0.3;
+      p[8] = 0.4; p[9] = 0.3; p[10] = 0.2; p[11] = 0.4; p[12] = 2e-15; p[13] = 1.4e-15; p[14] = 0.2e-15;
       break;
     default:
       break;
@@ -244,6 +248,16 @@ static void model_init(int model, const double* p, double* x, uint64_t seed, uin
       for (int i = 4; i < 9; ++i) x[i] = 0.0;
       break;
     }
+    case ORC_MODEL_QUAD_MONOD:
+      x[0] = trunc_normal(p[13], p[14], 0.5 * p[12], p[12], orc_u01_52(w[0], w[1]));
+      x[1] = 0.5 * p[0];
+      x[2] = 0.0;
+      x[3] = 0.0;
+      x[4] = 0.0;
+      x[5] = 0.0;
+      x[6] = x[0];
+      x[7] = 0.0;
+      break;
     default:
       break;
   }
@@ -253,6 +267,7 @@ static double model_mass(int model, const double* p, const double* x) {
   switch (model) {
     case ORC_MODEL_MONOD_MASS:
     case ORC_MODEL_DUAL_MONOD:
+    case ORC_MODEL_QUAD_MONOD:
       return x[0];
     case ORC_MODEL_MONOD:
       return x[0] * lin_density(p[4]);
@@ -339,6 +354,35 @@ static int model_update(int model, const double* p, double dt, double* x, const 
       contrib[1] = phi_a;
       return (x[0] >= x[1]) ? ORC_DIVISION : ORC_IDLE;
     }
+    case ORC_MODEL_QUAD_MONOD: {
+      /* glucose S, oxygen O, acetate A, CO2: saturations of the liquid */
+      const double S = c[0] > 0.0 ? c[0] : 0.0;
+      const double O = c[1] > 0.0 ? c[1] : 0.0;
+      const double A = c[2] > 0.0 ? c[2] : 0.0;
+      const double fs = S / (p[1] + S);
+      const double fo = O / (p[2] + O);
+      const double fa = A / (p[3] + A);
+      const double rep = p[1] / (p[1] + S);
+      const double qs_t = p[0] * fs;
+      const double qa_t = p[4] * fa * rep;
+      const double Us = x[1] * x[0];
+      const double Ua = x[2] * x[0] * fo;
+      const double Resp = Us * fo;
+      const double Ovf = Us - Resp;
+      const double G = p[5] * Resp + p[6] * Ua;
+      const double O2 = p[7] * Resp + p[8] * Ua;
+      contrib[0] = -Us;
+      contrib[1] = -O2;
+      contrib[2] = p[9] * Ovf - Ua;
+      contrib[3] = p[11] * (Resp + Ua);
+      x[0] = x[0] + dt * G;
+      x[3] = x[3] + dt * (Ovf * (1.0 - p[9]));
+      x[1] = x[1] + (dt / p[10]) * (qs_t - x[1]);
+      x[2] = x[2] + (dt / p[10]) * (qa_t - x[2]);
+      x[4] = O2;
+      x[5] = Ovf;
+      return (x[0] >= p[12]) ? ORC_DIVISION : ORC_IDLE;
+    }
     default:
       return ORC_IDLE;
   }
@@ -398,6 +442,14 @@ static void model_division(int model, const double* p, double* x, double* xd, ui
       x[1] = l1;
       xd[2] = g2;
       xd[1] = l2;
+      break;
+    }
+    case ORC_MODEL_QUAD_MONOD: {
+      x[0] = x[0] * 0.5;
+      x[3] = x[3] * 0.5;
+      x[6] = x[0];
+      x[7] = x[7] + 1.0;
+      for (int i = 0; i < 8; ++i) xd[i] = x[i];
       break;
     }
     default:
@@ -595,17 +647,21 @@ int orc_build_move_tables(uint32_t n, const double* V, uint64_t nnz, const uint3
   return 0;
 }
 
-/* m += dt*(C*M - C*diag(sink) + S); C = m*diag(1/V)   (implScalar.cpp:250-264) */
+/* m += dt*(C*M - C*diag(sink) + S); C = m*diag(1/V)   (implScalar.cpp:250-264).
+   With gas-liquid transfer (kla != NULL) the liquid also receives
+   mtr = kla*(C* - C)*V, C* the saturation set by a fixed gas phase
+   (performStepGL, implScalar.cpp:228-248; mass_transfer.cpp:140-157). */
 static void transport(const flowmap* f, uint32_t n_s, const double* sink, const double* S_total, double dt,
-                      const double* C_in, double* m, double* C_out) {
+                      const double* C_in, double* m, double* C_out, const double* kla, const double* cstar) {
   const uint32_t n = f->n;
   for (uint32_t s = 0; s < n_s; ++s) {
     const double* c = C_in + (size_t)s * n;
     for (uint32_t j = 0; j < n; ++j) {
       double acc = 0.0;
       for (uint64_t t = f->col_ptr[j]; t < f->col_ptr[j + 1]; ++t) acc += c[f->src[t]] * f->val[t];
-      const double dmdt = (acc - c[j] * sink[j]) + S_total[(size_t)s * n + j];
       const size_t o = (size_t)s * n + j;
+      double dmdt = (acc - c[j] * sink[j]) + S_total[o];
+      if (kla) dmdt = dmdt + (kla[o] * (cstar[o] - c[j])) * f->V[j];
       m[o] = m[o] + dt * dmdt;
       C_out[o] = m[o] * f->invV[j];
     }
@@ -618,7 +674,7 @@ int orc_transport_once(uint32_t n, uint32_t n_s, const double* V, uint64_t nnz, 
   flowmap f;
   const int rc = flowmap_build(&f, n, V, nnz, rows, cols, flows, dt);
   if (rc) return rc;
-  transport(&f, n_s, sink, S, dt, C_in, m_inout, C_out);
+  transport(&f, n_s, sink, S, dt, C_in, m_inout, C_out, NULL, NULL);
   flowmap_free(&f);
   return 0;
 }
@@ -660,6 +716,7 @@ struct orc_run {
   feed_t* feeds;
   uint32_t n_feeds;
   double *C, *C2, *m, *S, *S_total, *sink, *q_exit;
+  double *kla, *cstar; /* gas-liquid transfer [n_s][n_c], NULL without */
   uint64_t n, cap;
   uint32_t* pos;
   uint8_t* status;
@@ -707,6 +764,7 @@ void orc_destroy(orc_run* r) {
   free(r->maps);
   free(r->feeds);
   free(r->C); free(r->C2); free(r->m); free(r->S); free(r->S_total); free(r->sink); free(r->q_exit);
+  free(r->kla); free(r->cstar);
   free(r->pos); free(r->status); free(r->state); free(r->age_hyd); free(r->age_div);
   free(r->exit_ages.v);
   free(r->div_ages.v);
@@ -771,6 +829,17 @@ int orc_set_concentrations(orc_run* r, const double* C) {
 int orc_set_compaction(orc_run* r, double ratio, uint64_t min_removal) {
   r->ratio = ratio;
   r->min_removal = min_removal;
+  return 0;
+}
+
+int orc_set_gas_transfer(orc_run* r, const double* kla, const double* cstar) {
+  const size_t nsc = (size_t)r->n_s * r->n_c;
+  if (!kla || !cstar) return -1;
+  if (!r->kla) r->kla = (double*)malloc(sizeof(double) * nsc);
+  if (!r->cstar) r->cstar = (double*)malloc(sizeof(double) * nsc);
+  if (!r->kla || !r->cstar) return -2;
+  memcpy(r->kla, kla, sizeof(double) * nsc);
+  memcpy(r->cstar, cstar, sizeof(double) * nsc);
   return 0;
 }
 
@@ -1037,7 +1106,7 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
     for (size_t o = 0; o < nsc; ++o) r->S_total[o] = r->S[o] + r->S_total[o];
     /* ode_step then clear contributions */
     if (stages & ORC_STAGE_TRANSPORT) {
-      transport(f, (uint32_t)ns, r->sink, r->S_total, r->dt, r->C, r->m, r->C2);
+      transport(f, (uint32_t)ns, r->sink, r->S_total, r->dt, r->C, r->m, r->C2, r->kla, r->cstar);
       double* tmp = r->C;
       r->C = r->C2;
       r->C2 = tmp;

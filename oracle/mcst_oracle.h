@@ -24,6 +24,7 @@ enum orc_model {
   ORC_MODEL_MONOD = 2,
   ORC_MODEL_FIXED_LENGTH = 3,
   ORC_MODEL_SIMPLE_ACETATE = 4,
+  ORC_MODEL_QUAD_MONOD = 5,
   ORC_MODEL_COUNT
 };
 
@@ -99,6 +100,7 @@ int orc_add_feed(orc_run* r, int type, double f0, double param, uint32_t input, 
 int orc_set_concentrations(orc_run* r, const double* C);
 int orc_set_compaction(orc_run* r, double ratio, uint64_t min_removal);
 int orc_set_stage_cap(orc_run* r, uint32_t cap);
+int orc_set_gas_transfer(orc_run* r, const double* kla, const double* cstar);
 int orc_init_particles(orc_run* r, uint64_t n, uint64_t capacity, double x0);
 int orc_set_particles(orc_run* r, uint64_t n, uint64_t capacity, const uint32_t* pos,
                       const uint8_t* status, const double* state, const double* age_hyd,

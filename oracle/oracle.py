@@ -21,6 +21,7 @@ MODEL_IDS = {
     "monod": 2,
     "fixed_length": 3,
     "simple_acetate": 4,
+    "quad_monod": 5,
 }
 FEED_TYPES = {"constant": 0, "linear": 1, "exponential": 2}
 COUNTERS = ["steps", "births", "exits", "deaths", "moves", "overflow", "compactions", "n", "inactive", "processed"]
@@ -90,6 +91,7 @@ def lib():
         L.orc_get_counters.argtypes = [C.c_void_p, _u64p]
         L.orc_get_repartition.argtypes = [C.c_void_p, _u64p]
         L.orc_get_properties.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_set_gas_transfer.argtypes = [C.c_void_p, _f64p, _f64p]
         for name in ("orc_get_exit_ages", "orc_get_division_ages"):
             getattr(L, name).restype = C.c_uint64
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
@@ -203,6 +205,13 @@ class OracleRun:
 
     def set_stage_cap(self, cap):
         _check(lib().orc_set_stage_cap(self.h, cap), "set_stage_cap")
+
+    def set_gas_transfer(self, kla, cstar):
+        kla = np.ascontiguousarray(np.broadcast_to(np.asarray(kla, float).reshape(self.n_s, -1),
+                                                   (self.n_s, self.n_c)), np.float64)
+        cstar = np.ascontiguousarray(np.broadcast_to(np.asarray(cstar, float).reshape(self.n_s, -1),
+                                                     (self.n_s, self.n_c)), np.float64)
+        _check(lib().orc_set_gas_transfer(self.h, kla, cstar), "set_gas_transfer")
 
     def set_probes(self, cap):
         """The oracle always records exit and division ages."""

@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 
-@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate"])
+@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate",
+                                   "quad_monod"])
 def test_oracle_properties_match_independent_sums(oracle_lib, model):
     from mcst import cases
 
@@ -23,7 +24,7 @@ def test_oracle_properties_match_independent_sums(oracle_lib, model):
     np.testing.assert_array_equal(values[:nv], np.where(idle, p["state"], 0.0))
     # mass closed forms (Model::mass): monod_mass/dual_monod carry it in x0;
     # the length models convert length with the linear density of the cell
-    if model in ("monod_mass", "dual_monod"):
+    if model in ("monod_mass", "dual_monod", "quad_monod"):
         np.testing.assert_array_equal(values[nv], np.where(idle, p["state"][0], 0.0))
     else:
         idx = {"monod": 4, "fixed_length": 3}.get(model)  # d_m in the parameter vector

@@ -39,6 +39,55 @@ def test_monod_euler_closed_form(oracle_lib):
     np.testing.assert_allclose(r.contributions()[0, 0], -w * np.sum(mu * m_prev / Y), rtol=1e-12)
 
 
+QUAD = [2.0, 0.01, 1e-4, 0.01, 0.5, 0.5, 0.3, 0.3, 0.4, 0.3, 0.2, 0.4, 1e9, 1.4e-15, 0.2e-15]  # no division
+
+
+def _quad_box(oracle_lib, C, n=40, w=3.0):
+    r = one_box(oracle_lib, "quad_monod")
+    r.set_params(QUAD)
+    r.set_concentrations(np.array(C, float).reshape(4, 1))
+    m0 = np.linspace(1.0e-15, 1.8e-15, n)
+    x = np.zeros((8, n))
+    x[0], x[1], x[6] = m0, 0.5 * QUAD[0], m0
+    r.set_particles(2 * n, np.zeros(n, np.uint32), np.zeros(n, np.uint8), x, np.zeros(n), np.zeros(n), w)
+    return r, m0
+
+
+def test_quad_monod_without_oxygen_overflows_everything(oracle_lib):
+    """O = 0: nothing is respired, so no growth, no O2 or CO2 term, and the
+    acetate excreted is Yas times the glucose taken up; the rest is stored."""
+    r, m0 = _quad_box(oracle_lib, [5.0, 0.0, 0.0, 0.0])
+    r.step_stages(MODEL | CONTRIB)
+    S = r.contributions()[:, 0]
+    x = r.particles()["state"]
+    assert S[1] == 0.0 and S[3] == 0.0
+    np.testing.assert_allclose(S[2], -QUAD[9] * S[0], rtol=1e-14)
+    np.testing.assert_allclose(S[0], -3.0 * np.sum(0.5 * QUAD[0] * m0), rtol=1e-13)
+    np.testing.assert_array_equal(x[0], m0)
+    np.testing.assert_allclose(x[3], 0.01 * 0.5 * QUAD[0] * m0 * (1 - QUAD[9]), rtol=1e-13)
+    np.testing.assert_allclose(x[5], 0.5 * QUAD[0] * m0, rtol=1e-14)
+
+
+def test_quad_monod_uptake_lag_and_aerobic_growth(oracle_lib):
+    """No glucose: the lagged uptake rate decays geometrically by (1 - dt/tau)
+    per step; with oxygen and no acetate every cell grows by the same factor
+    1 + dt*Yxs*qs*fo, and O2/CO2 terms stand in the ratio Yos/Yco2."""
+    O = 2e-4
+    r, m0 = _quad_box(oracle_lib, [0.0, O, 0.0, 0.0])
+    fo = O / (QUAD[2] + O)
+    q = 0.5 * QUAD[0]
+    m = m0.copy()
+    for k in range(1, 6):
+        r.step_stages(MODEL | CONTRIB)
+        x = r.particles()["state"]
+        m = m * (1 + 0.01 * QUAD[5] * q * fo)
+        q = q * (1 - 0.01 / QUAD[10])
+        np.testing.assert_allclose(x[1], q, rtol=1e-13)
+        np.testing.assert_allclose(x[0], m, rtol=1e-12)
+    S = r.contributions()[:, 0]
+    np.testing.assert_allclose(S[1] / S[3], -QUAD[7] / QUAD[11], rtol=1e-13)
+
+
 def test_division_conserves_mass_and_respects_budget(oracle_lib):
     r = one_box(oracle_lib)
     r.set_params([0.5, 0.01, 0.5, 2.0, 1e-15, 0.1])
@@ -232,7 +281,8 @@ def test_initial_positions_uniform(oracle_lib):
     assert stats.chisquare(h).pvalue > 1e-3
 
 
-@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate"])
+@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate",
+                                   "quad_monod"])
 def test_threads_do_not_change_events(oracle_lib, model):
     from mcst import cases
 
@@ -253,3 +303,22 @@ def test_threads_do_not_change_events(oracle_lib, model):
     np.testing.assert_array_equal(runs[0].exit_ages(), runs[1].exit_ages())
     np.testing.assert_array_equal(runs[0].division_ages(), runs[1].division_ages())
     assert len(runs[0].division_ages()) > 0
+
+
+def test_gas_liquid_transfer_closed_form(oracle_lib):
+    """One well-mixed box, no particles, no flow: C_k = C* (1 - (1 - dt kla)^k)
+    for the transferred species; a species with kla = 0 stays put; mass
+    changes by exactly dt*kla*(C* - C)*V per step."""
+    r = one_box(oracle_lib, "dual_monod", V=2e-3)
+    r.set_concentrations(np.array([[1.5], [0.0]]))
+    kla, cs = 0.3, 8e-3
+    r.set_gas_transfer([0.0, kla], [0.0, cs])
+    r.init_particles(0, 0, 1.0)
+    for k in range(1, 31):
+        m_prev = r.mass()[1, 0]
+        c_prev = r.concentrations()[1, 0]
+        r.step_stages(TRANSPORT)
+        C = r.concentrations()
+        np.testing.assert_allclose(C[1, 0], cs * (1 - (1 - 0.01 * kla) ** k), rtol=1e-12)
+        np.testing.assert_allclose(r.mass()[1, 0] - m_prev, 0.01 * kla * (cs - c_prev) * 2e-3, rtol=1e-9)
+        assert C[0, 0] == 1.5
