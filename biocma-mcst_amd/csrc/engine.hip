@@ -43,7 +43,7 @@ int fail(int code, const std::string& msg) {
 // ------------------------------------------------------------ model registry
 struct ModelEntry {
   const char* name;
-  int id, nv, ns, np;
+  int id, nv, ns, np, nl;
   bool can_die;
   void (*defaults)(double*);
   void (*init)(const double*, double*, uint64_t, uint32_t);
@@ -57,7 +57,7 @@ double host_mass(const double* p, const double* x) {
 
 template <class M>
 ModelEntry entry() {
-  return ModelEntry{M::NAME, M::ID, M::NV, M::NS, M::NP, M::CAN_DIE, &M::defaults, &M::init, &host_mass<M>};
+  return ModelEntry{M::NAME, M::ID, M::NV, M::NS, M::NP, M::NL, M::CAN_DIE, &M::defaults, &M::init, &host_mass<M>};
 }
 
 const std::vector<ModelEntry>& registry() {
@@ -415,8 +415,14 @@ int choose_launch(mcst_engine* e) {
     }
   const size_t lds_cap = std::min<size_t>(std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin),
                                           96 * 1024);
-  bool priv = lds_need <= lds_cap;
+  // the accumulator holds the whole [n_s][n_c] of a run when it fits, else a
+  // window of compartments (particles outside it add to S directly)
+  const bool windowed = lds_need > lds_cap;
+  const size_t ns_b = sizeof(double) * std::max<uint32_t>(1, (uint32_t)M.ns);
+  lds_need = std::min(lds_need, (lds_cap / ns_b) * ns_b);
+  bool priv = lds_need > 0;
   if (const char* v = std::getenv("MCST_CONTRIB_LDS")) priv = priv && std::atoi(v) != 0;
+  b.acc_w = priv ? (uint32_t)(lds_need / ns_b) : 0u;
   int forced_block = e->tuned_block;
   if (const char* v = std::getenv("MCST_PARTICLE_BLOCK")) forced_block = std::atoi(v);
   ParticleLaunch best;
@@ -424,7 +430,8 @@ int choose_launch(mcst_engine* e) {
   for (int blk : {256, 512, 1024}) {
     if (!priv && blk != 256) continue;
     if (forced_block && blk != forced_block) continue;
-    const void* fn = particle_kernel_ptr(M.id, blk, priv, false);
+    const void* fn = particle_kernel_ptr(M.id, blk, priv, false, priv && windowed);
+    if (!fn) continue;
     const size_t lds = priv ? lds_need : 0;
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -441,35 +448,39 @@ int choose_launch(mcst_engine* e) {
       best_waves = waves;
       best.block = blk;
       best.priv = priv;
+      best.win = priv && windowed;
       best.lds = lds;
       best.grid = per_cu * prop.multiProcessorCount;
     }
   }
   if (best_waves < 0) {  // no resident configuration with the accumulator: go direct
     best = ParticleLaunch{};
+    b.acc_w = 0;
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false, false), 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false, false, false), 256, 0));
     best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
   }
   best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, b.T_max));
   best.probe = probe;
   if (best.probe && best.lds > 64 * 1024)
-    (void)hipFuncSetAttribute(particle_kernel_ptr(M.id, 256, best.priv, true),
+    (void)hipFuncSetAttribute(particle_kernel_ptr(M.id, 256, best.priv, true, best.win),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)best.lds);
   // one run: blocks take groups in turn, which keeps the wave front of the
   // whole grid inside a narrow window of the particle arrays; a batch: each
   // block takes one contiguous range, so it meets few runs
-  b.tile_group = e->tuned_group >= 0 ? (uint32_t)e->tuned_group : (live <= 1 ? 1u : 0u);
+  // one run whose accumulator fits: groups in turn; a batch, or a window of
+  // compartments: one contiguous range per block
+  b.tile_group = e->tuned_group >= 0 ? (uint32_t)e->tuned_group : (live <= 1 && !windowed ? 1u : 0u);
   if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
   e->plaunch = best;
   e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
   // one thread per compartment, plus the block that lays out the tiles
   e->grid_transport = (int)std::min<uint64_t>(((uint64_t)b.n_comp_total + 255) / 256, 8192) + 1;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
-                                                              ((uint64_t)b.T_max * NSUB + 63) / 64));
+                                                              ((uint64_t)b.T_max * b.nsub + 63) / 64));
   if (std::getenv("MCST_VERBOSE"))
-    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u (lds cap %zu)\n",
-                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, lds_cap);
+    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u (lds cap %zu)\n",
+                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, lds_cap);
   return MCST_OK;
 }
 
@@ -683,7 +694,7 @@ int commit(mcst_engine* e) {
   b.ST = T_max * stride;
   b.R = 0;
   b.n_comp_total = 0;
-  b.nsub = NSUB;
+  b.nsub = TILE / (64u * (uint32_t)M.nl);
   b.ns_max = std::max<uint32_t>(1, (uint32_t)M.ns);  // one model per engine; thread s == 0 also writes the map fields
   b.stages = STAGE_ALL;
   e->runs_cap = std::max(R0, e->reserve_runs);
@@ -706,15 +717,15 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * NSUB * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * NSUB * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * b.nsub * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * b.nsub * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_runs, (size_t)RC * sizeof(RunDev)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_rs, (size_t)RC * sizeof(RunState)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_comp, (size_t)(RC + 1) * sizeof(uint32_t)))) return rc;
   HIP_TRY(hipMemset(b.tile_births, 0, T_max * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.tile_off, 0, (RC + 1) * sizeof(uint32_t)));
-  HIP_TRY(hipMemset(b.sub_moves, 0, T_max * NSUB * sizeof(uint32_t)));
-  HIP_TRY(hipMemset(b.sub_divs, 0, T_max * NSUB * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.sub_moves, 0, T_max * b.nsub * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.sub_divs, 0, T_max * b.nsub * sizeof(uint32_t)));
   HIP_TRY(hipMemset(e->d_rs, 0, (size_t)RC * sizeof(RunState)));
   b.runs = e->d_runs;
   b.rs = e->d_rs;

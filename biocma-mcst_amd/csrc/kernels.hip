@@ -236,6 +236,29 @@ __device__ __forceinline__ void flush_counts(RunState* rs, TileCounts& c) {
   c = TileCounts{};
 }
 
+// Adds a block's LDS contribution accumulator, [n_s][win] for compartments
+// [w0, w0 + win) of a run with n_c compartments, to S: one atomic per
+// nonzero entry.
+template <int BLK>
+__device__ __forceinline__ void flush_window(const double* s_acc, GLOBAL_AS double* S, uint32_t ns, uint32_t win,
+                                             uint32_t n_c, uint32_t w0, uint32_t tid) {
+  const uint32_t len = ns * win;
+  if (win == n_c) {  // the whole run: the layouts coincide
+    for (uint32_t i = tid; i < len; i += BLK) {
+      const double v = s_acc[i];
+      if (v != 0.0) unsafeAtomicAdd((double*)(S + i), v);
+    }
+    return;
+  }
+  for (uint32_t i = tid; i < len; i += BLK) {
+    const double v = s_acc[i];
+    if (v != 0.0) {
+      const uint32_t q = i / win;
+      unsafeAtomicAdd((double*)(S + (size_t)q * n_c + w0 + (i - q * win)), v);
+    }
+  }
+}
+
 // First column of a cumulative row with cp[c] >= u, clamped to m-1: for a
 // non-decreasing row this is #{c < m-1 : cp[c] < u}, the result of the
 // reference's bisection (move_kernel.hpp:89-102).  The count runs over chunks
@@ -344,7 +367,7 @@ __device__ __forceinline__ void st_status(GLOBAL_AS uint8_t* p, uint32_t v) {
 // Every load is issued for every lane (slots past the count read valid
 // memory of the tile and are masked where used), so the waits the compiler
 // places for a load's operands do not depend on the data.
-template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE>
+template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE, bool WIN>
 __global__ void __launch_bounds__(BLK)
     k_particle(BatchDev b, const RunDev* __restrict__ runs, const RunState* __restrict__ rsr,
                const uint32_t* __restrict__ tile_off) {
@@ -354,8 +377,9 @@ __global__ void __launch_bounds__(BLK)
   constexpr uint32_t WPB = BLK / 64;     // waves per block = sub-chunks per group
   constexpr uint32_t SUB = 64 * NL;       // slots per sub-chunk
   static_assert(NL == 1 || NL == 2 || NL == 4, "1, 2 or 4 slots per lane");
-  static_assert(TILE / SUB == NSUB, "the lists of the pass are laid out for NSUB sub-chunks per tile");
-  extern __shared__ double s_acc[];      // [n_s][n_c] of run acc_run (PRIV)
+  constexpr uint32_t NSUB = TILE / SUB;  // sub-chunks per tile (b.nsub is set to the same)
+  extern __shared__ double s_acc[];      // [n_s][acc_win] of run acc_run (PRIV)
+  __shared__ uint32_t s_w0;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = uniform(tid >> 6);  // wave-uniform: scalar bookkeeping
   const uint32_t stages = b.stages;
   const uint64_t P = b.P;
@@ -442,9 +466,14 @@ __global__ void __launch_bounds__(BLK)
   // per-wave counters of the run cnt_run, flushed when the wave changes runs
   TileCounts cnt_acc;
   uint32_t cnt_run = 0xFFFFFFFFu;
+  // the LDS accumulator holds compartments [acc_w0, acc_w0 + acc_win) of run
+  // acc_run: the whole run when it fits (acc_win = n_c), else a window
+  // centred on the block's first slot of the run (particles are stored in
+  // compartment order, so a block's slots see few compartments); particles
+  // outside the window add to S directly
   uint32_t acc_run = 0xFFFFFFFFu;
   GLOBAL_AS double* acc_S = nullptr;
-  uint32_t acc_len = 0;
+  uint32_t acc_ns = 0, acc_nc = 0, acc_win = 0, acc_w0 = 0;
 
   SubRef cur = sub_ref(grp * WPB + wave);
   Sub A;
@@ -470,15 +499,25 @@ __global__ void __launch_bounds__(BLK)
       const uint32_t r0 = find_run(tile_off, R, (grp * WPB) / NSUB);
       if (r0 != acc_run) {
         __syncthreads();
-        for (uint32_t i = tid; i < acc_len; i += BLK) {
-          const double v = s_acc[i];
-          if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
-        }
+        flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
         acc_run = r0;
         acc_S = gp(runs[r0].S);
-        acc_len = runs[r0].n_s * runs[r0].n_c;
-        __syncthreads();
-        for (uint32_t i = tid; i < acc_len; i += BLK) s_acc[i] = 0.0;
+        acc_ns = runs[r0].n_s;
+        acc_nc = runs[r0].n_c;
+        acc_win = acc_nc;
+        if constexpr (WIN) {
+          acc_win = b.acc_w < acc_nc ? b.acc_w : acc_nc;
+          if (tid == 0) {
+            // wave 0 holds the group's first sub-chunk; its lane 0 the first slot
+            const uint32_t c0 = cur.cnt ? A.pos[0] : 0u;
+            const uint32_t half = acc_win / 2;
+            const uint32_t lo = c0 > half ? c0 - half : 0u;
+            s_w0 = lo + acc_win <= acc_nc ? lo : acc_nc - acc_win;
+          }
+          __syncthreads();
+          acc_w0 = s_w0;
+        }
+        for (uint32_t i = tid; i < acc_ns * acc_win; i += BLK) s_acc[i] = 0.0;
         __syncthreads();
       }
     }
@@ -571,10 +610,18 @@ __global__ void __launch_bounds__(BLK)
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
           const double c = w * contrib[j][q];
-          if (to_lds)
-            atomicAdd(s_acc + q * n_c + A.pos[j], c);
-          else
-            unsafeAtomicAdd((double*)(S + (size_t)q * n_c + A.pos[j]), c);
+          if constexpr (WIN) {
+            const uint32_t k = A.pos[j] - acc_w0;  // wraps above the window when below it
+            if (to_lds && k < acc_win)
+              atomicAdd(s_acc + q * acc_win + k, c);
+            else
+              unsafeAtomicAdd((double*)(S + (size_t)q * n_c + A.pos[j]), c);
+          } else {
+            if (to_lds)
+              atomicAdd(s_acc + q * n_c + A.pos[j], c);
+            else
+              unsafeAtomicAdd((double*)(S + (size_t)q * n_c + A.pos[j]), c);
+          }
         }
       }
     }
@@ -695,10 +742,7 @@ __global__ void __launch_bounds__(BLK)
   }
   if (PRIV) {
     __syncthreads();
-    for (uint32_t i = tid; i < acc_len; i += BLK) {
-      const double v = s_acc[i];
-      if (v != 0.0) unsafeAtomicAdd((double*)(acc_S + i), v);
-    }
+    flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
   }
   if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
 }
@@ -1164,21 +1208,32 @@ __global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint
 }
 
 // ------------------------------------------------------------------ launchers
-template <class M, int BLK, bool PRIV, bool PROBE>
+template <class M, int BLK, bool PRIV, bool PROBE, bool WIN>
 static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_particle<M, BLK, LANE_SLOTS, LANE_INTERLEAVE, PRIV, PROBE>), dim3(grid), dim3(BLK), lds, s, b, b.runs,
-                     (const RunState*)b.rs, (const uint32_t*)b.tile_off);
+  hipLaunchKernelGGL((k_particle<M, BLK, M::NL, LANE_INTERLEAVE, PRIV, PROBE, WIN>), dim3(grid), dim3(BLK), lds, s, b,
+                     b.runs, (const RunState*)b.rs, (const uint32_t*)b.tile_off);
 }
 
+// The variants: direct global atomics (256 threads); an LDS accumulator of
+// whole runs (256/512/1024) or of a compartment window (256/512); and the
+// 256-thread probe variants of each accumulator kind.
 template <class M>
-static const void* kernel_ptr_t(int blk, bool priv, bool probe) {
-  if (probe) return priv ? (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, true, true> : (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, false, true>;
-  if (priv) {
-    if (blk == 256) return (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, true, false>;
-    if (blk == 512) return (const void*)&k_particle<M, 512, LANE_SLOTS, LANE_INTERLEAVE, true, false>;
-    return (const void*)&k_particle<M, 1024, LANE_SLOTS, LANE_INTERLEAVE, true, false>;
+static const void* kernel_ptr_t(int blk, bool priv, bool probe, bool win) {
+  if (probe) {
+    if (!priv) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, true, false>;
+    return win ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, true>
+               : (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, false>;
   }
-  return (const void*)&k_particle<M, 256, LANE_SLOTS, LANE_INTERLEAVE, false, false>;
+  if (!priv) return blk == 256 ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, false, false> : nullptr;
+  if (win) {
+    if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, true>;
+    if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, true>;
+    return nullptr;
+  }
+  if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, false>;
+  if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, false>;
+  if (blk == 1024) return (const void*)&k_particle<M, 1024, M::NL, LANE_INTERLEAVE, true, false, false>;
+  return nullptr;
 }
 
 // Runs with probes take a 256-thread variant that records exit and division
@@ -1188,18 +1243,25 @@ static const void* kernel_ptr_t(int blk, bool priv, bool probe) {
 template <class M>
 static void launch_t(const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
   if (L.probe) {
-    if (L.priv)
-      launch_particle_t<M, 256, true, true>(b, L.grid, L.lds, s);
+    if (!L.priv)
+      launch_particle_t<M, 256, false, true, false>(b, L.grid, 0, s);
+    else if (L.win)
+      launch_particle_t<M, 256, true, true, true>(b, L.grid, L.lds, s);
     else
-      launch_particle_t<M, 256, false, true>(b, L.grid, 0, s);
+      launch_particle_t<M, 256, true, true, false>(b, L.grid, L.lds, s);
   } else if (!L.priv) {
-    launch_particle_t<M, 256, false, false>(b, L.grid, 0, s);
+    launch_particle_t<M, 256, false, false, false>(b, L.grid, 0, s);
+  } else if (L.win) {
+    if (L.block == 512)
+      launch_particle_t<M, 512, true, false, true>(b, L.grid, L.lds, s);
+    else
+      launch_particle_t<M, 256, true, false, true>(b, L.grid, L.lds, s);
   } else if (L.block == 256) {
-    launch_particle_t<M, 256, true, false>(b, L.grid, L.lds, s);
+    launch_particle_t<M, 256, true, false, false>(b, L.grid, L.lds, s);
   } else if (L.block == 512) {
-    launch_particle_t<M, 512, true, false>(b, L.grid, L.lds, s);
+    launch_particle_t<M, 512, true, false, false>(b, L.grid, L.lds, s);
   } else {
-    launch_particle_t<M, 1024, true, false>(b, L.grid, L.lds, s);
+    launch_particle_t<M, 1024, true, false, false>(b, L.grid, L.lds, s);
   }
 }
 
@@ -1215,14 +1277,14 @@ void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, h
   }
 }
 
-const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe) {
+const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe, bool win) {
   switch (model_id) {
-    case MonodMass::ID: return kernel_ptr_t<MonodMass>(block, priv, probe);
-    case DualMonod::ID: return kernel_ptr_t<DualMonod>(block, priv, probe);
-    case Monod::ID: return kernel_ptr_t<Monod>(block, priv, probe);
-    case FixedLength::ID: return kernel_ptr_t<FixedLength>(block, priv, probe);
-    case SimpleAcetate::ID: return kernel_ptr_t<SimpleAcetate>(block, priv, probe);
-    case QuadMonod::ID: return kernel_ptr_t<QuadMonod>(block, priv, probe);
+    case MonodMass::ID: return kernel_ptr_t<MonodMass>(block, priv, probe, win);
+    case DualMonod::ID: return kernel_ptr_t<DualMonod>(block, priv, probe, win);
+    case Monod::ID: return kernel_ptr_t<Monod>(block, priv, probe, win);
+    case FixedLength::ID: return kernel_ptr_t<FixedLength>(block, priv, probe, win);
+    case SimpleAcetate::ID: return kernel_ptr_t<SimpleAcetate>(block, priv, probe, win);
+    case QuadMonod::ID: return kernel_ptr_t<QuadMonod>(block, priv, probe, win);
     default: return nullptr;
   }
 }

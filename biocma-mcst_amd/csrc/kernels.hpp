@@ -17,11 +17,12 @@ struct ParticleLaunch {
   int grid = 1;
   bool priv = false;
   bool probe = false;
+  bool win = false;  // the accumulator holds a window of compartments, not whole runs
   size_t lds = 0;
 };
 
 void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
-const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe);
+const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe, bool win);
 void launch_bookkeep(const BatchDev& b, hipStream_t s);
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_resolve(int model_id, const BatchDev& b, int grid, hipStream_t s);

@@ -16,9 +16,8 @@ namespace mcst {
 constexpr uint32_t BLOCK = 256;            // threads per particle block (4 wave64)
 constexpr uint32_t PPT = 4;                // particles per thread per tile
 constexpr uint32_t TILE = BLOCK * PPT;     // 1024 slots per tile (the reference's 1024 particles/team)
-constexpr uint32_t LANE_SLOTS = 4;         // particle pass: slots per lane in a sub-chunk
-constexpr bool LANE_INTERLEAVE = true;     // ... 64 slots apart (else consecutive)
-constexpr uint32_t NSUB = TILE / (64 * LANE_SLOTS);  // sub-chunks (the pass's unit of work) per tile
+constexpr bool LANE_INTERLEAVE = true;     // particle pass: a lane's slots lie 64 apart (else consecutive)
+constexpr uint32_t NSUB_MAX = TILE / 128;  // sub-chunks (the pass's unit of work) per tile: TILE / (64 * M::NL)
 constexpr uint32_t WAVES = BLOCK / 64;
 constexpr int MAX_PARAMS = 16;
 constexpr int MAX_VAR = 16;
@@ -148,13 +147,13 @@ struct BatchDev {
   uint32_t* tile_fill_off;
   uint32_t* fill_src;       // [P] scratch list of filler slots during removal
   uint32_t* tile_off;       // [R+1] first global tile of each run in the current layout
-  // lists of the particle pass, one segment of TILE/NSUB entries per sub-chunk
+  // lists of the particle pass, one segment of TILE/nsub entries per sub-chunk
   uint16_t* move_slot;      // [T_max*TILE] tile-local slots of the particles leaving their compartment
   uint32_t* move_pos;       // [T_max*TILE] their compartment before the move
   uint16_t* div_slot;       // [T_max*TILE] tile-local slots of the division requests
   uint32_t* div_pos;        // [T_max*TILE] their compartment (the newborn's)
-  uint32_t* sub_moves;      // [T_max*NSUB] entries of each sub-chunk's mover segment
-  uint32_t* sub_divs;       // [T_max*NSUB] entries of each sub-chunk's request segment
+  uint32_t* sub_moves;      // [T_max*nsub] entries of each sub-chunk's mover segment
+  uint32_t* sub_divs;       // [T_max*nsub] entries of each sub-chunk's request segment
   // runs
   const RunDev* runs;
   RunState* rs;
@@ -164,8 +163,9 @@ struct BatchDev {
   uint32_t n_comp_total;
   uint32_t ns_max;          // largest species count of the batch
   uint32_t T_max;
-  uint32_t nsub;            // NSUB
+  uint32_t nsub;            // sub-chunks per tile of the model's particle pass, TILE / (64 * M::NL)
   uint32_t tile_group;      // particle pass: consecutive tiles per block turn (0: one range per block)
+  uint32_t acc_w;           // particle pass: compartments per LDS contribution window (PRIV)
   uint32_t stages;
 };
 

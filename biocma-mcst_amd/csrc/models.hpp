@@ -9,6 +9,8 @@
 //   division(p, x, xd, key)                device+host              (Model::division)
 //   mass(p, x)                             device+host              (Model::mass)
 //   CAN_DIE                                whether update() may return DEAD
+//   NL                                     particle pass: slots per lane of a sub-chunk (4, or 2
+//                                          for large states, which halves the registers)
 // State is FP64; contributions (kg/s per particle) are returned in
 // registers instead of a per-particle contribs row, because the step kernel
 // scatters them immediately.  record() maps a compartment's concentrations
@@ -139,6 +141,7 @@ constexpr int rec_stride(int ns) { return (ns + 2) <= 4 ? 4 : ((ns + 2) <= 8 ? 8
 struct MonodMass {
   static constexpr int ID = 0, NV = 2, NS = 1, NP = 6;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr int NL = 4;
   static constexpr const char* NAME = "monod_mass";
   static void defaults(double* p) {
     p[0] = 0.5 / 3600.0; p[1] = 0.01; p[2] = 0.5; p[3] = 2.0; p[4] = 1e-15; p[5] = 0.1;
@@ -177,6 +180,7 @@ struct MonodMass {
 struct DualMonod {
   static constexpr int ID = 1, NV = 3, NS = 2, NP = 10;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr int NL = 4;
   static constexpr const char* NAME = "dual_monod";
   static void defaults(double* p) {
     p[0] = 2.8e-4; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.5; p[4] = 1.0; p[5] = 100.0; p[6] = 1e-4;
@@ -227,6 +231,7 @@ struct DualMonod {
 struct Monod {
   static constexpr int ID = 2, NV = 6, NS = 1, NP = 6;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr int NL = 4;
   static constexpr const char* NAME = "monod";
   static void defaults(double* p) {
     p[0] = 0.77 / 3600.0; p[1] = 1e-3; p[2] = 2.0; p[3] = 2e-6; p[4] = 0.6e-6; p[5] = 3600.0 / 0.77;
@@ -274,6 +279,7 @@ struct Monod {
 struct FixedLength {
   static constexpr int ID = 3, NV = 2, NS = 1, NP = 4;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr int NL = 4;
   static constexpr const char* NAME = "fixed_length";
   static void defaults(double* p) {
     p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6;
@@ -315,6 +321,7 @@ struct FixedLength {
 struct SimpleAcetate {
   static constexpr int ID = 4, NV = 9, NS = 2, NP = 0;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr int NL = 4;
   static constexpr const char* NAME = "simple_acetate";
   static constexpr double A_MAX = 2e-6 / 3600.0;
   static constexpr double L_MAX = 2e-6;
@@ -392,6 +399,7 @@ struct SimpleAcetate {
 struct QuadMonod {
   static constexpr int ID = 5, NV = 8, NS = 4, NP = 15;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
+  static constexpr int NL = 2;
   static constexpr const char* NAME = "quad_monod";
   static void defaults(double* p) {
     p[0] = 2.0; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.01; p[4] = 0.5; p[5] = 0.5; p[6] = 0.3; p[7] = 0.3;
