@@ -148,6 +148,46 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
             "autotuned": {"block": tuned["block"], "tile_group": tuned["group"], "trial_steps": done}}
 
 
+def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5):
+    """The 50 000-compartment lattice tank at full size: one run of
+    quad_monod (8 FP64 states, division age, hydraulic age), 4 species,
+    outlet washout with device compaction, 200 steps of 5 ms, all timed."""
+    import mcst
+    from mcst import cases
+
+    log(f"building the 50x50x20 lattice case with {n_particles:.0e} particles")
+    case = cases.config4(n_particles=n_particles)
+    eng = mcst.Engine(case.model)
+    run = eng.add_run(case.n_c, case.dt, case.seed)
+    t0 = time.perf_counter()
+    case.apply(run)
+    eng.commit()
+    eng.synchronize()
+    log(f"initialised and uploaded in {time.perf_counter() - t0:.1f} s")
+    c0 = run.counters()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.synchronize()
+    el = time.perf_counter() - t0
+    c1 = run.counters()
+    ps = c1["processed"] - c0["processed"]
+    p0 = c1["processed"]
+    _, per = eng.profile(profile_steps)
+    per_launch = (run.counters()["processed"] - p0) / profile_steps
+    _, nv, ns, _ = mcst.model_info(case.model)
+    b_ps = 2 * (8 * (nv + 2) + 4 + 1)  # states, division age, hydraulic age (the top layer is an outlet)
+    achieved = per_launch * b_ps / (per[1] * 1e-3) / 1e9
+    eng.close()
+    log(f"config4: {ps / el:.3e} particle-steps/s, {el * 1e3 / steps:.2f} ms/step, particle pass {achieved:.0f} GB/s")
+    return {"workload": "50x50x20 lattice tank, 50 000 compartments, 4 species, quad_monod (8 states), "
+                        "outlet washout, dt=0.005 s", "particles_initial": int(c0["n"]), "particles_final": int(c1["n"]),
+            "steps": steps, "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / steps,
+            "births": int(c1["births"]), "exits": int(c1["exits"]), "compactions": int(c1["compactions"]),
+            "kernel_ms_per_step": dict(zip(["transport_layout", "particle", "resolve", "bookkeep", "removal", "merge"],
+                                           map(float, per))),
+            "particle_pass": {"B_ps": b_ps, "achieved_GBps": achieved, "frac_of_8TBps": achieved / HBM_PEAK_GBS}}
+
+
 def copy_calibration(mib, reps=5):
     """Plain device-to-device copies of a known size: what FETCH_SIZE and
     WRITE_SIZE read for them calibrates the particle kernel's counters."""
@@ -178,6 +218,8 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--served-requests", type=int, default=40)
     ap.add_argument("--ensemble", action="store_true", help="also measure the 512-run batched ensemble")
+    ap.add_argument("--config4", type=float, default=0, metavar="N",
+                    help="also run the 50 000-compartment lattice case with N particles (e.g. 1e8)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this many MiB (PMC counter calibration)")
     a = ap.parse_args()
@@ -269,6 +311,7 @@ def main():
     if a.ensemble:
         log("batched ensemble of 512 runs")
         ensemble = ensemble_measurement()
+    lattice = config4_measurement(int(a.config4)) if a.config4 else None
     nnz = case.nnz + case.n_c
     out = {
         "metric": METRIC,
@@ -302,6 +345,8 @@ def main():
     }
     if ensemble is not None:
         out["ensemble"] = ensemble
+    if lattice is not None:
+        out["config4"] = lattice
     eng.close()
     print(json.dumps(out), flush=True)
 
