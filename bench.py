@@ -148,7 +148,7 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
             "autotuned": {"block": tuned["block"], "tile_group": tuned["group"], "trial_steps": done}}
 
 
-def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5):
+def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sort_interval=20):
     """The 50 000-compartment lattice tank at full size: one run of
     quad_monod (8 FP64 states, division age, hydraulic age), 4 species,
     outlet washout with device compaction, 200 steps of 5 ms, all timed."""
@@ -161,6 +161,7 @@ def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5):
     run = eng.add_run(case.n_c, case.dt, case.seed)
     t0 = time.perf_counter()
     case.apply(run)
+    run.set_sort_interval(sort_interval)
     eng.commit()
     eng.synchronize()
     log(f"initialised and uploaded in {time.perf_counter() - t0:.1f} s")
@@ -182,6 +183,7 @@ def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5):
     return {"workload": "50x50x20 lattice tank, 50 000 compartments, 4 species, quad_monod (8 states), "
                         "outlet washout, dt=0.005 s", "particles_initial": int(c0["n"]), "particles_final": int(c1["n"]),
             "steps": steps, "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / steps,
+            "sort_interval": sort_interval,
             "births": int(c1["births"]), "exits": int(c1["exits"]), "compactions": int(c1["compactions"]),
             "kernel_ms_per_step": dict(zip(["transport_layout", "particle", "resolve", "bookkeep", "removal", "merge"],
                                            map(float, per))),
@@ -220,6 +222,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true", help="also measure the 512-run batched ensemble")
     ap.add_argument("--config4", type=float, default=0, metavar="N",
                     help="also run the 50 000-compartment lattice case with N particles (e.g. 1e8)")
+    ap.add_argument("--config4-sort", type=int, default=20, metavar="K",
+                    help="reorder the lattice case's particles by compartment every K steps (0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this many MiB (PMC counter calibration)")
     a = ap.parse_args()
@@ -311,7 +315,7 @@ def main():
     if a.ensemble:
         log("batched ensemble of 512 runs")
         ensemble = ensemble_measurement()
-    lattice = config4_measurement(int(a.config4)) if a.config4 else None
+    lattice = config4_measurement(int(a.config4), sort_interval=a.config4_sort) if a.config4 else None
     nnz = case.nnz + case.n_c
     out = {
         "metric": METRIC,

@@ -204,6 +204,8 @@ struct HostRun {
   uint32_t stage_cap = 64;
   uint32_t probe_cap = 0;
   uint32_t step_target = 0xFFFFFFFFu;
+  uint32_t sort_interval = 0;    // reorder the particles by compartment every k steps (0: never)
+  uint32_t host_step = 0;        // the run's step count as the device will have it
   // initial particles
   bool has_particles = false;
   uint64_t n = 0, cap = 0;
@@ -255,6 +257,9 @@ struct mcst_engine {
   int grid_particle = 1, grid_transport = 1;
   mcst::ParticleLaunch plaunch;
   int tuned_block = 0;              // particle-pass shape chosen by mcst_engine_autotune (0: none)
+  void* sort_mem = nullptr;         // scratch of the particle reorder, sized for sort_n slots
+  uint64_t sort_n = 0;
+  size_t sort_temp = 0;
   int tuned_group = -1;
   int grid_move = 1;
   bool removal_possible = true;
@@ -314,6 +319,10 @@ void drop_graph(mcst_engine* e) {
 
 void release(mcst_engine* e) {
   drop_graph(e);
+  if (e->sort_mem) (void)hipFree(e->sort_mem);
+  e->sort_mem = nullptr;
+  e->sort_n = 0;
+  e->sort_temp = 0;
   for (auto& h : e->runs) free_run_memory(h);
   for (void* p : e->allocs) (void)hipFree(p);
   e->allocs.clear();
@@ -606,6 +615,7 @@ int upload_run(mcst_engine* e, uint32_t r) {
   uint64_t inactive = 0;
   for (uint64_t q = 0; q < h.n; ++q) inactive += h.status[q] != IDLE;
   st.inactive = (uint32_t)inactive;
+  h.host_step = h.restored ? h.r_step : 0u;
   if (h.restored) {
     std::memcpy(st.counters, h.r_counters, sizeof(st.counters));
     st.step = h.r_step;
@@ -783,6 +793,78 @@ void launch_step(mcst_engine* e, uint32_t stages) {
 int read_state(mcst_engine* e, int32_t run, RunState& s) {
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(&s, e->d_rs + run, sizeof(RunState), hipMemcpyDeviceToHost));
+  return MCST_OK;
+}
+
+// Reorders run r's particles by compartment, stable in slot order
+// (sort.hip).  Reads the particle count, so it waits for the stream.
+int sort_run(mcst_engine* e, uint32_t r) {
+  RunState st;
+  int rc = read_state(e, (int32_t)r, st);
+  if (rc) return rc;
+  const HostRun& h = e->runs[r];
+  const uint64_t n = st.n;
+  if (n < 2 || h.n_c < 2) return MCST_OK;
+  uint32_t end_bit = 1;
+  while (end_bit < 32 && ((uint64_t)1 << end_bit) < h.n_c) ++end_bit;
+  if (n > e->sort_n) {
+    size_t temp = 0;
+    HIP_TRY(sort_temp_bytes(n, end_bit, &temp));
+    temp = std::max(temp, (size_t)(1u << 20)) * 2;  // headroom for wider keys of other runs
+    if (e->sort_mem) HIP_TRY(hipFree(e->sort_mem));
+    e->sort_mem = nullptr;
+    e->sort_n = 0;
+    const uint64_t cap = n + n / 4;
+    HIP_TRY(hipMalloc(&e->sort_mem, cap * (3 * sizeof(uint32_t) + sizeof(double)) + temp + 256));
+    e->sort_n = cap;
+    e->sort_temp = temp;
+  }
+  size_t need = 0;
+  HIP_TRY(sort_temp_bytes(n, end_bit, &need));
+  if (need > e->sort_temp) return fail(MCST_ERR_MEMORY, "reorder scratch too small");
+  const BatchDev& b = e->bd;
+  const uint64_t o = h.p_off, cap = e->sort_n;
+  char* m = (char*)e->sort_mem;
+  SortArgs a;
+  a.n = n;
+  a.end_bit = end_bit;
+  a.pos = b.pos + o;
+  a.status = b.status + o;
+  const uint32_t nv = (uint32_t)e->model->nv;
+  for (uint32_t v = 0; v < nv; ++v) a.f64[v] = b.state + (uint64_t)v * b.P + o;
+  a.f64[nv] = b.age_hyd + o;
+  a.f64[nv + 1] = b.age_div + o;
+  a.n_f64 = nv + 2;
+  a.keys = (uint32_t*)m;
+  a.iota = a.keys + cap;
+  a.perm = a.iota + cap;
+  a.buf = (void*)(((uintptr_t)(a.perm + cap) + 255) & ~(uintptr_t)255);
+  a.temp = (char*)a.buf + cap * sizeof(double);
+  a.temp_bytes = e->sort_temp;
+  HIP_TRY(sort_run_particles(a, e->stream));
+  return MCST_OK;
+}
+
+// The host's mirror of the device step counters: k steps of the batch
+// advance every live run that has not reached its step target.
+void count_steps(mcst_engine* e, uint64_t k) {
+  for (auto& h : e->runs)
+    if (h.phase == 1) h.host_step = (uint32_t)std::min<uint64_t>(h.step_target, h.host_step + k);
+}
+
+// One step of the batch, then the reorders that fall due.
+int step_once(mcst_engine* e, uint32_t stages) {
+  std::vector<uint32_t> before(e->runs.size());
+  for (size_t r = 0; r < e->runs.size(); ++r) before[r] = e->runs[r].host_step;
+  launch_step(e, stages);
+  count_steps(e, 1);
+  for (size_t r = 0; r < e->runs.size(); ++r) {
+    const HostRun& h = e->runs[r];
+    if (h.phase == 1 && h.sort_interval && h.host_step != before[r] && h.host_step % h.sort_interval == 0) {
+      const int rc = sort_run(e, (uint32_t)r);
+      if (rc) return rc;
+    }
+  }
   return MCST_OK;
 }
 
@@ -965,6 +1047,20 @@ int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, co
   return MCST_OK;
 }
 
+int mcst_run_set_sort_interval(mcst_engine* e, int32_t run, uint32_t steps) {
+  CHECK_ENGINE(e);
+  HostRun* h = get_run(e, run);
+  if (!h) return fail(MCST_ERR_ARG, "bad run id");
+  h->sort_interval = steps;
+  return MCST_OK;
+}
+
+int mcst_run_sort_particles(mcst_engine* e, int32_t run) {
+  CHECK_READY(e, run);
+  HIP_TRY(hipSetDevice(e->device));
+  return sort_run(e, (uint32_t)run);
+}
+
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap) {
   CHECK_DEFINE(e, run, h);
   if (cap == 0 || cap > TILE) return fail(MCST_ERR_ARG, "division cap must be in 1..1024");
@@ -1120,7 +1216,10 @@ int mcst_engine_step(mcst_engine* e, uint32_t n_steps) {
   CHECK_ENGINE(e);
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
   HIP_TRY(hipSetDevice(e->device));
-  for (uint32_t k = 0; k < n_steps; ++k) launch_step(e, STAGE_ALL);
+  for (uint32_t k = 0; k < n_steps; ++k) {
+    const int rc = step_once(e, STAGE_ALL);
+    if (rc) return rc;
+  }
   HIP_TRY(hipGetLastError());
   return MCST_OK;
 }
@@ -1130,7 +1229,8 @@ int mcst_engine_step_stages(mcst_engine* e, uint32_t stages) {
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
   if (stages & ~STAGE_ALL) return fail(MCST_ERR_ARG, "unknown stage bits");
   HIP_TRY(hipSetDevice(e->device));
-  launch_step(e, stages);
+  const int rc = step_once(e, stages);
+  if (rc) return rc;
   HIP_TRY(hipGetLastError());
   return MCST_OK;
 }
@@ -1158,6 +1258,7 @@ int mcst_engine_replay(mcst_engine* e, uint32_t times) {
   if (!e->graph_exec) return fail(MCST_ERR_STATE, "no captured graph");
   HIP_TRY(hipSetDevice(e->device));
   for (uint32_t k = 0; k < times; ++k) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+  count_steps(e, (uint64_t)times * e->graph_steps);
   return MCST_OK;
 }
 
@@ -1243,6 +1344,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   }
   HIP_TRY(hipEventRecord(t1, e->stream));
   HIP_TRY(hipGetLastError());
+  count_steps(e, n_steps);
   HIP_TRY(hipEventSynchronize(t1));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, t0, t1));

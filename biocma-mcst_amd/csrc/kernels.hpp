@@ -32,4 +32,23 @@ void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* his
 void launch_properties(int model_id, const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial,
                        hipStream_t s);
 
+// Stable sort of one run's particles by position (sort.hip).  pos, status
+// and the n_f64 FP64 fields (state variables and both ages) are the run's
+// own ranges; keys, iota, perm (n u32 each), buf (8n bytes) and temp
+// (temp_bytes, from sort_temp_bytes) are scratch.
+struct SortArgs {
+  uint64_t n = 0;
+  uint32_t end_bit = 1;      // keys are below 2^end_bit
+  uint32_t* pos = nullptr;
+  uint8_t* status = nullptr;
+  double* f64[MAX_VAR + 2] = {};
+  uint32_t n_f64 = 0;
+  uint32_t *keys = nullptr, *iota = nullptr, *perm = nullptr;
+  void* buf = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+};
+hipError_t sort_temp_bytes(uint64_t n, uint32_t end_bit, size_t* bytes);
+hipError_t sort_run_particles(const SortArgs& a, hipStream_t s);
+
 }  // namespace mcst

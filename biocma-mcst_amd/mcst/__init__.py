@@ -34,7 +34,7 @@ EXPORTS = [
     "mcst_run_get_exit_ages", "mcst_run_get_division_ages",
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
-    "mcst_run_set_gas_transfer",
+    "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -102,6 +102,8 @@ def lib():
             "mcst_run_get_properties": [_vp, i32, _vp, _vp],
             "mcst_engine_autotune": [_vp, u32, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)],
             "mcst_run_set_gas_transfer": [_vp, i32, _f64p, _f64p],
+            "mcst_run_set_sort_interval": [_vp, i32, u32],
+            "mcst_run_sort_particles": [_vp, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -251,6 +253,12 @@ class Run:
 
     def set_stage_cap(self, cap):
         _ck(lib().mcst_run_set_division_cap(self._h, self.id, cap), "set_division_cap")
+
+    def set_sort_interval(self, steps):
+        _ck(lib().mcst_run_set_sort_interval(self._h, self.id, steps), "set_sort_interval")
+
+    def sort_particles(self):
+        _ck(lib().mcst_run_sort_particles(self._h, self.id), "sort_particles")
 
     def set_gas_transfer(self, kla, cstar):
         """kla [1/s] and saturation C* [kg/m3], per species (broadcast over

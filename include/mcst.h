@@ -125,6 +125,18 @@ int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min
  * liquid mass of species s in compartment j gains dt*kla[s][j]*(c_star[s][j] - C)*V_j
  * each step; c_star = Henry*C_gas is held constant (no gas transport). */
 int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, const double* c_star);
+/* Particle order (no counterpart in the reference, whose container keeps
+ * particles in creation order): mcst_run_sort_particles reorders the run's
+ * particles by compartment now, stable in slot order; with an interval k > 0
+ * mcst_engine_step does so after every step whose count is a multiple of k
+ * (replays of a captured graph do not).  Slots of one compartment then sit
+ * together, which the particle pass's record reads and contribution window
+ * profit from; moves and newborns scatter the order over time.  The reorder
+ * changes which slot holds which particle, hence the slot-keyed random
+ * draws: results are reproducible for a given interval, and the population
+ * statistics do not depend on it. */
+int mcst_run_set_sort_interval(mcst_engine* e, int32_t run, uint32_t steps);
+int mcst_run_sort_particles(mcst_engine* e, int32_t run);
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
 /* Keep up to `cap` exit ages and division ages (the reference's probes). */
 int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap);

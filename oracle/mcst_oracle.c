@@ -727,6 +727,7 @@ struct orc_run {
   uint64_t inactive, min_removal;
   double ratio;
   uint32_t stage_cap;
+  uint32_t sort_interval; /* reorder by compartment every k steps (0: never) */
   uint64_t counters[ORC_C_COUNT];
   dvec exit_ages, div_ages;
 };
@@ -865,6 +866,44 @@ static int alloc_particles(orc_run* r, uint64_t n, uint64_t cap) {
 
 /* unit.cpp:102-163 (position uniform over compartments, Model::init, sum of
  * mass) and unit.cpp:232-257 (w = X0 * V_total / sum mass). */
+/* Stable reorder of the first n slots by compartment (slot order among equal
+   positions), every per-particle field; the engine's mcst_run_sort_particles
+   applies the same permutation. */
+static int sort_by_compartment(orc_run* r) {
+  const uint64_t n = r->n;
+  if (n < 2 || r->n_c < 2) return 0;
+  uint64_t* next = (uint64_t*)calloc((size_t)r->n_c + 1, sizeof(uint64_t));
+  uint64_t* dst = (uint64_t*)malloc(sizeof(uint64_t) * n);
+  double* tmp = (double*)malloc(sizeof(double) * n);
+  if (!next || !dst || !tmp) {
+    free(next), free(dst), free(tmp);
+    return -2;
+  }
+  for (uint64_t p = 0; p < n; ++p) next[r->pos[p] + 1]++;
+  for (uint32_t c = 0; c < r->n_c; ++c) next[c + 1] += next[c];
+  for (uint64_t p = 0; p < n; ++p) dst[p] = next[r->pos[p]]++;
+  uint32_t* u = (uint32_t*)tmp;
+  for (uint64_t p = 0; p < n; ++p) u[dst[p]] = r->pos[p];
+  memcpy(r->pos, u, sizeof(uint32_t) * n);
+  uint8_t* b = (uint8_t*)tmp;
+  for (uint64_t p = 0; p < n; ++p) b[dst[p]] = r->status[p];
+  memcpy(r->status, b, n);
+  for (int v = 0; v < r->n_var + 2; ++v) {
+    double* x = v < r->n_var ? r->state + (size_t)v * r->cap : (v == r->n_var ? r->age_hyd : r->age_div);
+    for (uint64_t p = 0; p < n; ++p) tmp[dst[p]] = x[p];
+    memcpy(x, tmp, sizeof(double) * n);
+  }
+  free(next), free(dst), free(tmp);
+  return 0;
+}
+
+int orc_sort_particles(orc_run* r) { return sort_by_compartment(r); }
+
+int orc_set_sort_interval(orc_run* r, uint32_t steps) {
+  r->sort_interval = steps;
+  return 0;
+}
+
 int orc_init_particles(orc_run* r, uint64_t n, uint64_t capacity, double x0) {
   if (r->n_maps == 0) return -1;
   if (alloc_particles(r, n, capacity)) return -2;
@@ -882,30 +921,8 @@ int orc_init_particles(orc_run* r, uint64_t n, uint64_t capacity, double x0) {
   for (uint32_t j = 0; j < r->n_c; ++j) vt += r->maps[0].V[j];
   r->weight = n ? (x0 * vt) / total_mass : 0.0;
   /* the drawn particles are stored in compartment order, draw order among
-     equal positions (a stable counting sort; the weight above is summed in
-     draw order) */
-  if (n > 1 && r->n_c > 1) {
-    uint64_t* next = (uint64_t*)calloc((size_t)r->n_c + 1, sizeof(uint64_t));
-    uint64_t* dst = (uint64_t*)malloc(sizeof(uint64_t) * n);
-    uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * n);
-    double* col = (double*)malloc(sizeof(double) * n);
-    if (!next || !dst || !pos || !col) {
-      free(next), free(dst), free(pos), free(col);
-      return -2;
-    }
-    for (uint64_t p = 0; p < n; ++p) next[r->pos[p] + 1]++;
-    for (uint32_t c = 0; c < r->n_c; ++c) next[c + 1] += next[c];
-    for (uint64_t p = 0; p < n; ++p) dst[p] = next[r->pos[p]]++;
-    for (uint64_t p = 0; p < n; ++p) pos[dst[p]] = r->pos[p];
-    memcpy(r->pos, pos, sizeof(uint32_t) * n);
-    for (int v = 0; v < r->n_var; ++v) {
-      double* x = r->state + (size_t)v * r->cap;
-      for (uint64_t p = 0; p < n; ++p) col[dst[p]] = x[p];
-      memcpy(x, col, sizeof(double) * n);
-    }
-    free(next), free(dst), free(pos), free(col);
-  }
-  return 0;
+     equal positions (the weight above is summed in draw order) */
+  return sort_by_compartment(r);
 }
 
 int orc_set_particles(orc_run* r, uint64_t n, uint64_t capacity, const uint32_t* pos, const uint8_t* status,
@@ -1190,6 +1207,7 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
     free(outs);
     r->n = dst;
     r->counters[ORC_C_STEPS]++;
+    if (r->sort_interval && r->counters[ORC_C_STEPS] % r->sort_interval == 0 && sort_by_compartment(r)) return -2;
   }
   return 0;
 }

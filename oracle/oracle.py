@@ -92,6 +92,8 @@ def lib():
         L.orc_get_repartition.argtypes = [C.c_void_p, _u64p]
         L.orc_get_properties.argtypes = [C.c_void_p, _f64p, _f64p]
         L.orc_set_gas_transfer.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_set_sort_interval.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_sort_particles.argtypes = [C.c_void_p]
         for name in ("orc_get_exit_ages", "orc_get_division_ages"):
             getattr(L, name).restype = C.c_uint64
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
@@ -205,6 +207,12 @@ class OracleRun:
 
     def set_stage_cap(self, cap):
         _check(lib().orc_set_stage_cap(self.h, cap), "set_stage_cap")
+
+    def set_sort_interval(self, steps):
+        _check(lib().orc_set_sort_interval(self.h, steps), "set_sort_interval")
+
+    def sort_particles(self):
+        _check(lib().orc_sort_particles(self.h), "sort_particles")
 
     def set_gas_transfer(self, kla, cstar):
         kla = np.ascontiguousarray(np.broadcast_to(np.asarray(kla, float).reshape(self.n_s, -1),

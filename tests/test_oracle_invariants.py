@@ -322,3 +322,40 @@ def test_gas_liquid_transfer_closed_form(oracle_lib):
         np.testing.assert_allclose(C[1, 0], cs * (1 - (1 - 0.01 * kla) ** k), rtol=1e-12)
         np.testing.assert_allclose(r.mass()[1, 0] - m_prev, 0.01 * kla * (cs - c_prev) * 2e-3, rtol=1e-9)
         assert C[0, 0] == 1.5
+
+
+def test_reorder_is_a_stable_sort_by_compartment(oracle_lib):
+    """sort_particles permutes whole particles into compartment order, slot
+    order among equal positions; an interval k sorts after every k-th step."""
+    from mcst import cases
+
+    c = cases.small_case("dual_monod", n_c=9, n_particles=4000, seed=12)
+    c.capacity = 40000
+    r = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    c.apply(r)
+    r.step(15)
+    before = r.particles()
+    n = r.n
+    r.sort_particles()
+    after = r.particles()
+    order = np.argsort(before["pos"], kind="stable")
+    np.testing.assert_array_equal(after["pos"], before["pos"][order])
+    np.testing.assert_array_equal(after["status"], before["status"][order])
+    np.testing.assert_array_equal(after["state"], before["state"][:, order])
+    np.testing.assert_array_equal(after["age_hyd"], before["age_hyd"][order])
+    np.testing.assert_array_equal(after["age_div"], before["age_div"][order])
+    assert np.all(np.diff(after["pos"].astype(np.int64)) >= 0) and r.n == n
+    # an interval equals explicit sorts at the same steps
+    a = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    b = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    for x in (a, b):
+        c.apply(x)
+    a.set_sort_interval(4)
+    a.step(12)
+    for _ in range(3):
+        b.step(4)
+        b.sort_particles()
+    pa, pb = a.particles(), b.particles()
+    for k in pa:
+        np.testing.assert_array_equal(pa[k], pb[k])
+    assert a.counters() == b.counters()

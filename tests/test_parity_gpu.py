@@ -261,3 +261,29 @@ def test_config4_lattice_reduced():
     cnt = g.counters()
     assert cnt["moves"] > 0 and cnt["exits"] > 0 and cnt["births"] > 0 and cnt["compactions"] > 0
     eng.close()
+
+
+@pytest.mark.parametrize("model", ["monod_mass", "quad_monod"])
+def test_periodic_reorder_parity(model):
+    """Reordering by compartment every 5 steps on the device (radix sort of
+    the slots) matches the oracle's counting sort at the same steps; an
+    explicit reorder too."""
+    from mcst import cases
+
+    c = cases.small_case(model, n_c=12, n_particles=6000, seed=31)
+    c.capacity = 60000
+    eng, g, o = make_pair(c)
+    for r in (g, o):
+        r.set_sort_interval(5)
+    for _ in range(4):
+        eng.step(6)
+        o.step(6)
+        assert_parity(g, o)
+    g.sort_particles()
+    o.sort_particles()
+    assert_parity(g, o)
+    eng.step(3)
+    o.step(3)
+    assert_parity(g, o)
+    assert g.counters()["compactions"] > 0 and g.counters()["births"] > 0
+    eng.close()
