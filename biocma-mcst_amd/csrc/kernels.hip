@@ -1146,33 +1146,24 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) { layout_pa
 // ------------------------------------------------------------------ merge
 // Appends staged newborns in tile order.
 __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
-  // each wave takes 64 tiles at a time: one load of their birth counts per
-  // lane, then the wave copies the newborns of the tiles that have any
+  // one lane per tile: it reads the tile's birth count and copies the
+  // tile's newborns (few per tile), so tiles without births cost one load
   const uint32_t T = b.tile_off[b.R];
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
-  for (uint32_t t0 = (blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6)) * 64u; t0 < T; t0 += waves * 64u) {
-    const uint32_t my = t0 + lane;
-    const uint32_t nb_l = my < T ? b.tile_births[my] : 0u;
-    uint64_t todo = __ballot(nb_l != 0u);
-    while (todo) {
-      const uint32_t k = (uint32_t)__ffsll((long long)todo) - 1u;
-      todo &= todo - 1ull;
-      const uint32_t t = t0 + k;
-      const uint32_t nb = __shfl(nb_l, (int)k);
-      const uint32_t r = find_run(b.tile_off, b.R, t);
-      const RunDev& rd = b.runs[r];
-      const RunState* rs = b.rs + r;
-      const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
-      const uint64_t src0 = (uint64_t)t * b.stage_stride;
-      for (uint32_t j = lane; j < nb; j += 64u) {
-        const uint64_t d = dst0 + j, s = src0 + j;
-        b.pos[d] = b.st_pos[s];
-        b.status[d] = IDLE;
-        b.age_hyd[d] = 0.0;
-        b.age_div[d] = 0.0;
-        for (uint32_t v = 0; v < n_var; ++v) b.state[(uint64_t)v * b.P + d] = b.st_state[(uint64_t)v * b.ST + s];
-      }
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+    const uint32_t nb = b.tile_births[t];
+    if (nb == 0) continue;
+    const uint32_t r = find_run(b.tile_off, b.R, t);
+    const RunDev& rd = b.runs[r];
+    const RunState* rs = b.rs + r;
+    const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
+    const uint64_t src0 = (uint64_t)t * b.stage_stride;
+    for (uint32_t j = 0; j < nb; ++j) {
+      const uint64_t d = dst0 + j, s = src0 + j;
+      b.pos[d] = b.st_pos[s];
+      b.status[d] = IDLE;
+      b.age_hyd[d] = 0.0;
+      b.age_div[d] = 0.0;
+      for (uint32_t v = 0; v < n_var; ++v) b.state[(uint64_t)v * b.P + d] = b.st_state[(uint64_t)v * b.ST + s];
     }
   }
 }
