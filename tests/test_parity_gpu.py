@@ -133,6 +133,22 @@ def test_config0_full_size():
     assert_parity(g, o)
 
 
+def test_config0_all_steps():
+    """The 64-compartment ring as specified: 1e5 particles, all 10 000 steps,
+    compared every 2500 steps (the oracle on 8 threads)."""
+    from mcst import cases
+
+    c = cases.config0(n_particles=100_000)
+    eng, g, o = make_pair(c)
+    for _ in range(4):
+        eng.step(2500)
+        o.step(2500, threads=8)
+        assert_parity(g, o)
+    cnt = g.counters()
+    assert cnt["steps"] == 10_000 and cnt["births"] > 0 and cnt["exits"] > 0 and cnt["compactions"] > 0
+    eng.close()
+
+
 def test_config1_full_size_few_steps():
     """The 5000-compartment network at 1e7 particles: parity over 3 steps
     (the oracle is single-threaded here)."""
