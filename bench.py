@@ -63,6 +63,33 @@ def cpu_baseline(case_fn, seconds=12.0, max_steps=20000, n_particles=1_000_000):
                       f"{n_particles:.0e} particles, {done} steps, {el:.1f} s"}
 
 
+def served_with_lattice(n_requests=40, rate=20.0, seed=13, chunk=20, lattice_particles=10_000_000):
+    """The served stream with the lattice tank (quad_monod, its own engine)
+    arriving first and running its 200 steps alongside: the small requests'
+    latencies with a large run in the mix."""
+    from mcst import cases
+    from mcst.serve import Request, Server, percentile
+
+    reqs = [Request(i + 1, t, c, c.n_steps) for i, (t, c) in enumerate(cases.config3_requests(n_requests, rate, seed))]
+    big = cases.config4(n_particles=lattice_particles)
+    big.capacity = int(2.5 * lattice_particles)
+    reqs.append(Request(0, 0.0, big, 200))
+    slots = 2 * sum(r.case.n_particles for r in reqs[:-1]) + 1024 * n_requests
+    srv = Server(pools={"monod_mass": (slots, n_requests), "quad_monod": (big.capacity + 1024, 1)},
+                 budget_bytes=200 * 10**9, chunk_steps=chunk)
+    res = srv.serve(reqs)
+    srv.close()
+    small = [r for r in res if r.rid != 0]
+    lat = [r.latency for r in small]
+    ps = sum(r.particle_steps for r in res)
+    span = max(r.finished for r in res) - min(r.arrival for r in reqs)
+    big_res = [r for r in res if r.rid == 0][0]
+    return {"requests": n_requests, "lattice_particles": lattice_particles,
+            "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99),
+            "lattice_latency_s": big_res.latency, "aggregate_particle_steps_per_s": ps / span,
+            "workload": "the served stream plus the 50 000-compartment lattice run (200 steps) arriving first"}
+
+
 def served_measurement(n_requests=40, rate=20.0, seed=13, chunk=20):
     """The served stream (Poisson arrivals at `rate`/s, ensemble-sized runs with
     200..5000 steps) through the request loop, and the same requests one at a
@@ -222,6 +249,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true", help="also measure the 512-run batched ensemble")
     ap.add_argument("--config4", type=float, default=0, metavar="N",
                     help="also run the 50 000-compartment lattice case with N particles (e.g. 1e8)")
+    ap.add_argument("--served-lattice", type=float, default=0, metavar="N",
+                    help="also serve the stream with an N-particle lattice run in the mix")
     ap.add_argument("--config4-sort", type=int, default=20, metavar="K",
                     help="reorder the lattice case's particles by compartment every K steps (0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -316,6 +345,7 @@ def main():
         log("batched ensemble of 512 runs")
         ensemble = ensemble_measurement()
     lattice = config4_measurement(int(a.config4), sort_interval=a.config4_sort) if a.config4 else None
+    mixed = served_with_lattice(a.served_requests, lattice_particles=int(a.served_lattice)) if a.served_lattice else None
     nnz = case.nnz + case.n_c
     out = {
         "metric": METRIC,
@@ -351,6 +381,8 @@ def main():
         out["ensemble"] = ensemble
     if lattice is not None:
         out["config4"] = lattice
+    if mixed is not None:
+        out["served_with_lattice"] = mixed
     eng.close()
     print(json.dumps(out), flush=True)
 

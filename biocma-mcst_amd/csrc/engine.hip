@@ -807,21 +807,22 @@ int sort_run(mcst_engine* e, uint32_t r) {
   if (n < 2 || h.n_c < 2) return MCST_OK;
   uint32_t end_bit = 1;
   while (end_bit < 32 && ((uint64_t)1 << end_bit) < h.n_c) ++end_bit;
-  if (n > e->sort_n) {
+  size_t need = 0;
+  HIP_TRY(sort_temp_bytes(n, end_bit, &need));
+  if (n > e->sort_n || need > e->sort_temp) {  // grow the scratch (with headroom for later calls)
+    const uint64_t cap = std::max<uint64_t>(n + n / 4, e->sort_n);
     size_t temp = 0;
-    HIP_TRY(sort_temp_bytes(n, end_bit, &temp));
-    temp = std::max(temp, (size_t)(1u << 20)) * 2;  // headroom for wider keys of other runs
+    HIP_TRY(sort_temp_bytes(cap, 32, &temp));  // the widest keys: enough for every run
+    temp = std::max(temp, need);
+    HIP_TRY(hipStreamSynchronize(e->stream));  // the old scratch may still be in use
     if (e->sort_mem) HIP_TRY(hipFree(e->sort_mem));
     e->sort_mem = nullptr;
     e->sort_n = 0;
-    const uint64_t cap = n + n / 4;
+    e->sort_temp = 0;
     HIP_TRY(hipMalloc(&e->sort_mem, cap * (3 * sizeof(uint32_t) + sizeof(double)) + temp + 256));
     e->sort_n = cap;
     e->sort_temp = temp;
   }
-  size_t need = 0;
-  HIP_TRY(sort_temp_bytes(n, end_bit, &need));
-  if (need > e->sort_temp) return fail(MCST_ERR_MEMORY, "reorder scratch too small");
   const BatchDev& b = e->bd;
   const uint64_t o = h.p_off, cap = e->sort_n;
   char* m = (char*)e->sort_mem;

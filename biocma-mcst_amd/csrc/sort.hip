@@ -9,6 +9,8 @@
 // stable sort by position (slot order among equal positions), the same
 // permutation the CPU oracle applies, so runs stay comparable step by step.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <hipcub/device/device_radix_sort.hpp>
 
 #include "kernels.hpp"

@@ -58,84 +58,111 @@ def footprint(case, n_var):
 
 
 class Server:
-    """Continuous batching over one engine with a reserved slot pool."""
+    """Continuous batching with a reserved slot pool: one engine per model
+    (an engine batches runs of one model), all stepping on the same GPU on
+    their own streams.
 
-    def __init__(self, model, slots, max_runs, budget_bytes, chunk_steps=20, device=0, engine=None):
-        self.model = model
-        if engine is None:
-            _, self.n_var, _, _ = model_info(model)
-            engine = Engine(model, device)
-            engine.reserve(slots, max_runs)
-            engine.commit()
-        else:
-            self.n_var = engine.n_var
-        self.eng = engine
+    Server(model, slots, max_runs, ...) serves one model; `pools` maps more
+    models to their (slots, max_runs).  A request's model is its case's."""
+
+    def __init__(self, model=None, slots=0, max_runs=0, budget_bytes=200 * 10**9, chunk_steps=20, device=0,
+                 engine=None, pools=None):
+        self.engines = {}
+        self.n_var = {}
+        if engine is not None:
+            self.engines[model] = engine
+            self.n_var[model] = engine.n_var
+        pools = dict(pools or {})
+        if model is not None and engine is None:
+            pools.setdefault(model, (slots, max_runs))
+        for m, (sl, mr) in pools.items():
+            if m in self.engines:
+                continue
+            _, self.n_var[m], _, _ = model_info(m)
+            e = Engine(m, device)
+            e.reserve(sl, mr)
+            e.commit()
+            self.engines[m] = e
+        self.model = model if model is not None else next(iter(self.engines))
+        self.eng = self.engines[self.model]
         self.budget = budget_bytes
         self.chunk = chunk_steps
 
     def close(self):
-        self.eng.close()
+        for e in self.engines.values():
+            e.close()
+
+    def _engine(self, req):
+        m = getattr(req.case, "model", self.model)
+        if m not in self.engines:
+            raise KeyError(f"request {req.rid}: no engine serves model {m!r}")
+        return m, self.engines[m]
 
     def serve(self, requests, on_finish=None, clock=time.perf_counter, sleep=time.sleep):
         """Serve `requests` (in arrival order); returns one Result each.
         `on_finish(request, run)` may read a run's results before it leaves."""
-        eng = self.eng
         queue = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        waiting = collections.deque()   # arrived, not yet defined on the engine
-        defined = collections.deque()   # defined, waiting for room in the pool
-        live = {}                       # run id -> (request, run, admitted time, bytes)
+        waiting = collections.deque()   # arrived, not yet defined on an engine
+        defined = {m: collections.deque() for m in self.engines}  # defined, waiting for room in the pool
+        live = {}                       # (model, run id) -> (request, run, admitted time, bytes)
         used = 0
         results = []
         t0 = clock()
-        while queue or waiting or defined or live:
+        while queue or waiting or any(defined.values()) or live:
             now = clock() - t0
             while queue and queue[0].arrival <= now:
                 waiting.append(queue.popleft())
             while waiting:  # define arrivals while the footprint budget allows
                 req = waiting[0]
-                fp = footprint(req.case, self.n_var)
+                m, eng = self._engine(req)
+                fp = footprint(req.case, self.n_var[m])
                 if live and used + fp > self.budget:
                     break
                 waiting.popleft()
                 run = eng.add_run(req.case.n_c, req.case.dt, req.case.seed)
                 req.case.apply(run)
                 run.set_step_target(req.steps)
-                defined.append((req, run, fp))
+                defined[m].append((req, run, fp))
                 used += fp
-            if defined:
-                k = eng.admit()
+            for m, dq in defined.items():
+                if not dq:
+                    continue
+                k = self.engines[m].admit()
                 t_adm = clock() - t0
                 for _ in range(k):
-                    req, run, fp = defined.popleft()
-                    live[run.id] = (req, run, t_adm, fp)
-                if defined and not live:
-                    raise RuntimeError(f"request {defined[0][0].rid} does not fit the engine's pool")
+                    req, run, fp = dq.popleft()
+                    live[(m, run.id)] = (req, run, t_adm, fp)
+                if dq and not any(key[0] == m for key in live):
+                    raise RuntimeError(f"request {dq[0][0].rid} does not fit the engine's pool")
             if not live:
                 if queue:
                     sleep(max(0.0, min(0.05, queue[0].arrival - (clock() - t0))))
                 continue
-            eng.step(self.chunk)
-            eng.synchronize()
+            stepping = {key[0] for key in live}
+            for m in stepping:  # launches are asynchronous: the engines' streams overlap
+                self.engines[m].step(self.chunk)
+            for m in stepping:
+                self.engines[m].synchronize()
             t_now = clock() - t0
-            for rid in list(live):
-                req, run, t_adm, fp = live[rid]
+            for key in list(live):
+                req, run, t_adm, fp = live[key]
                 c = run.counters()
                 if c["steps"] < req.steps:
                     continue
                 if on_finish is not None:
                     on_finish(req, run)
                 results.append(Result(req.rid, req.arrival, t_adm, t_now, int(c["steps"]), int(c["processed"])))
-                eng.retire(run)
+                self.engines[key[0]].retire(run)
                 used -= fp
-                del live[rid]
+                del live[key]
         return sorted(results, key=lambda r: r.rid)
 
 
-def run_alone(req, model, device=0):
+def run_alone(req, model=None, device=0):
     """The same request on its own engine, stepped to completion; returns
     (seconds, particle-steps)."""
     t0 = time.perf_counter()
-    eng = Engine(model, device)
+    eng = Engine(model or req.case.model, device)
     run = eng.add_run(req.case.n_c, req.case.dt, req.case.seed)
     req.case.apply(run)
     eng.commit()

@@ -134,3 +134,26 @@ def test_ragged_ensemble_of_48_runs():
     for c, r in zip(cs, runs):
         assert_parity(r, _oracle(c, 8), probes=False)
     eng.close()
+
+
+def test_two_models_served_together():
+    """Requests of two models (one engine each, on their own streams) served
+    in one loop: every result matches its oracle run."""
+    from mcst import cases
+    from mcst.serve import Request, Server
+
+    reqs = []
+    for i, k in enumerate([20, 35, 15]):
+        c = _fast(cases.ensemble_run(i, seed=5, n_steps=k, n_c_range=(8, 60), n_range=(1000, 6000)))
+        reqs.append(Request(i, 0.01 * i, c, k))
+    q = cases.small_case("quad_monod", n_c=12, n_particles=5000, seed=8)
+    q.capacity = 40000
+    reqs.append(Request(3, 0.005, q, 26))
+    snaps = {}
+    srv = Server(pools={"monod_mass": (8 * 12 * 1024, 8), "quad_monod": (48 * 1024, 2)}, budget_bytes=1 << 40,
+                 chunk_steps=6)
+    res = srv.serve(reqs, on_finish=lambda req, run: snaps.__setitem__(req.rid, snapshot(run)))
+    srv.close()
+    assert [r.steps for r in res] == [20, 35, 15, 26]
+    for req in reqs:
+        assert_snapshot(snaps[req.rid], _oracle(req.case, req.steps))
