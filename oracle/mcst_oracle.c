@@ -26,6 +26,23 @@
  *   repartition ........... apps/libs/mc/src/unit.cpp:67-100,190-230
  *   models ................ apps/libs/models/public/models/monod.hpp:80-193,
  *                           fixed_length.hpp:360-413, simple_acetate.hpp:132-248, utils.hpp:63-67
+ *                           (monod_mass, dual_monod, quad_monod: the benchmark cases' models)
+ *   gas-liquid transfer ... apps/libs/simulation/src/implScalar.cpp:228-248 (liquid side, fixed gas)
+ *   export properties ..... apps/core/public/core/post_process.hpp:54-98
+ *
+ * Checks that pin each part (tests/test_oracle_kat.py, test_oracle_invariants.py,
+ * test_export_cpu.py, test_case_io_cpu.py): Random123 and numpy Philox vectors;
+ * scipy for the normal quantile and for sparse transport; the reference's
+ * transition-matrix, 16-compartment flow, feed and container fixtures; closed
+ * forms of the model updates (Euler at constant substrate, uptake lag, the
+ * oxygen-free overflow of quad_monod), of gas-liquid transfer in a 0-D box and
+ * of the move/exit probabilities; exact distributions (chi-square, KS) of moves,
+ * residence times and initial states; mass balance; a numpy stable sort for the
+ * reorder; math.fsum for the property sums.
+ * Parity unpinned: which random number each particle receives in the
+ * reference (Kokkos XorShift1024 pool, replaced by Philox by design), rcmtool's
+ * neighbour order and CDF padding (here: ascending neighbours, padding repeats
+ * the last), and Eigen's summation order inside C*M (here: ascending source).
  *
  * Deliberate, documented differences from the reference (the reference is
  * itself scheduling-dependent, so parity is defined against these rules):
@@ -38,7 +55,9 @@
  *     parent slot order;
  *   - removal fills the i-th gap (ascending) with the i-th live particle from
  *     the tail (descending) instead of an atomic tail offset;
- *   - per-particle state, contributions and ages are FP64.
+ *   - per-particle state, contributions and ages are FP64;
+ *   - initial particles are stored in compartment order, and a run may be
+ *     reordered by compartment every k steps (the engine does the same).
  */
 #include "mcst_oracle.h"
 
