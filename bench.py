@@ -33,8 +33,8 @@ def log(msg):
 
 def read_traffic():
     """HBM bytes per particle-kernel launch measured with rocprofv3 PMC
-    counters (profiles/particle_config1_traffic.txt), if present."""
-    path = os.path.join(ROOT, "profiles", "particle_config1_traffic.txt")
+    counters (profiles/particle_rgg5000_traffic.txt), if present."""
+    path = os.path.join(ROOT, "profiles", "particle_rgg5000_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -70,8 +70,8 @@ def served_with_lattice(n_requests=40, rate=20.0, seed=13, chunk=20, lattice_par
     from mcst import cases
     from mcst.serve import Request, Server, percentile
 
-    reqs = [Request(i + 1, t, c, c.n_steps) for i, (t, c) in enumerate(cases.config3_requests(n_requests, rate, seed))]
-    big = cases.config4(n_particles=lattice_particles)
+    reqs = [Request(i + 1, t, c, c.n_steps) for i, (t, c) in enumerate(cases.served_stream(n_requests, rate, seed))]
+    big = cases.lattice50k(n_particles=lattice_particles)
     big.capacity = int(2.5 * lattice_particles)
     reqs.append(Request(0, 0.0, big, 200))
     slots = 2 * sum(r.case.n_particles for r in reqs[:-1]) + 1024 * n_requests
@@ -98,7 +98,7 @@ def served_measurement(n_requests=40, rate=20.0, seed=13, chunk=20):
     from mcst import cases
     from mcst.serve import Request, Server, percentile, run_alone
 
-    reqs = [Request(i, t, c, c.n_steps) for i, (t, c) in enumerate(cases.config3_requests(n_requests, rate, seed))]
+    reqs = [Request(i, t, c, c.n_steps) for i, (t, c) in enumerate(cases.served_stream(n_requests, rate, seed))]
     slots = 2 * sum(r.case.n_particles for r in reqs) + 1024 * n_requests
 
     def serve(rs):
@@ -136,7 +136,7 @@ def served_measurement(n_requests=40, rate=20.0, seed=13, chunk=20):
                   "p99_s": percentile([r.latency for r in burst], 99),
                   "aggregate_particle_steps_per_s": sum(r.particle_steps for r in burst) / burst_span,
                   "speedup_vs_sequential": (sum(r.particle_steps for r in burst) / burst_span) / (ps_seq / seq_span)},
-        "workload": "config3 stream: monod_mass runs, 32..4000 compartments, 1e4..2e6 particles, "
+        "workload": "served stream: monod_mass runs, 32..4000 compartments, 1e4..2e6 particles, "
                     "200..5000 steps, Poisson arrivals",
     }
 
@@ -148,7 +148,7 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
     import mcst
     from mcst import cases
 
-    cs = cases.config2(n_runs, seed, steps)
+    cs = cases.ensemble512(n_runs, seed, steps)
     eng = mcst.Engine("monod_mass")
     runs = []
     for c in cs:
@@ -175,7 +175,7 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
             "autotuned": {"block": tuned["block"], "tile_group": tuned["group"], "trial_steps": done}}
 
 
-def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sort_interval=20):
+def lattice_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sort_interval=20):
     """The 50 000-compartment lattice tank at full size: one run of
     quad_monod (8 FP64 states, division age, hydraulic age), 4 species,
     outlet washout with device compaction, 200 steps of 5 ms, all timed."""
@@ -183,7 +183,7 @@ def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sor
     from mcst import cases
 
     log(f"building the 50x50x20 lattice case with {n_particles:.0e} particles")
-    case = cases.config4(n_particles=n_particles)
+    case = cases.lattice50k(n_particles=n_particles)
     eng = mcst.Engine(case.model)
     run = eng.add_run(case.n_c, case.dt, case.seed)
     t0 = time.perf_counter()
@@ -206,7 +206,7 @@ def config4_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sor
     b_ps = 2 * (8 * (nv + 2) + 4 + 1)  # states, division age, hydraulic age (the top layer is an outlet)
     achieved = per_launch * b_ps / (per[1] * 1e-3) / 1e9
     eng.close()
-    log(f"config4: {ps / el:.3e} particle-steps/s, {el * 1e3 / steps:.2f} ms/step, particle pass {achieved:.0f} GB/s")
+    log(f"lattice50k: {ps / el:.3e} particle-steps/s, {el * 1e3 / steps:.2f} ms/step, particle pass {achieved:.0f} GB/s")
     return {"workload": "50x50x20 lattice tank, 50 000 compartments, 4 species, quad_monod (8 states), "
                         "outlet washout, dt=0.005 s", "particles_initial": int(c0["n"]), "particles_final": int(c1["n"]),
             "steps": steps, "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / steps,
@@ -247,11 +247,11 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--served-requests", type=int, default=40)
     ap.add_argument("--ensemble", action="store_true", help="also measure the 512-run batched ensemble")
-    ap.add_argument("--config4", type=float, default=0, metavar="N",
+    ap.add_argument("--lattice50k", type=float, default=0, metavar="N",
                     help="also run the 50 000-compartment lattice case with N particles (e.g. 1e8)")
     ap.add_argument("--served-lattice", type=float, default=0, metavar="N",
                     help="also serve the stream with an N-particle lattice run in the mix")
-    ap.add_argument("--config4-sort", type=int, default=20, metavar="K",
+    ap.add_argument("--lattice50k-sort", type=int, default=20, metavar="K",
                     help="reorder the lattice case's particles by compartment every K steps (0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this many MiB (PMC counter calibration)")
@@ -261,14 +261,15 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "see DESIGN.md for how the step shards across GPUs", file=sys.stderr)
+              "the step would shard by particles with one all-reduce of the contribution matrix per step",
+              file=sys.stderr)
         sys.exit(2)
 
     import mcst
     from mcst import cases
 
     log("building the 5000-compartment case")
-    case = cases.config1(n_particles=a.particles)
+    case = cases.rgg5000(n_particles=a.particles)
     eng = mcst.Engine(case.model)
     run = eng.add_run(case.n_c, case.dt, case.seed)
     case.capacity = 2 * a.particles
@@ -332,7 +333,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("timing the CPU baseline")
-        cpu = cpu_baseline(cases.config1)
+        cpu = cpu_baseline(cases.rgg5000)
         log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
     served = None
     if not a.no_served:
@@ -344,7 +345,7 @@ def main():
     if a.ensemble:
         log("batched ensemble of 512 runs")
         ensemble = ensemble_measurement()
-    lattice = config4_measurement(int(a.config4), sort_interval=a.config4_sort) if a.config4 else None
+    lattice = lattice_measurement(int(a.lattice50k), sort_interval=a.lattice50k_sort) if a.lattice50k else None
     mixed = served_with_lattice(a.served_requests, lattice_particles=int(a.served_lattice)) if a.served_lattice else None
     nnz = case.nnz + case.n_c
     out = {
@@ -380,7 +381,7 @@ def main():
     if ensemble is not None:
         out["ensemble"] = ensemble
     if lattice is not None:
-        out["config4"] = lattice
+        out["lattice50k"] = lattice
     if mixed is not None:
         out["served_with_lattice"] = mixed
     eng.close()

@@ -96,7 +96,7 @@ def geometric_graph_cylinder(n, mean_degree=6.0, rng=None, radius=1.0, height=1.
     return e[np.lexsort((e[:, 1], e[:, 0]))], pts
 
 
-def config0(n_particles=100_000, n_steps=10_000, seed=1):
+def ring64(n_particles=100_000, n_steps=10_000, seed=1):
     """64-compartment ring + 16 chords, 1 species, mass-based Monod, static flowmap."""
     rng = np.random.default_rng(seed)
     n = 64
@@ -106,12 +106,12 @@ def config0(n_particles=100_000, n_steps=10_000, seed=1):
     rows, cols, flows = _symmetric(edges, q)
     C0 = np.full((1, n), 5.0)
     p = np.array([0.5 / 3600.0, 0.01, 0.5, 2.0, 1e-15, 0.1])
-    return Case("config0_ring64", "monod_mass", n, 1, 0.01, seed, [(V, rows, cols, flows)],
+    return Case("ring64", "monod_mass", n, 1, 0.01, seed, [(V, rows, cols, flows)],
                 feeds=[("constant", 1e-5, 0.0, 0, 32, [0], [10.0])], C0=C0, params=p,
                 n_particles=n_particles, x0=1.0, n_steps=n_steps)
 
 
-def config1(n_particles=10_000_000, n_steps=1000, seed=7, n_c=5000, n_maps=20, steps_per_map=100):
+def rgg5000(n_particles=10_000_000, n_steps=1000, seed=7, n_c=5000, n_maps=20, steps_per_map=100):
     """5000-compartment geometric graph, 20 cyclic flowmaps, 2 species, dual-substrate Monod."""
     rng = np.random.default_rng(seed)
     edges, _ = geometric_graph_cylinder(n_c, 6.0, rng)
@@ -123,7 +123,7 @@ def config1(n_particles=10_000_000, n_steps=1000, seed=7, n_c=5000, n_maps=20, s
         maps.append((V, rows, cols, flows))
     C0 = np.stack([np.full(n_c, 5.0), np.full(n_c, 8e-3)])
     p = np.array([2.8e-4, 0.01, 1e-4, 0.5, 1.0, 100.0, 1e-4, 2e-15, 1.4e-15, 0.2e-15])
-    return Case("config1_rgg5000", "dual_monod", n_c, 2, 0.01, seed, maps, steps_per_map=steps_per_map,
+    return Case("rgg5000", "dual_monod", n_c, 2, 0.01, seed, maps, steps_per_map=steps_per_map,
                 C0=C0, params=p, n_particles=n_particles, x0=1.0, n_steps=n_steps)
 
 
@@ -197,10 +197,10 @@ def lattice_tank(dims=(50, 50, 20), loop_flow=1e-3, exchange=(1e-6, 1e-4), rng=N
     return rows[first].astype(np.uint32), cols[first].astype(np.uint32), summed
 
 
-QUAD_MONOD_CONFIG4 = np.array([2.0, 0.01, 1e-4, 0.01, 0.5, 0.5, 0.3, 0.3, 0.4, 0.3, 0.2, 0.4, 2e-15, 1.5e-15, 0.2e-15])
+QUAD_MONOD_LATTICE = np.array([2.0, 0.01, 1e-4, 0.01, 0.5, 0.5, 0.3, 0.3, 0.4, 0.3, 0.2, 0.4, 2e-15, 1.5e-15, 0.2e-15])
 
 
-def config4(n_particles=100_000_000, n_steps=200, seed=17, dims=(50, 50, 20), feed_flow=2e-3, kla_o2=1.0):
+def lattice50k(n_particles=100_000_000, n_steps=200, seed=17, dims=(50, 50, 20), feed_flow=2e-3, kla_o2=1.0):
     """50x50x20 lattice tank (50 000 compartments, 1 L each +-20 %), loop flows
     1e-3 m3/s plus log-uniform 1e-6..1e-4 m3/s exchanges; feeds enter through
     the bottom layer and leave through the top layer (one feed per column,
@@ -221,8 +221,8 @@ def config4(n_particles=100_000_000, n_steps=200, seed=17, dims=(50, 50, 20), fe
             feeds.append(("constant", feed_flow, 0.0, bottom, top, [0, 1], [10.0, 8e-3]))
     C0 = np.stack([np.full(n, 5.0), np.full(n, 8e-3), np.zeros(n), np.zeros(n)])
     gas = (np.array([0.0, kla_o2, 0.0, 0.0]), np.array([0.0, 8e-3, 0.0, 0.0]))
-    return Case("config4_lattice50k", "quad_monod", n, 4, 0.005, seed, [(V, rows, cols, flows)], feeds=feeds,
-                C0=C0, params=QUAD_MONOD_CONFIG4.copy(), n_particles=n_particles, capacity=int(2.5 * n_particles),
+    return Case("lattice50k", "quad_monod", n, 4, 0.005, seed, [(V, rows, cols, flows)], feeds=feeds,
+                C0=C0, params=QUAD_MONOD_LATTICE.copy(), n_particles=n_particles, capacity=int(2.5 * n_particles),
                 x0=0.005, n_steps=n_steps, gas=gas)
 
 
@@ -268,14 +268,14 @@ def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, s
                 C0=C0, params=p, n_particles=n_particles, x0=1.0)
 
 
-MONOD_MASS_CONFIG0 = np.array([0.5 / 3600.0, 0.01, 0.5, 2.0, 1e-15, 0.1])
+MONOD_MASS_RING64 = np.array([0.5 / 3600.0, 0.01, 0.5, 2.0, 1e-15, 0.1])
 
 
 def ensemble_run(i, seed=11, n_steps=500, n_c_range=(32, 4000), n_range=(1e4, 2e6)):
-    """Run i of the batched ensemble: a geometric-graph network as in config1
+    """Run i of the batched ensemble: a geometric-graph network as in rgg5000
     with a log-uniform compartment count, a log-uniform particle count, its own
     seed and Monod parameters (mu_max, K_s, yield) drawn uniform +-30 % around
-    config0's.  One species (glucose 5 g/L), one static flowmap, no feed."""
+    ring64's.  One species (glucose 5 g/L), one static flowmap, no feed."""
     rng = np.random.default_rng([seed, i])
     n_c = int(round(math.exp(rng.uniform(math.log(n_c_range[0]), math.log(n_c_range[1])))))
     n_p = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
@@ -283,19 +283,19 @@ def ensemble_run(i, seed=11, n_steps=500, n_c_range=(32, 4000), n_range=(1e4, 2e
     V = rng.uniform(0.1, 1.0, n_c) * L
     q = rng.lognormal(math.log(1e-4), 1.0, len(edges))
     rows, cols, flows = _symmetric(edges, q)
-    p = MONOD_MASS_CONFIG0.copy()
+    p = MONOD_MASS_RING64.copy()
     p[:3] *= rng.uniform(0.7, 1.3, 3)
     C0 = np.full((1, n_c), 5.0)
     return Case(f"ensemble_{i}", "monod_mass", n_c, 1, 0.01, int(seed) * 1_000_003 + i, [(V, rows, cols, flows)],
                 C0=C0, params=p, n_particles=n_p, x0=1.0, n_steps=n_steps)
 
 
-def config2(n_runs=512, seed=11, n_steps=500, **kw):
+def ensemble512(n_runs=512, seed=11, n_steps=500, **kw):
     """512 independent runs of uneven size (the batched ensemble)."""
     return [ensemble_run(i, seed, n_steps, **kw) for i in range(n_runs)]
 
 
-def config3_requests(n_requests, rate=20.0, seed=13, steps_range=(200, 5000), **kw):
+def served_stream(n_requests, rate=20.0, seed=13, steps_range=(200, 5000), **kw):
     """The served stream: Poisson arrivals at `rate` per second, each request
     an ensemble run with a log-uniform step count.  Returns
     [(arrival_seconds, case)] in arrival order."""

@@ -29,7 +29,7 @@ def test_reads_the_reference_0d_case():
 def test_round_trip_of_a_cyclic_case(tmp_path, sparse):
     from mcst import case_io, cases
 
-    c = cases.config1(n_particles=0, n_c=300, n_maps=4, steps_per_map=7)
+    c = cases.rgg5000(n_particles=0, n_c=300, n_maps=4, steps_per_map=7)
     case_io.write_case(str(tmp_path), [m[0] for m in c.maps], [m[1:] for m in c.maps], sparse=sparse)
     cma = case_io.load_case(str(tmp_path))
     assert cma.n_maps == 4 and cma.n_c == 300

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 def test_hip_run_from_a_case_directory(tmp_path):
     from mcst import case_io, cases
 
-    c = cases.config1(n_particles=200_000, n_c=400, n_maps=3, steps_per_map=5)
+    c = cases.rgg5000(n_particles=200_000, n_c=400, n_maps=3, steps_per_map=5)
     case_io.write_case(str(tmp_path), [m[0] for m in c.maps], [m[1:] for m in c.maps])
     cma = case_io.load_case(str(tmp_path))
     d = case_io.to_case(cma, c.model, c.C0, dt=c.dt, t_per_flowmap=4 * c.dt, params=c.params,

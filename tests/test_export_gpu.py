@@ -28,7 +28,7 @@ def test_properties_match_oracle(model):
 def test_exporter_on_a_hip_run(tmp_path):
     from mcst import cases, export
 
-    c = cases.config0(n_particles=20000)
+    c = cases.ring64(n_particles=20000)
     eng, g, o = make_pair(c)
     ex = export.Exporter(str(tmp_path / "run.npz"), c.model)
     for _ in range(2):

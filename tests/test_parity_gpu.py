@@ -121,11 +121,11 @@ def test_graph_replay_matches_eager():
         assert_parity(g, o)
 
 
-def test_config0_full_size():
+def test_ring64_full_size():
     """The 64-compartment ring at its full 1e5 particles, 300 steps."""
     from mcst import cases
 
-    c = cases.config0(n_particles=100_000)
+    c = cases.ring64(n_particles=100_000)
     eng, g, o = make_pair(c)
     for _ in range(3):
         eng.step(100)
@@ -133,12 +133,12 @@ def test_config0_full_size():
     assert_parity(g, o)
 
 
-def test_config0_all_steps():
+def test_ring64_all_steps():
     """The 64-compartment ring as specified: 1e5 particles, all 10 000 steps,
     compared every 2500 steps (the oracle on 8 threads)."""
     from mcst import cases
 
-    c = cases.config0(n_particles=100_000)
+    c = cases.ring64(n_particles=100_000)
     eng, g, o = make_pair(c)
     for _ in range(4):
         eng.step(2500)
@@ -149,12 +149,12 @@ def test_config0_all_steps():
     eng.close()
 
 
-def test_config1_full_size_few_steps():
+def test_rgg5000_full_size_few_steps():
     """The 5000-compartment network at 1e7 particles: parity over 3 steps
     (the oracle is single-threaded here)."""
     from mcst import cases
 
-    c = cases.config1(n_particles=10_000_000)
+    c = cases.rgg5000(n_particles=10_000_000)
     eng, g, o = make_pair(c)
     eng.step(3)
     o.step(3)
@@ -213,13 +213,13 @@ def test_step_target_stops_a_run():
     assert_parity(b, ob)
 
 
-def test_glucose_equivalent_mass_balance_config1():
+def test_glucose_equivalent_mass_balance_rgg5000():
     """Closed dual-substrate system: liquid glucose + pending sources +
     weight * sum(m/Yxs + X) is conserved to 1e-10 relative."""
     from mcst import cases
     import mcst
 
-    c = cases.config1(n_particles=1_000_000, n_maps=3, steps_per_map=2)
+    c = cases.rgg5000(n_particles=1_000_000, n_maps=3, steps_per_map=2)
     eng = mcst.Engine(c.model)
     g = eng.add_run(c.n_c, c.dt, c.seed)
     c.apply(g)
@@ -261,13 +261,13 @@ def test_gas_liquid_transfer_parity():
     eng.close()
 
 
-def test_config4_lattice_reduced():
+def test_lattice50k_reduced():
     """The 50 000-compartment lattice tank (4 species, fixed-gas oxygen
     supply, 2500 bottom-to-top feeds, quad_monod) at 2e5 particles: parity
     over 24 steps of 5 ms, with exits, divisions and compactions."""
     from mcst import cases
 
-    c = cases.config4(n_particles=200_000)
+    c = cases.lattice50k(n_particles=200_000)
     c.capacity = 600_000
     eng, g, o = make_pair(c)
     for _ in range(3):

@@ -116,12 +116,12 @@ def test_admit_reports_a_full_pool_and_retries():
 
 
 def test_ragged_ensemble_of_48_runs():
-    """A config2-shaped batch (uneven compartment and particle counts, own
+    """A ensemble512-shaped batch (uneven compartment and particle counts, own
     seeds and parameters), smaller sizes: every run matches its oracle."""
     import mcst
     from mcst import cases
 
-    cs = cases.config2(n_runs=48, seed=3, n_steps=8, n_c_range=(32, 600), n_range=(500, 40000))
+    cs = cases.ensemble512(n_runs=48, seed=3, n_steps=8, n_c_range=(32, 600), n_range=(500, 40000))
     eng = mcst.Engine("monod_mass")
     runs = []
     for c in cs:
