@@ -1380,6 +1380,20 @@ int mcst_run_get_counters(mcst_engine* e, int32_t run, uint64_t* counters) {
   return MCST_OK;
 }
 
+int mcst_engine_get_steps(mcst_engine* e, uint32_t* steps, uint32_t n_runs) {
+  CHECK_ENGINE(e);
+  if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
+  if (!steps) return fail(MCST_ERR_ARG, "null output");
+  if (n_runs > e->runs.size()) return fail(MCST_ERR_ARG, "more runs asked for than defined");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  const uint32_t on_dev = std::min<uint32_t>(n_runs, e->runs_cap);
+  std::vector<RunState> st(on_dev);
+  if (on_dev) HIP_TRY(hipMemcpy(st.data(), e->d_rs, on_dev * sizeof(RunState), hipMemcpyDeviceToHost));
+  for (uint32_t r = 0; r < n_runs; ++r) steps[r] = (r < on_dev && e->runs[r].phase == 1) ? st[r].step : 0u;
+  return MCST_OK;
+}
+
 int mcst_run_get_n(mcst_engine* e, int32_t run, uint64_t* n) {
   CHECK_READY(e, run);
   RunState s;

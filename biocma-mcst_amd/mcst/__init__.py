@@ -35,6 +35,7 @@ EXPORTS = [
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
     "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
+    "mcst_engine_get_steps",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -104,6 +105,7 @@ def lib():
             "mcst_run_set_gas_transfer": [_vp, i32, _f64p, _f64p],
             "mcst_run_set_sort_interval": [_vp, i32, u32],
             "mcst_run_sort_particles": [_vp, i32],
+            "mcst_engine_get_steps": [_vp, _u32p, u32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -199,6 +201,13 @@ class Engine:
 
     def synchronize(self):
         _ck(lib().mcst_engine_synchronize(self.h), "synchronize")
+
+    def run_steps(self):
+        """Step count of every run defined on this engine (0 off the device),
+        read from the device at once."""
+        out = np.zeros(max(len(self.runs), 1), np.uint32)
+        _ck(lib().mcst_engine_get_steps(self.h, out, len(self.runs)), "get_steps")
+        return out[: len(self.runs)]
 
     def autotune(self, steps=3):
         """Pick the particle pass's launch shape by timing each candidate over

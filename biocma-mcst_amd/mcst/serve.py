@@ -100,7 +100,12 @@ class Server:
 
     def serve(self, requests, on_finish=None, clock=time.perf_counter, sleep=time.sleep):
         """Serve `requests` (in arrival order); returns one Result each.
-        `on_finish(request, run)` may read a run's results before it leaves."""
+        `on_finish(request, run)` may read a run's results before it leaves.
+
+        Each turn launches a chunk of steps on every engine with live runs,
+        defines the requests that have arrived (building their particles on
+        the host) while the GPU steps, then retires the runs that are done
+        and admits waiting ones, which join at the next chunk."""
         queue = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()   # arrived, not yet defined on an engine
         defined = {m: collections.deque() for m in self.engines}  # defined, waiting for room in the pool
@@ -109,6 +114,9 @@ class Server:
         results = []
         t0 = clock()
         while queue or waiting or any(defined.values()) or live:
+            stepping = {key[0] for key in live}
+            for m in stepping:  # launches are asynchronous: the engines' streams overlap
+                self.engines[m].step(self.chunk)
             now = clock() - t0
             while queue and queue[0].arrival <= now:
                 waiting.append(queue.popleft())
@@ -116,7 +124,7 @@ class Server:
                 req = waiting[0]
                 m, eng = self._engine(req)
                 fp = footprint(req.case, self.n_var[m])
-                if live and used + fp > self.budget:
+                if (live or any(defined.values())) and used + fp > self.budget:
                     break
                 waiting.popleft()
                 run = eng.add_run(req.case.n_c, req.case.dt, req.case.seed)
@@ -124,6 +132,22 @@ class Server:
                 run.set_step_target(req.steps)
                 defined[m].append((req, run, fp))
                 used += fp
+            for m in stepping:
+                self.engines[m].synchronize()
+            if stepping:
+                t_now = clock() - t0
+                steps = {m: self.engines[m].run_steps() for m in stepping}  # one device read per engine
+                for key in list(live):
+                    req, run, t_adm, fp = live[key]
+                    if steps[key[0]][run.id] < req.steps:
+                        continue
+                    if on_finish is not None:
+                        on_finish(req, run)
+                    c = run.counters()
+                    results.append(Result(req.rid, req.arrival, t_adm, t_now, int(c["steps"]), int(c["processed"])))
+                    self.engines[key[0]].retire(run)
+                    used -= fp
+                    del live[key]
             for m, dq in defined.items():
                 if not dq:
                     continue
@@ -134,27 +158,8 @@ class Server:
                     live[(m, run.id)] = (req, run, t_adm, fp)
                 if dq and not any(key[0] == m for key in live):
                     raise RuntimeError(f"request {dq[0][0].rid} does not fit the engine's pool")
-            if not live:
-                if queue:
-                    sleep(max(0.0, min(0.05, queue[0].arrival - (clock() - t0))))
-                continue
-            stepping = {key[0] for key in live}
-            for m in stepping:  # launches are asynchronous: the engines' streams overlap
-                self.engines[m].step(self.chunk)
-            for m in stepping:
-                self.engines[m].synchronize()
-            t_now = clock() - t0
-            for key in list(live):
-                req, run, t_adm, fp = live[key]
-                c = run.counters()
-                if c["steps"] < req.steps:
-                    continue
-                if on_finish is not None:
-                    on_finish(req, run)
-                results.append(Result(req.rid, req.arrival, t_adm, t_now, int(c["steps"]), int(c["processed"])))
-                self.engines[key[0]].retire(run)
-                used -= fp
-                del live[key]
+            if not live and queue and not waiting:
+                sleep(max(0.0, min(0.05, queue[0].arrival - (clock() - t0))))
         return sorted(results, key=lambda r: r.rid)
 
 

@@ -200,6 +200,10 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
 int mcst_engine_autotune(mcst_engine* e, uint32_t steps, int32_t* block, int32_t* group, double* particle_ms);
 
 /* ---- results ---- */
+/* Step counts of runs 0..n_runs-1 with one read of the device (0 for runs
+ * not on the device): what a request loop polls between chunks of steps.
+ * No counterpart in the reference (one simulation per process there). */
+int mcst_engine_get_steps(mcst_engine* e, uint32_t* steps, uint32_t n_runs);
 int mcst_run_get_counters(mcst_engine* e, int32_t run, uint64_t* counters /* MCST_C_COUNT */);
 int mcst_run_get_n(mcst_engine* e, int32_t run, uint64_t* n);
 int mcst_run_get_weight(mcst_engine* e, int32_t run, double* weight);

@@ -84,6 +84,9 @@ class FakeEngine:
     def synchronize(self):
         pass
 
+    def run_steps(self):
+        return [r.steps if r.state == "live" else 0 for r in self.runs]
+
     def retire(self, r):
         assert r.state == "live" and r.steps == r.target
         r.state = "retired"
