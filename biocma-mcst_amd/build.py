@@ -56,6 +56,14 @@ def build(verbose: bool = True) -> str:
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stderr)
         os.replace(out, LIB)
+    # the C++ host driver example over the C ABI (host code only: g++)
+    ex_src = os.path.join(HERE, "examples", "ring_driver.cpp")
+    ex_out = os.path.join(HERE, "examples", "ring_driver")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"), ex_src,
+           "-o", ex_out, "-L", HERE, "-lmcst", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("example driver failed to build:\n" + r.stderr)
     if verbose:
         print(f"built {LIB}")
     return LIB

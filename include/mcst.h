@@ -84,7 +84,8 @@ const char* mcst_last_error(void);
 /* Model registry: the reference's ModelType implementations
  * (apps/libs/models/public/models/{monod,fixed_length,simple_acetate}.hpp)
  * compiled for gfx950, plus the mass-based models of the benchmark cases.
- * Names: "monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate". */
+ * Names: "monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate",
+ * "quad_monod". */
 int mcst_model_lookup(const char* name, int* model_id, int* n_var, int* n_species, int* n_params);
 int mcst_model_default_params(int model_id, double* params, int n);
 
@@ -118,7 +119,6 @@ int mcst_run_set_concentrations(mcst_engine* e, int32_t run, const double* C);
  * (particles_container.hpp:539-557; BIOMC_MC_REMOVE_RATIO_THRESHOLD /
  * BIOMC_MC_MINIMUM_REMOVAL). */
 int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min_removal);
-/* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
 /* Gas-liquid mass transfer with a fixed gas phase (the liquid side of
  * ScalarSimulation::performStepGL, apps/libs/simulation/src/implScalar.cpp:228-248,
  * with mtr = kla*(Henry*C_gas - C)*V as in mass_transfer.cpp:140-157): the
@@ -137,6 +137,7 @@ int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, co
  * statistics do not depend on it. */
 int mcst_run_set_sort_interval(mcst_engine* e, int32_t run, uint32_t steps);
 int mcst_run_sort_particles(mcst_engine* e, int32_t run);
+/* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
 /* Keep up to `cap` exit ages and division ages (the reference's probes). */
 int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap);
