@@ -1089,19 +1089,57 @@ int mcst_run_set_step_target(mcst_engine* e, int32_t run, uint32_t steps) {
 static void order_by_compartment(HostRun& h, uint32_t nv) {
   const uint64_t n = h.n;
   if (n < 2 || h.n_c < 2) return;
-  std::vector<uint64_t> next(h.n_c + 1, 0);
-  for (uint64_t p = 0; p < n; ++p) ++next[h.pos[p] + 1];
-  for (uint32_t c = 0; c < h.n_c; ++c) next[c + 1] += next[c];
+  const uint32_t n_c = h.n_c;
+  // a stable counting sort in T chunks: per-chunk histograms, offsets in
+  // (compartment, chunk) order, then each chunk places its particles in order
+  const unsigned T = n < 1000000 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint64_t chunk = (n + T - 1) / T;
+  std::vector<uint64_t> off((size_t)T * n_c, 0);
   std::vector<uint64_t> dst(n);
-  for (uint64_t p = 0; p < n; ++p) dst[p] = next[h.pos[p]]++;
+  auto par = [&](auto&& fn) {
+    if (T == 1) {
+      fn(0u);
+      return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t) th.emplace_back(fn, t);
+    for (auto& x : th) x.join();
+  };
+  par([&](unsigned t) {
+    const uint64_t a = t * chunk, z = std::min<uint64_t>(n, a + chunk);
+    uint64_t* hist = off.data() + (size_t)t * n_c;
+    for (uint64_t p = a; p < z; ++p) ++hist[h.pos[p]];
+  });
+  uint64_t run = 0;
+  for (uint32_t c = 0; c < n_c; ++c)
+    for (unsigned t = 0; t < T; ++t) {
+      uint64_t& o = off[(size_t)t * n_c + c];
+      const uint64_t k = o;
+      o = run;
+      run += k;
+    }
+  par([&](unsigned t) {
+    const uint64_t a = t * chunk, z = std::min<uint64_t>(n, a + chunk);
+    uint64_t* next = off.data() + (size_t)t * n_c;
+    for (uint64_t p = a; p < z; ++p) dst[p] = next[h.pos[p]]++;
+  });
   std::vector<uint32_t> pos(n);
-  for (uint64_t p = 0; p < n; ++p) pos[dst[p]] = h.pos[p];
+  par([&](unsigned t) {
+    const uint64_t a = t * chunk, z = std::min<uint64_t>(n, a + chunk);
+    for (uint64_t p = a; p < z; ++p) pos[dst[p]] = h.pos[p];
+  });
   h.pos.swap(pos);
   std::vector<double> x(n);
   for (uint32_t v = 0; v < nv; ++v) {
     double* col = h.state.data() + (size_t)v * n;
-    for (uint64_t p = 0; p < n; ++p) x[dst[p]] = col[p];
-    std::copy(x.begin(), x.end(), col);
+    par([&](unsigned t) {
+      const uint64_t a = t * chunk, z = std::min<uint64_t>(n, a + chunk);
+      for (uint64_t p = a; p < z; ++p) x[dst[p]] = col[p];
+    });
+    par([&](unsigned t) {
+      const uint64_t a = t * chunk, z = std::min<uint64_t>(n, a + chunk);
+      std::copy(x.begin() + (long)a, x.begin() + (long)z, col + a);
+    });
   }
 }
 
