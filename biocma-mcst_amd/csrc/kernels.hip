@@ -519,6 +519,26 @@ __global__ void __launch_bounds__(BLK)
         }
         for (uint32_t i = tid; i < acc_ns * acc_win; i += BLK) s_acc[i] = 0.0;
         __syncthreads();
+      } else if constexpr (WIN) {
+        // the window follows the block's slots: when the group's first
+        // compartment leaves the window's lower half (slots are in
+        // compartment order, newborns at the end of the run in parent
+        // order), flush it and re-centre a quarter ahead
+        if (acc_win < acc_nc) {
+          if (tid == 0) s_w0 = cur.cnt ? A.pos[0] : acc_w0;
+          __syncthreads();
+          const uint32_t c0 = s_w0;
+          __syncthreads();  // every wave has read s_w0
+          if (c0 - acc_w0 >= acc_win / 2) {  // wraps when below the window
+            flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
+            const uint32_t q = acc_win / 4;
+            const uint32_t lo = c0 > q ? c0 - q : 0u;
+            acc_w0 = lo + acc_win <= acc_nc ? lo : acc_nc - acc_win;
+            __syncthreads();  // the flush has read every entry
+            for (uint32_t i = tid; i < acc_ns * acc_win; i += BLK) s_acc[i] = 0.0;
+            __syncthreads();
+          }
+        }
       }
     }
     // ---- the wave's sub-chunk of this group
