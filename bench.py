@@ -175,7 +175,7 @@ def ensemble_measurement(n_runs=512, steps=500, graph=50, seed=11):
             "autotuned": {"block": tuned["block"], "tile_group": tuned["group"], "trial_steps": done}}
 
 
-def lattice_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sort_interval=20):
+def lattice_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sort_interval=0):
     """The 50 000-compartment lattice tank at full size: one run of
     quad_monod (8 FP64 states, division age, hydraulic age), 4 species,
     outlet washout with device compaction, 200 steps of 5 ms, all timed."""
@@ -251,7 +251,7 @@ def main():
                     help="also run the 50 000-compartment lattice case with N particles (e.g. 1e8)")
     ap.add_argument("--served-lattice", type=float, default=0, metavar="N",
                     help="also serve the stream with an N-particle lattice run in the mix")
-    ap.add_argument("--lattice50k-sort", type=int, default=20, metavar="K",
+    ap.add_argument("--lattice50k-sort", type=int, default=0, metavar="K",
                     help="reorder the lattice case's particles by compartment every K steps (0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this many MiB (PMC counter calibration)")
