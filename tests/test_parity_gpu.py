@@ -303,3 +303,30 @@ def test_periodic_reorder_parity(model):
     assert_parity(g, o)
     assert g.counters()["compactions"] > 0 and g.counters()["births"] > 0
     eng.close()
+
+
+def test_lattice50k_batch_of_two_window_runs():
+    """Two lattice runs in one engine (their accumulators do not fit LDS, so
+    the pass keeps a following window per block and switches windows between
+    runs): each matches its oracle run."""
+    import mcst
+    from mcst import cases
+    from oracle.oracle import OracleRun
+
+    eng = mcst.Engine("quad_monod")
+    pairs = []
+    for n, seed in ((120_000, 3), (60_000, 4)):
+        c = cases.lattice50k(n_particles=n, seed=seed)
+        c.capacity = 4 * n
+        g = eng.add_run(c.n_c, c.dt, c.seed)
+        o = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+        for r in (g, o):
+            c.apply(r)
+        pairs.append((g, o))
+    eng.commit()
+    eng.step(16)
+    for g, o in pairs:
+        o.step(16, threads=8)
+        assert_parity(g, o, probes=False)
+        assert g.counters()["births"] > 0
+    eng.close()
