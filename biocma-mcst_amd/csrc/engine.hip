@@ -199,6 +199,10 @@ struct HostRun {
   std::vector<HostFeed> feeds;
   std::vector<double> C0;
   std::vector<double> kla, cstar;  // gas-liquid transfer, [n_s][n_c] (empty: none)
+  std::vector<uint32_t> vf;        // time-dependent feeds (indices into feeds), in feed order
+  std::vector<uint32_t> vx_out;    // outlets whose first leaving feed is time-dependent ...
+  std::vector<uint32_t> vx_feed;   // ... and that feed (index into vf)
+  double* d_vf = nullptr;          // device: this step's flows of vf, then the outlets' thresholds
   double ratio = 0.01;
   uint64_t min_removal = 0;
   uint32_t stage_cap = 64;
@@ -375,9 +379,22 @@ void feed_tables(const HostRun& r, std::vector<double>& F, std::vector<double>& 
   F.assign((size_t)r.n_s * r.n_c, 0.0);
   sink.assign(r.n_c, 0.0);
   for (const auto& fd : r.feeds) {
+    if (fd.kind != MCST_FEED_CONSTANT) continue;  // added by the transport kernel each step
     const double q = fd.f0;
     for (size_t v = 0; v < fd.sp.size(); ++v) F[(size_t)fd.sp[v] * r.n_c + fd.in] += q * fd.conc[v];
     if (fd.out >= 0) sink[(size_t)fd.out] += q;
+  }
+}
+
+// feed_descriptor.cpp:18-28
+double feed_flow(const HostFeed& fd, double t) {
+  switch (fd.kind) {
+    case MCST_FEED_LINEAR:
+      return fd.f0 + t * fd.param;
+    case MCST_FEED_EXPONENTIAL:
+      return fd.f0 + std::exp(fd.param * t);
+    default:
+      return fd.f0;
   }
 }
 
@@ -389,16 +406,14 @@ void exit_thresholds(const HostRun& r, HostMap& mp) {
     const uint32_t o = (uint32_t)fd.out;
     if (seen[o]) continue;
     seen[o] = 1;
-    mp.q_exit[o] = std::exp(-(r.dt * fd.f0) / mp.V[o]);
+    if (fd.kind == MCST_FEED_CONSTANT) mp.q_exit[o] = std::exp(-(r.dt * fd.f0) / mp.V[o]);
+    // a time-dependent first feed: its threshold is written before each step
   }
 }
 
 int check_definable(const mcst_engine* e, const HostRun& h, uint32_t r) {
   if (h.maps.empty()) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no flowmap");
   if (!h.has_particles) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no particles");
-  for (const auto& fd : h.feeds)
-    if (fd.kind != MCST_FEED_CONSTANT)
-      return fail(MCST_ERR_ARG, "time-dependent feeds are not supported on the device path");
   if (h.stage_cap > e->bd.stage_stride)
     return fail(MCST_ERR_ARG, "division cap above the engine's staging rows");
   return MCST_OK;
@@ -574,6 +589,65 @@ int upload_run(mcst_engine* e, uint32_t r) {
   if ((rc = run_upload(h, &d.S, h.restored ? h.r_S.data() : zeros.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.F, F.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.sink, sink.data(), sink.size()))) return rc;
+  h.vf.clear();
+  h.vx_out.clear();
+  h.vx_feed.clear();
+  std::vector<int32_t> vf_of(h.feeds.size(), -1);
+  for (uint32_t k = 0; k < h.feeds.size(); ++k)
+    if (h.feeds[k].kind != MCST_FEED_CONSTANT) {
+      vf_of[k] = (int32_t)h.vf.size();
+      h.vf.push_back(k);
+    }
+  if (!h.vf.empty()) {
+    // sources and sinks of the time-dependent feeds by compartment, in feed order
+    std::vector<uint32_t> sp_ptr(h.n_c + 1, 0), sk_ptr(h.n_c + 1, 0);
+    for (uint32_t q = 0; q < h.vf.size(); ++q) {
+      const HostFeed& fd = h.feeds[h.vf[q]];
+      sp_ptr[fd.in + 1] += (uint32_t)fd.sp.size();
+      if (fd.out >= 0) sk_ptr[(size_t)fd.out + 1] += 1;
+    }
+    for (uint32_t j = 0; j < h.n_c; ++j) sp_ptr[j + 1] += sp_ptr[j], sk_ptr[j + 1] += sk_ptr[j];
+    std::vector<uint32_t> src_k(sp_ptr[h.n_c]), src_s(sp_ptr[h.n_c]), sink_k(sk_ptr[h.n_c]);
+    std::vector<double> src_c(sp_ptr[h.n_c]);
+    std::vector<uint32_t> fill_s(sp_ptr.begin(), sp_ptr.end() - 1), fill_k(sk_ptr.begin(), sk_ptr.end() - 1);
+    for (uint32_t q = 0; q < h.vf.size(); ++q) {
+      const HostFeed& fd = h.feeds[h.vf[q]];
+      for (size_t v = 0; v < fd.sp.size(); ++v) {
+        const uint32_t at = fill_s[fd.in]++;
+        src_k[at] = q, src_s[at] = fd.sp[v], src_c[at] = fd.conc[v];
+      }
+      if (fd.out >= 0) sink_k[fill_k[(size_t)fd.out]++] = q;
+    }
+    // outlets whose first leaving feed is time-dependent take this step's threshold
+    std::vector<int32_t> vx_slot(h.n_c, -1);
+    std::vector<char> seen(h.n_c, 0);
+    for (uint32_t k = 0; k < h.feeds.size(); ++k) {
+      const HostFeed& fd = h.feeds[k];
+      if (fd.out < 0 || seen[(size_t)fd.out]) continue;
+      seen[(size_t)fd.out] = 1;
+      if (vf_of[k] >= 0) {
+        vx_slot[(size_t)fd.out] = (int32_t)h.vx_out.size();
+        h.vx_out.push_back((uint32_t)fd.out);
+        h.vx_feed.push_back((uint32_t)vf_of[k]);
+      }
+    }
+    auto up_u32 = [&](const uint32_t** dst, std::vector<uint32_t> v) {
+      if (v.empty()) v.push_back(0u);  // keep every pointer valid
+      return run_upload(h, (uint32_t**)dst, v.data(), v.size());
+    };
+    if (src_c.empty()) src_c.push_back(0.0);
+    std::vector<double> zero_vf(h.vf.size() + h.vx_out.size(), 0.0);
+    if ((rc = run_upload(h, &h.d_vf, zero_vf.data(), zero_vf.size()))) return rc;
+    d.vf_q = h.d_vf;
+    if ((rc = up_u32(&d.vf_src_ptr, sp_ptr))) return rc;
+    if ((rc = up_u32(&d.vf_src_k, src_k))) return rc;
+    if ((rc = up_u32(&d.vf_src_s, src_s))) return rc;
+    if ((rc = run_upload(h, (double**)&d.vf_src_c, src_c.data(), src_c.size()))) return rc;
+    if ((rc = up_u32(&d.vf_sink_ptr, sk_ptr))) return rc;
+    if ((rc = up_u32(&d.vf_sink_k, sink_k))) return rc;
+    if ((rc = run_upload(h, (int32_t**)&d.vx_slot, vx_slot.data(), vx_slot.size()))) return rc;
+    d.n_vf = (uint32_t)h.vf.size();
+  }
   if (!h.kla.empty()) {
     if ((rc = run_upload(h, (double**)&d.kla, h.kla.data(), nsc))) return rc;
     if ((rc = run_upload(h, (double**)&d.cstar, h.cstar.data(), nsc))) return rc;
@@ -853,8 +927,41 @@ void count_steps(mcst_engine* e, uint64_t k) {
     if (h.phase == 1) h.host_step = (uint32_t)std::min<uint64_t>(h.step_target, h.host_step + k);
 }
 
+// Flows of the time-dependent feeds for the step each live run is about to
+// take (t = step * dt, as update_feed), and the exit thresholds of the
+// outlets they lead, computed on the host like the constant feeds'.
+int prepare_feeds(mcst_engine* e) {
+  bool synced = false;
+  std::vector<double> v;
+  for (auto& h : e->runs) {
+    if (h.phase != 1 || h.vf.empty() || h.host_step >= h.step_target) continue;
+    if (!synced) {  // the previous step's transport has read the last values
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      synced = true;
+    }
+    const double t = (double)h.host_step * h.dt;
+    v.assign(h.vf.size() + h.vx_out.size(), 0.0);
+    for (size_t q = 0; q < h.vf.size(); ++q) v[q] = feed_flow(h.feeds[h.vf[q]], t);
+    const HostMap& mp = h.maps[(h.host_step / h.steps_per_map) % h.maps.size()];
+    for (size_t x = 0; x < h.vx_out.size(); ++x) {
+      const uint32_t o = h.vx_out[x];
+      v[h.vf.size() + x] = std::exp(-(h.dt * v[h.vx_feed[x]]) / mp.V[o]);
+    }
+    HIP_TRY(hipMemcpy(h.d_vf, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return MCST_OK;
+}
+
+bool has_time_feeds(const mcst_engine* e) {
+  for (const auto& h : e->runs)
+    if (h.phase == 1 && !h.vf.empty()) return true;
+  return false;
+}
+
 // One step of the batch, then the reorders that fall due.
 int step_once(mcst_engine* e, uint32_t stages) {
+  const int rcf = prepare_feeds(e);
+  if (rcf) return rcf;
   std::vector<uint32_t> before(e->runs.size());
   for (size_t r = 0; r < e->runs.size(); ++r) before[r] = e->runs[r].host_step;
   launch_step(e, stages);
@@ -1278,6 +1385,8 @@ int mcst_engine_capture(mcst_engine* e, uint32_t n_steps) {
   CHECK_ENGINE(e);
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
   if (n_steps == 0) return fail(MCST_ERR_ARG, "capture at least one step");
+  if (has_time_feeds(e))
+    return fail(MCST_ERR_STATE, "a run has time-dependent feeds: their flows are set by the host before each step");
   HIP_TRY(hipSetDevice(e->device));
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (e->graph_exec) HIP_TRY(hipGraphExecDestroy(e->graph_exec));
@@ -1366,6 +1475,11 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   const uint32_t nv = (uint32_t)e->model->nv;
   for (uint32_t s = 0; s < n_steps && e->bd.R; ++s) {
     hipEvent_t* x = &ev[(size_t)s * (K + 1)];
+    if (has_time_feeds(e)) {
+      const int rc = prepare_feeds(e);
+      if (rc) return rc;
+      count_steps(e, 1);  // the next step's flows need the count of this one
+    }
     BatchDev b = e->bd;
     HIP_TRY(hipEventRecord(x[0], e->stream));
     launch_transport(e->model->id, b, e->grid_transport, e->stream);
@@ -1383,7 +1497,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   }
   HIP_TRY(hipEventRecord(t1, e->stream));
   HIP_TRY(hipGetLastError());
-  count_steps(e, n_steps);
+  if (!has_time_feeds(e)) count_steps(e, n_steps);
   HIP_TRY(hipEventSynchronize(t1));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, t0, t1));

@@ -159,7 +159,9 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
     const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
     double* rec = rd.rec + (size_t)j * rd.rec_w;
     rec[NS] = mp.p_move[j];
-    rec[NS + 1] = mp.q_exit[j];
+    double qx = mp.q_exit[j];
+    if (rd.n_vf && rd.vx_slot[j] >= 0) qx = rd.vf_q[rd.n_vf + (uint32_t)rd.vx_slot[j]];
+    rec[NS + 1] = qx;
     const double* c = rd.C[step & 1u];
     double* Cn = rd.C[(step + 1u) & 1u];
     double cn[NS];
@@ -195,11 +197,25 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) acc[s] += cv[k][s] * v[k];
       }
-      const double sink = rd.sink[j], inv_v = mp.inv_v[j];
+      double sink = rd.sink[j];
+      const double inv_v = mp.inv_v[j];
+      double fv[NS];  // time-dependent feeds of this compartment, in feed order
+#pragma unroll
+      for (int s = 0; s < NS; ++s) fv[s] = 0.0;
+      if (rd.n_vf) {
+        for (uint32_t q = rd.vf_sink_ptr[j]; q < rd.vf_sink_ptr[j + 1]; ++q) sink = sink + rd.vf_q[rd.vf_sink_k[q]];
+        for (uint32_t q = rd.vf_src_ptr[j]; q < rd.vf_src_ptr[j + 1]; ++q) {
+          const double add = rd.vf_q[rd.vf_src_k[q]] * rd.vf_src_c[q];
+          const uint32_t sp = rd.vf_src_s[q];
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+            if ((uint32_t)s == sp) fv[s] += add;
+        }
+      }
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         const size_t o = (size_t)s * n_c + j;
-        double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + rd.F[o]);
+        double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]));
         // gas-liquid transfer from a fixed gas phase (performStepGL,
         // implScalar.cpp:228-248): kla*(C* - C)*V
         if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];

@@ -86,6 +86,19 @@ struct RunDev {
   const double* sink;       // [n_c]      outlet flow per compartment
   const double* kla;        // [n_s][n_c] gas-liquid transfer coefficient (null: no transfer)
   const double* cstar;      // [n_s][n_c] saturation concentration set by the gas phase
+  // time-dependent (linear, exponential) feeds: their flows this step, then
+  // the exit thresholds of the outlets they lead (written by the host before
+  // each step); sources and sinks by compartment (CSR); vx_slot[j] >= 0
+  // selects the threshold of outlet j from vf_q[n_vf + slot]
+  const double* vf_q;
+  const uint32_t* vf_src_ptr;  // [n_c+1]
+  const uint32_t* vf_src_k;    // feed (index into vf_q) of each source entry
+  const uint32_t* vf_src_s;    // its species
+  const double* vf_src_c;      // its concentration
+  const uint32_t* vf_sink_ptr; // [n_c+1]
+  const uint32_t* vf_sink_k;
+  const int32_t* vx_slot;      // [n_c]
+  uint32_t n_vf;
   double* rec;              // [n_c][rec] per-compartment record read by the particle pass
   double* probe_exit;       // exit ages (residence times), may be null
   double* probe_div;        // division ages, may be null

@@ -110,7 +110,11 @@ int mcst_run_add_flowmap(mcst_engine* e, int32_t run, const double* volumes, uin
 int mcst_run_set_steps_per_flowmap(mcst_engine* e, int32_t run, uint32_t steps);
 /* A feed (FeedDescriptor, apps/libs/simulation/public/simulation/feed_descriptor.hpp:11-161):
  * flow(t) = f0 (constant), f0 + t*param (linear), f0 + exp(param*t) (exponential);
- * adds flow*conc[v] of species[v] at `input`; output >= 0 is a leaving flow. */
+ * adds flow*conc[v] of species[v] at `input`; output >= 0 is a leaving flow.
+ * A linear or exponential feed takes t = step*dt at the start of each step
+ * (update_feed, simulation.model.cpp:72-122): its flow, and the exit
+ * threshold of an outlet it is the first feed of, are set by the host before
+ * each step, so an engine holding such a run cannot capture a graph. */
 int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
                       int64_t output, uint32_t n_values, const uint32_t* species, const double* conc);
 /* Initial concentrations [n_species][n_c] (kg/m3); liquid mass = C * V. */
@@ -180,7 +184,8 @@ int mcst_engine_retire(mcst_engine* e, int32_t run);
 int mcst_engine_step(mcst_engine* e, uint32_t n_steps);
 /* One step restricted to the MCST_STAGE_* bits in `stages`. */
 int mcst_engine_step_stages(mcst_engine* e, uint32_t stages);
-/* Capture `n_steps` steps into a HIP graph (replaces any earlier one). */
+/* Capture `n_steps` steps into a HIP graph (replaces any earlier one).
+ * MCST_ERR_STATE while a live run has a time-dependent feed. */
 int mcst_engine_capture(mcst_engine* e, uint32_t n_steps);
 /* Replay the captured graph `times` times (times * n_steps steps). */
 int mcst_engine_replay(mcst_engine* e, uint32_t times);

@@ -324,6 +324,31 @@ def test_gas_liquid_transfer_closed_form(oracle_lib):
         assert C[0, 0] == 1.5
 
 
+def test_time_dependent_feed_closed_form(oracle_lib):
+    """One box, no particles, no outlet: a feed of flow q(t) at concentration
+    c adds dt*q(k*dt)*c of mass at step k (t is the step's start), so
+    after n steps the mass grew by c*dt*sum_k q(k*dt): for a linear feed
+    c*p*dt^2*n(n-1)/2, for an exponential one c*dt*(n*f0 + (e^{a n dt}-1)/(e^{a dt}-1))."""
+    dt, V, c_in = 0.01, 2.0, 4.0
+    for kind, f0, a in (("linear", 0.0, 0.3), ("exponential", 0.5, -2.0)):
+        r = one_box(oracle_lib, "monod_mass", V=V, dt=dt)
+        r.set_concentrations(np.array([[0.25]]))
+        r.add_feed(kind, f0, a, 0, None, [0], [c_in])
+        r.init_particles(0, 0, 1.0)
+        m0 = r.mass()[0, 0]
+        r.step_stages(TRANSPORT)
+        if kind == "linear":
+            assert r.mass()[0, 0] == m0  # q(0) = 0: the first step adds nothing
+        for n in range(2, 41):
+            r.step_stages(TRANSPORT)
+            if kind == "linear":
+                want = c_in * a * dt * dt * n * (n - 1) / 2
+            else:
+                want = c_in * dt * (n * f0 + math.expm1(a * n * dt) / math.expm1(a * dt))
+            np.testing.assert_allclose(r.mass()[0, 0] - m0, want, rtol=1e-11)
+            np.testing.assert_allclose(r.concentrations()[0, 0], r.mass()[0, 0] / V, rtol=1e-14)
+
+
 def test_reorder_is_a_stable_sort_by_compartment(oracle_lib):
     """sort_particles permutes whole particles into compartment order, slot
     order among equal positions; an interval k sorts after every k-th step."""

@@ -261,6 +261,36 @@ def test_gas_liquid_transfer_parity():
     eng.close()
 
 
+def test_time_dependent_feeds_parity():
+    """Linear and exponential feeds (flows set per step from t = step*dt):
+    the first feed of an outlet (its exit threshold follows the flow), a
+    second feed of a constant feed's outlet and feeds without an outlet,
+    against the oracle over 36 steps.  Capturing a graph is refused
+    while such a run is on the device."""
+    import mcst
+    from mcst import cases
+
+    c = cases.small_case("dual_monod", n_c=12, n_particles=6000, seed=21)
+    c.capacity = 60000
+    c.feeds = [
+        ("constant", 2e-3, 0.0, 0, 6, [0], [10.0]),
+        ("linear", 1e-3, 0.02, 3, 9, [0, 1], [8.0, 0.5]),    # first feed of outlet 9
+        ("linear", 1e-3, 0.01, 5, 6, [1], [2.0]),           # second feed of outlet 6
+        ("linear", 5e-4, -1e-3, 2, None, [0], [3.0]),       # no outlet
+        ("exponential", 0.0, -3.0, 10, None, [1], [1e-4]),  # flow e^{-3t} m3/s, no outlet
+    ]
+    eng, g, o = make_pair(c)
+    with pytest.raises(mcst.MCSTError):
+        eng.capture(2)
+    for _ in range(6):
+        eng.step(6)
+        o.step(6)
+        assert_parity(g, o)
+    cnt = g.counters()
+    assert cnt["exits"] > 0 and cnt["births"] > 0
+    eng.close()
+
+
 def test_lattice50k_reduced():
     """The 50 000-compartment lattice tank (4 species, fixed-gas oxygen
     supply, 2500 bottom-to-top feeds, quad_monod) at 2e5 particles: parity
