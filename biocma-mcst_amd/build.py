@@ -1,6 +1,12 @@
-"""Build libmcst.so (the HIP kernels and the C ABI) in-tree for gfx950.
+"""Build libmcst.so (the HIP kernels and the C ABI) in-tree for gfx950, and
+model plugins for it.
 
     python biocma-mcst_amd/build.py
+    python biocma-mcst_amd/build.py --udf model.hpp --model Struct -o plugin.so
+
+The first builds the library, the C++ example driver and the example model
+plugin (examples/libudf_minimal.so); the second builds a plugin from a model
+struct (the contract of csrc/models.hpp), for mcst_model_load.
 
 Flags: -ffp-contract=off keeps every FP64 expression rounded as written (the
 CPU oracle uses the same rule), packed FP32 instructions are disabled, and
@@ -8,7 +14,9 @@ CPU oracle uses the same rule), packed FP32 instructions are disabled, and
 """
 from __future__ import annotations
 
+import argparse
 import os
+import shlex
 import subprocess
 import sys
 import tempfile
@@ -33,16 +41,38 @@ FLAGS = [
 ]
 
 
-def _compile(src: str, obj: str) -> None:
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(SRC, src), "-o", obj]
+def _run_hipcc(cmd: list, what: str) -> None:
     r = subprocess.run(cmd, capture_output=True, text=True)
     # the host pass warns that it does not know the packed-fp32 feature; harmless
     noise = [ln for ln in r.stderr.splitlines() if ln.strip() and "packed-fp32-ops" not in ln
              and "is not a recognized feature" not in ln]
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n" + "\n".join(noise))
+        raise RuntimeError(f"hipcc failed for {what}:\n" + "\n".join(noise))
     if noise:
         sys.stderr.write("\n".join(noise) + "\n")
+
+
+def _compile(src: str, obj: str) -> None:
+    _run_hipcc([HIPCC, *FLAGS, "-c", os.path.join(SRC, src), "-o", obj], src)
+
+
+def build_udf(header: str, model: str, out: str, verbose: bool = True) -> str:
+    """A model plugin: csrc/udf_plugin.hip instantiated for the struct `model`
+    defined in `header`, linked into the shared library `out`."""
+    header = os.path.abspath(header)
+    if not os.path.isfile(header):
+        raise FileNotFoundError(header)
+    if not model.replace("::", "_").replace("_", "a").isalnum():
+        raise ValueError(f"not a C++ type name: {model!r}")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-shared", "-fvisibility=hidden", "-I", SRC, "-I", os.path.dirname(header),
+           f"-DMCST_UDF_HEADER=\"{header}\"", f"-DMCST_UDF_MODEL={model}",
+           os.path.join(SRC, "udf_plugin.hip"), "-o", tmp]
+    _run_hipcc(cmd, f"the plugin of {model} ({shlex.quote(header)})")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 def build(verbose: bool = True) -> str:
@@ -51,7 +81,7 @@ def build(verbose: bool = True) -> str:
         with ThreadPoolExecutor(len(SOURCES)) as ex:
             list(ex.map(lambda a: _compile(*a), zip(SOURCES, objs)))
         out = LIB + ".tmp"
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs, "-lpthread"]
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs, "-lpthread", "-ldl"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stderr)
@@ -64,10 +94,23 @@ def build(verbose: bool = True) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("example driver failed to build:\n" + r.stderr)
+    # the example model plugin
+    build_udf(os.path.join(HERE, "examples", "udf_minimal.hpp"), "UdfMinimal",
+              os.path.join(HERE, "examples", "libudf_minimal.so"), verbose=False)
     if verbose:
         print(f"built {LIB}")
     return LIB
 
 
 if __name__ == "__main__":
-    build()
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--udf", help="header that defines a model struct: build a plugin of it")
+    ap.add_argument("--model", help="the model struct's name (with --udf)")
+    ap.add_argument("-o", "--output", help="the plugin's file (with --udf)")
+    a = ap.parse_args()
+    if a.udf:
+        if not a.model or not a.output:
+            ap.error("--udf needs --model and -o")
+        build_udf(a.udf, a.model, a.output)
+    else:
+        build()

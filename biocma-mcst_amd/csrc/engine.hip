@@ -6,6 +6,7 @@
 // one batch of runs (apps/libs/simulation/public/simulation/simulation.hpp:44-239,
 // apps/libs/simulation/public/simulation/kernels/kernels.hpp:25-224).  All
 // state stays on the device between steps; a step needs no host round trip.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,7 +14,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -23,6 +26,7 @@
 #include "../../include/mcst.h"
 #include "kernels.hpp"
 #include "models.hpp"
+#include "udf.hpp"
 
 namespace mcst {
 namespace {
@@ -48,6 +52,7 @@ struct ModelEntry {
   void (*defaults)(double*);
   void (*init)(const double*, double*, uint64_t, uint32_t);
   double (*mass)(const double*, const double*);
+  const ModelOps* ops;
 };
 
 template <class M>
@@ -57,7 +62,8 @@ double host_mass(const double* p, const double* x) {
 
 template <class M>
 ModelEntry entry() {
-  return ModelEntry{M::NAME, M::ID, M::NV, M::NS, M::NP, M::NL, M::CAN_DIE, &M::defaults, &M::init, &host_mass<M>};
+  return ModelEntry{M::NAME,  M::ID,    M::NV,          M::NS,   M::NP, M::NL, M::CAN_DIE, &M::defaults,
+                    &M::init, &host_mass<M>, builtin_model_ops(M::ID)};
 }
 
 const std::vector<ModelEntry>& registry() {
@@ -66,17 +72,126 @@ const std::vector<ModelEntry>& registry() {
   return r;
 }
 
+// Models loaded from plugins (mcst_model_load), numbered from UDF_ID_BASE in
+// load order.  Entries are never removed and plugins never unloaded, so
+// pointers to entries stay valid for the life of the process.
+constexpr int UDF_ID_BASE = 100;
+constexpr const char* UDF_ALIAS = "udf_model";  // the reference's name for the plugin model
+
+struct UdfRegistry {
+  std::mutex mu;
+  std::deque<ModelEntry> entries;
+  std::deque<ModelOps> ops;
+  std::deque<std::string> names, paths;
+  const ModelEntry* alias = nullptr;  // the plugin named by BIOMC_LIB_UDF
+};
+
+UdfRegistry& udf_registry() {
+  static UdfRegistry r;
+  return r;
+}
+
+// Loads the plugin at `path` (registry lock held).  Loading the same path
+// again returns the entry of the first load.
+int load_plugin(UdfRegistry& u, const char* path, const ModelEntry** out) {
+  if (!path || !*path) return fail(MCST_ERR_ARG, "empty plugin path");
+  for (size_t k = 0; k < u.paths.size(); ++k)
+    if (u.paths[k] == path) {
+      *out = &u.entries[k];
+      return MCST_OK;
+    }
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) {
+    const char* why = dlerror();
+    return fail(MCST_ERR_ARG, std::string("cannot load model plugin: ") + (why ? why : path));
+  }
+  auto describe = (mcst_udf_describe_fn)dlsym(h, "mcst_udf_describe");
+  UdfModel d{};
+  if (!describe || describe(&d, UDF_ABI_VERSION) != 0) {
+    dlclose(h);
+    return fail(MCST_ERR_ARG, std::string(path) + " is not a model plugin of this library version");
+  }
+  UdfModel ref{};
+  udf_sizes(ref);
+  if (d.batch_bytes != ref.batch_bytes || d.run_bytes != ref.run_bytes || d.map_bytes != ref.map_bytes ||
+      d.state_bytes != ref.state_bytes) {
+    dlclose(h);
+    return fail(MCST_ERR_ARG, std::string(path) + " was built against other device structures: rebuild it");
+  }
+  std::string why;
+  if (!d.name || !*d.name) why = "has no name";
+  else if (d.nv < 1 || d.nv > MAX_VAR || d.ns < 1 || d.ns > MAX_SPECIES || d.np < 0 || d.np > MAX_PARAMS ||
+           (d.nl != 2 && d.nl != 4))
+    why = "has sizes out of range";
+  else if (!d.defaults || !d.init || !d.mass || !d.ops.transport || !d.ops.particle || !d.ops.particle_kernel ||
+           !d.ops.resolve || !d.ops.properties)
+    why = "misses a function";
+  else if (std::strcmp(d.name, UDF_ALIAS) == 0)
+    why = std::string("may not be named ") + UDF_ALIAS;
+  else {
+    for (const auto& m : registry())
+      if (std::strcmp(m.name, d.name) == 0) why = std::string("has the name of a built-in model (") + d.name + ")";
+    for (const auto& n : u.names)
+      if (n == d.name) why = std::string("has the name of a model already loaded (") + d.name + ")";
+  }
+  if (!why.empty()) {
+    dlclose(h);
+    return fail(MCST_ERR_ARG, std::string("model plugin ") + path + " " + why);
+  }
+  u.names.emplace_back(d.name);
+  u.paths.emplace_back(path);
+  u.ops.push_back(d.ops);
+  u.entries.push_back(ModelEntry{u.names.back().c_str(), UDF_ID_BASE + (int)u.entries.size(), d.nv, d.ns, d.np,
+                                 d.nl, d.can_die, d.defaults, d.init, d.mass, &u.ops.back()});
+  *out = &u.entries.back();
+  return MCST_OK;
+}
+
+// The model of a name: built-in, loaded, or "udf_model" (the plugin named by
+// the environment variable BIOMC_LIB_UDF, loaded on first use).  Sets the
+// error message when there is none.
 const ModelEntry* find_model(const char* name) {
-  if (!name) return nullptr;
+  if (!name) {
+    fail(MCST_ERR_ARG, "null model name");
+    return nullptr;
+  }
   for (const auto& m : registry())
     if (std::strcmp(m.name, name) == 0) return &m;
+  UdfRegistry& u = udf_registry();
+  std::lock_guard<std::mutex> lock(u.mu);
+  for (const auto& m : u.entries)
+    if (std::strcmp(m.name, name) == 0) return &m;
+  if (std::strcmp(name, UDF_ALIAS) == 0) {
+    if (u.alias) return u.alias;
+    const char* path = std::getenv("BIOMC_LIB_UDF");
+    if (!path || !*path) {
+      fail(MCST_ERR_ARG, "udf_model: BIOMC_LIB_UDF names no model plugin");
+      return nullptr;
+    }
+    const ModelEntry* m = nullptr;
+    if (load_plugin(u, path, &m) != MCST_OK) return nullptr;
+    u.alias = m;
+    return m;
+  }
+  fail(MCST_ERR_ARG, std::string("unknown model: ") + name);
   return nullptr;
 }
 
 const ModelEntry* find_model(int id) {
   for (const auto& m : registry())
     if (m.id == id) return &m;
+  UdfRegistry& u = udf_registry();
+  std::lock_guard<std::mutex> lock(u.mu);
+  if (id >= UDF_ID_BASE && (size_t)(id - UDF_ID_BASE) < u.entries.size()) return &u.entries[id - UDF_ID_BASE];
   return nullptr;
+}
+
+// FNV-1a of the model name: what a checkpoint records of its model (loaded
+// models are numbered in load order, so their ids vary between processes).
+uint32_t model_tag(const char* name) {
+  uint32_t h = 2166136261u;
+  for (const char* c = name; *c; ++c) h = (h ^ (uint8_t)*c) * 16777619u;
+  return h;
 }
 
 // ------------------------------------------------------------ flowmap tables
@@ -454,7 +569,7 @@ int choose_launch(mcst_engine* e) {
   for (int blk : {256, 512, 1024}) {
     if (!priv && blk != 256) continue;
     if (forced_block && blk != forced_block) continue;
-    const void* fn = particle_kernel_ptr(M.id, blk, priv, false, priv && windowed);
+    const void* fn = M.ops->particle_kernel(blk, priv, false, priv && windowed);
     if (!fn) continue;
     const size_t lds = priv ? lds_need : 0;
     if (lds > 64 * 1024)
@@ -481,13 +596,13 @@ int choose_launch(mcst_engine* e) {
     best = ParticleLaunch{};
     b.acc_w = 0;
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, particle_kernel_ptr(M.id, 256, false, false, false), 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, M.ops->particle_kernel(256, false, false, false), 256, 0));
     best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
   }
   best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, b.T_max));
   best.probe = probe;
   if (best.probe && best.lds > 64 * 1024)
-    (void)hipFuncSetAttribute(particle_kernel_ptr(M.id, 256, best.priv, true, best.win),
+    (void)hipFuncSetAttribute(M.ops->particle_kernel(256, best.priv, true, best.win),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)best.lds);
   // one run: blocks take groups in turn, which keeps the wave front of the
   // whole grid inside a narrow window of the particle arrays; a batch: each
@@ -856,9 +971,10 @@ void launch_step(mcst_engine* e, uint32_t stages) {
   BatchDev b = e->bd;
   b.stages = stages;
   const uint32_t nv = (uint32_t)e->model->nv;
-  launch_transport(e->model->id, b, e->grid_transport, e->stream);
-  launch_particle(e->model->id, b, e->plaunch, e->stream);
-  launch_resolve(e->model->id, b, e->grid_move, e->stream);
+  const ModelOps& ops = *e->model->ops;
+  ops.transport(b, e->grid_transport, e->stream);
+  ops.particle(b, e->plaunch, e->stream);
+  ops.resolve(b, e->grid_move, e->stream);
   launch_bookkeep(b, e->stream);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   launch_merge(b, nv, e->grid_particle, e->stream);
@@ -1007,9 +1123,19 @@ int mcst_version(int* major, int* minor, int* patch) {
 
 const char* mcst_last_error(void) { return g_err.c_str(); }
 
+int mcst_model_load(const char* path, int* model_id) {
+  UdfRegistry& u = udf_registry();
+  std::lock_guard<std::mutex> lock(u.mu);
+  const ModelEntry* m = nullptr;
+  const int rc = load_plugin(u, path, &m);
+  if (rc) return rc;
+  if (model_id) *model_id = m->id;
+  return MCST_OK;
+}
+
 int mcst_model_lookup(const char* name, int* model_id, int* n_var, int* n_species, int* n_params) {
   const ModelEntry* m = find_model(name);
-  if (!m) return fail(MCST_ERR_ARG, std::string("unknown model: ") + (name ? name : "(null)"));
+  if (!m) return MCST_ERR_ARG;  // find_model set the message
   if (model_id) *model_id = m->id;
   if (n_var) *n_var = m->nv;
   if (n_species) *n_species = m->ns;
@@ -1030,7 +1156,7 @@ int mcst_engine_create(const char* model_name, int device, mcst_engine** out) {
   if (!out) return fail(MCST_ERR_ARG, "null output handle");
   *out = nullptr;
   const ModelEntry* m = find_model(model_name);
-  if (!m) return fail(MCST_ERR_ARG, std::string("unknown model: ") + (model_name ? model_name : "(null)"));
+  if (!m) return MCST_ERR_ARG;  // find_model set the message
   int count = 0;
   HIP_TRY(hipGetDeviceCount(&count));
   if (device < 0 || device >= count) return fail(MCST_ERR_DEVICE, "no such HIP device");
@@ -1482,11 +1608,11 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
     }
     BatchDev b = e->bd;
     HIP_TRY(hipEventRecord(x[0], e->stream));
-    launch_transport(e->model->id, b, e->grid_transport, e->stream);
+    e->model->ops->transport(b, e->grid_transport, e->stream);
     HIP_TRY(hipEventRecord(x[1], e->stream));
-    launch_particle(e->model->id, b, e->plaunch, e->stream);
+    e->model->ops->particle(b, e->plaunch, e->stream);
     HIP_TRY(hipEventRecord(x[2], e->stream));
-    launch_resolve(e->model->id, b, e->grid_move, e->stream);
+    e->model->ops->resolve(b, e->grid_move, e->stream);
     HIP_TRY(hipEventRecord(x[3], e->stream));
     launch_bookkeep(b, e->stream);
     HIP_TRY(hipEventRecord(x[4], e->stream));
@@ -1634,7 +1760,7 @@ int mcst_run_get_properties(mcst_engine* e, int32_t run, double* values, double*
   double* ds = (double*)(buf + vb);
   hipError_t err = hipMemsetAsync(ds, 0, sb, e->stream);
   if (err == hipSuccess && n) {
-    launch_properties(e->model->id, e->bd, (uint32_t)run, n, dv, ds, e->stream);
+    e->model->ops->properties(e->bd, (uint32_t)run, n, dv, ds, e->stream);
     err = hipGetLastError();
   }
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
@@ -1672,6 +1798,7 @@ int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_
 // recorded exit and division ages.  It holds everything a step carries to
 // the next (random draws are keyed by step, so none is stored); the network,
 // feeds and model parameters are the run's definition and are not stored.
+// The header's `model` is the model name's tag (model_tag).
 namespace {
 struct CkptHeader {
   char magic[8];
@@ -1681,7 +1808,7 @@ struct CkptHeader {
   unsigned long long counters[C_COUNT];
 };
 constexpr char CKPT_MAGIC[8] = {'M', 'C', 'S', 'T', 'C', 'K', 'P', 'T'};
-constexpr uint32_t CKPT_VERSION = 1;
+constexpr uint32_t CKPT_VERSION = 2;
 
 uint64_t ckpt_bytes(uint64_t n, uint32_t nv, uint64_t nsc, uint64_t n_exit, uint64_t n_div) {
   return sizeof(CkptHeader) + 4 * n + ((n + 7) / 8) * 8 + 8 * ((uint64_t)nv * n + 2 * n + 3 * nsc + n_exit + n_div);
@@ -1705,7 +1832,7 @@ int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, u
   CkptHeader hd{};
   std::memcpy(hd.magic, CKPT_MAGIC, 8);
   hd.version = CKPT_VERSION;
-  hd.model = (uint32_t)e->model->id;
+  hd.model = model_tag(e->model->name);
   hd.n_c = d.n_c;
   hd.n_s = d.n_s;
   hd.nv = nv;
@@ -1751,7 +1878,7 @@ int mcst_run_restore(mcst_engine* e, int32_t run, const void* buf, uint64_t size
   if (std::memcmp(hd.magic, CKPT_MAGIC, 8) != 0 || hd.version != CKPT_VERSION)
     return fail(MCST_ERR_ARG, "not a checkpoint of this library version");
   const uint32_t nv = (uint32_t)e->model->nv;
-  if (hd.model != (uint32_t)e->model->id || hd.nv != nv) return fail(MCST_ERR_ARG, "checkpoint of another model");
+  if (hd.model != model_tag(e->model->name) || hd.nv != nv) return fail(MCST_ERR_ARG, "checkpoint of another model");
   if (hd.n_c != h->n_c || hd.n_s != h->n_s || hd.dt != h->dt || hd.seed != h->seed)
     return fail(MCST_ERR_ARG, "checkpoint of a run with another compartment count, species count, dt or seed");
   const uint64_t n = hd.n, nsc = (uint64_t)hd.n_s * hd.n_c;

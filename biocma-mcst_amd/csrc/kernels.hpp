@@ -1,4 +1,5 @@
-// kernels.hpp -- launchers of the step kernels (defined in kernels.hip).
+// kernels.hpp -- launchers of the step kernels: the model-independent ones
+// (kernels.hip) and the table of a model's own (step_kernels.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,7 +8,6 @@
 
 namespace mcst {
 
-void launch_transport(int model_id, const BatchDev& b, int grid, hipStream_t s);
 // Launch shape of the particle pass: block size (256/512/1024 threads per
 // 1024-slot tile), grid, the LDS bytes of the per-block contribution
 // accumulator (priv) or none (direct global atomics), and whether some run
@@ -21,16 +21,25 @@ struct ParticleLaunch {
   size_t lds = 0;
 };
 
-void launch_particle(int model_id, const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
-const void* particle_kernel_ptr(int model_id, int block, bool priv, bool probe, bool win);
+// A model's kernels: transport (its record() per compartment), the particle
+// pass (particle_kernel: the kernel of a launch shape, for occupancy queries;
+// nullptr when the shape has no variant), resolve (its division()) and the
+// export values (its mass()).  The same table comes from the library for the
+// built-in models and from a plugin for a model loaded at run time.
+struct ModelOps {
+  void (*transport)(const BatchDev& b, int grid, hipStream_t s);
+  void (*particle)(const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
+  const void* (*particle_kernel)(int block, bool priv, bool probe, bool win);
+  void (*resolve)(const BatchDev& b, int grid, hipStream_t s);
+  void (*properties)(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s);
+};
+const ModelOps* builtin_model_ops(int model_id);
+
 void launch_bookkeep(const BatchDev& b, hipStream_t s);
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
-void launch_resolve(int model_id, const BatchDev& b, int grid, hipStream_t s);
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_layout(const BatchDev& b, hipStream_t s);
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s);
-void launch_properties(int model_id, const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial,
-                       hipStream_t s);
 
 // Stable sort of one run's particles by position (sort.hip).  pos, status
 // and the n_f64 FP64 fields (state variables and both ages) are the run's

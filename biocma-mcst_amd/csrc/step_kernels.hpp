@@ -1,0 +1,1048 @@
+// step_kernels.hpp -- the model-dependent kernels of the step (k_transport,
+// k_particle, k_resolve, k_properties) as templates over the model, and the
+// ModelOps table that launches them.  kernels.hip instantiates them for the
+// built-in models; udf_plugin.hip for a model compiled into a plugin and
+// loaded at run time (mcst_model_load).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "models.hpp"
+
+namespace mcst {
+
+namespace {
+
+// Pointers read from the run tables are generic; tell the compiler they point
+// to global memory so it emits global_* rather than flat_* accesses.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T* gp(T* p) {
+  return (GLOBAL_AS T*)p;
+}
+
+// A value every lane of the wave holds identically, moved to scalar registers.
+template <class T>
+__device__ __forceinline__ T uniform(T v) {
+  static_assert(sizeof(T) % 4 == 0, "uniform() works on whole dwords");
+  union {
+    T t;
+    uint32_t w[sizeof(T) / 4];
+  } u;
+  u.t = v;
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) u.w[i] = __builtin_amdgcn_readfirstlane(u.w[i]);
+  return u.t;
+}
+
+// Streaming accesses to the particle arrays: non-temporal, so the particle
+// stream does not evict the flowmap and compartment tables from L2.
+template <class T>
+__device__ __forceinline__ T ld_stream(const GLOBAL_AS T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_stream(GLOBAL_AS T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane) { return lane ? (~0ull >> (64u - lane)) : 0ull; }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += t;
+  }
+  return v;
+}
+
+// largest r in [0, R) with off[r] <= t, given off[0] = 0 <= t < off[R]
+__device__ __forceinline__ uint32_t find_run(const uint32_t* off, uint32_t R, uint32_t t) {
+  uint32_t lo = 0, hi = R;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+constexpr uint32_t SCAN_THREADS = 1024;
+constexpr uint32_t SCAN_WAVES = SCAN_THREADS / 64;
+
+}  // namespace
+
+// ------------------------------------------------------------------ transport
+// m += dt*(C*M - C*diag(sink) + S + F);  C = m*diag(1/V)  for every
+// compartment of every active run, one thread each, all species at once (the
+// flow matrix entries are read once for all of them); also refreshes the
+// per-compartment record {M::record(C), p_move, q_exit} the particle pass
+// gathers, and clears S.  The extra last block lays out the step's tiles
+// (layout_pass), which needs only the counts of the previous step.
+template <uint32_t NT>
+__device__ void layout_pass(const BatchDev& b);
+
+template <class M>
+__global__ void __launch_bounds__(256) k_transport(BatchDev b) {
+  if (blockIdx.x == gridDim.x - 1) {
+    layout_pass<256>(b);
+    return;
+  }
+  constexpr int NS = M::NS;
+  const uint32_t total = b.n_comp_total;
+  const uint32_t stride = (gridDim.x - 1) * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const uint32_t r = find_run(b.comp_off, b.R, g);
+    const RunDev& rd = b.runs[r];
+    const RunState* rs = b.rs + r;
+    // the run's activity is recomputed here rather than read from the
+    // layout block, which writes it concurrently
+    if (rs->step >= rd.step_target) continue;
+    const uint32_t j = g - b.comp_off[r];
+    const uint32_t n_c = rd.n_c;
+    const uint32_t step = rs->step;
+    const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
+    double* rec = rd.rec + (size_t)j * rd.rec_w;
+    rec[NS] = mp.p_move[j];
+    double qx = mp.q_exit[j];
+    if (rd.n_vf && rd.vx_slot[j] >= 0) qx = rd.vf_q[rd.n_vf + (uint32_t)rd.vx_slot[j]];
+    rec[NS + 1] = qx;
+    const double* c = rd.C[step & 1u];
+    double* Cn = rd.C[(step + 1u) & 1u];
+    double cn[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) cn[s] = c[(size_t)s * n_c + j];
+    if (b.stages & STAGE_TRANSPORT) {
+      const uint32_t e0 = mp.col_ptr[j], e1 = mp.col_ptr[j + 1];
+      const uint32_t* __restrict__ src = mp.src;
+      const double* __restrict__ val = mp.val;
+      double acc[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+      // entries in groups of 8 whose loads are all in flight together (a
+      // column holds the degree plus the diagonal, rarely more than 8);
+      // the sums still run in entry order
+      constexpr uint32_t EG = 8;
+      for (uint32_t e = e0; e < e1; e += EG) {
+        uint32_t i[EG];
+        double v[EG], cv[EG][NS];
+#pragma unroll
+        for (uint32_t k = 0; k < EG; ++k) {
+          const uint32_t ek = e + k < e1 ? e + k : e1 - 1u;
+          i[k] = src[ek];
+          v[k] = val[ek];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < EG; ++k)
+#pragma unroll
+          for (int s = 0; s < NS; ++s) cv[k][s] = c[(size_t)s * n_c + i[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < EG; ++k)
+          if (e + k < e1)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc[s] += cv[k][s] * v[k];
+      }
+      double sink = rd.sink[j];
+      const double inv_v = mp.inv_v[j];
+      double fv[NS];  // time-dependent feeds of this compartment, in feed order
+#pragma unroll
+      for (int s = 0; s < NS; ++s) fv[s] = 0.0;
+      if (rd.n_vf) {
+        for (uint32_t q = rd.vf_sink_ptr[j]; q < rd.vf_sink_ptr[j + 1]; ++q) sink = sink + rd.vf_q[rd.vf_sink_k[q]];
+        for (uint32_t q = rd.vf_src_ptr[j]; q < rd.vf_src_ptr[j + 1]; ++q) {
+          const double add = rd.vf_q[rd.vf_src_k[q]] * rd.vf_src_c[q];
+          const uint32_t sp = rd.vf_src_s[q];
+#pragma unroll
+          for (int s = 0; s < NS; ++s)
+            if ((uint32_t)s == sp) fv[s] += add;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const size_t o = (size_t)s * n_c + j;
+        double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]));
+        // gas-liquid transfer from a fixed gas phase (performStepGL,
+        // implScalar.cpp:228-248): kla*(C* - C)*V
+        if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];
+        const double mm = rd.m[o] + rd.dt * dmdt;
+        rd.m[o] = mm;
+        cn[s] = mm * inv_v;
+      }
+    }
+    double rt[NS];
+    M::record(rd.params, cn, rt);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const size_t o = (size_t)s * n_c + j;
+      Cn[o] = cn[s];
+      rec[s] = rt[s];
+      rd.S[o] = 0.0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ particles
+// Counters of the tiles a block processed for one run, flushed with one
+// atomic each when the block moves on to another run (or finishes).
+struct TileCounts {
+  uint32_t exits = 0, deaths = 0, moves = 0, overflow = 0, processed = 0;
+};
+
+__device__ __forceinline__ void flush_counts(RunState* rs, TileCounts& c) {
+  if (c.exits) atomicAdd(&rs->step_exits, (unsigned long long)c.exits);
+  if (c.deaths) atomicAdd(&rs->step_deaths, (unsigned long long)c.deaths);
+  if (c.moves) atomicAdd(&rs->step_moves, (unsigned long long)c.moves);
+  if (c.overflow) atomicAdd(&rs->step_overflow, (unsigned long long)c.overflow);
+  if (c.processed) atomicAdd(&rs->step_processed, (unsigned long long)c.processed);
+  c = TileCounts{};
+}
+
+// Adds a block's LDS contribution accumulator, [n_s][win] for compartments
+// [w0, w0 + win) of a run with n_c compartments, to S: one atomic per
+// nonzero entry.
+template <int BLK>
+__device__ __forceinline__ void flush_window(const double* s_acc, GLOBAL_AS double* S, uint32_t ns, uint32_t win,
+                                             uint32_t n_c, uint32_t w0, uint32_t tid) {
+  const uint32_t len = ns * win;
+  if (win == n_c) {  // the whole run: the layouts coincide
+    for (uint32_t i = tid; i < len; i += BLK) {
+      const double v = s_acc[i];
+      if (v != 0.0) unsafeAtomicAdd((double*)(S + i), v);
+    }
+    return;
+  }
+  for (uint32_t i = tid; i < len; i += BLK) {
+    const double v = s_acc[i];
+    if (v != 0.0) {
+      const uint32_t q = i / win;
+      unsafeAtomicAdd((double*)(S + (size_t)q * n_c + w0 + (i - q * win)), v);
+    }
+  }
+}
+
+// First column of a cumulative row with cp[c] >= u, clamped to m-1: for a
+// non-decreasing row this is #{c < m-1 : cp[c] < u}, the result of the
+// reference's bisection (move_kernel.hpp:89-102).  The count runs over chunks
+// of 8 independent loads, one memory round trip each, and stops at the first
+// chunk holding a column with cp[c] >= u: in a non-decreasing row no later
+// column can count.  Most rows end within the first chunk.
+__device__ __forceinline__ uint32_t cdf_column(const GLOBAL_AS double* cp, uint32_t m, double u) {
+  constexpr int CH = 8;
+  uint32_t left = 0;
+  const int last = (int)m - 1;
+  for (int c0 = 0; c0 < last; c0 += CH) {
+    double v[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) v[j] = (c0 + j < last) ? cp[c0 + j] : 2.0;
+    uint32_t k = 0;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) k += (u > v[j]) ? 1u : 0u;
+    left += k;
+    if (k < (uint32_t)CH) break;
+  }
+  return left;
+}
+
+// Loads and stores of the NL consecutive slots of one lane: 8-byte fields
+// move in 16-byte accesses per lane when NL > 1 (aligned: every run's slots
+// start on a tile boundary), 4-byte fields in one access of 4*NL bytes.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int NL>
+__device__ __forceinline__ void ld_lane(const GLOBAL_AS double* p, double (&o)[NL]) {
+  if constexpr (NL == 1) {
+    o[0] = ld_stream(p);
+  } else {
+#pragma unroll
+    for (int h = 0; h < NL / 2; ++h) {
+      const f64x2 v = ld_stream((const GLOBAL_AS f64x2*)p + h);
+      o[2 * h] = v[0];
+      o[2 * h + 1] = v[1];
+    }
+  }
+}
+template <int NL>
+__device__ __forceinline__ void ld_lane(const GLOBAL_AS uint32_t* p, uint32_t (&o)[NL]) {
+  if constexpr (NL == 1) {
+    o[0] = ld_stream(p);
+  } else if constexpr (NL == 2) {
+    const u32x2 v = ld_stream((const GLOBAL_AS u32x2*)p);
+    o[0] = v[0];
+    o[1] = v[1];
+  } else {
+    const u32x4 v = ld_stream((const GLOBAL_AS u32x4*)p);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = v[j];
+  }
+}
+template <int NL>
+__device__ __forceinline__ uint32_t ld_lane(const GLOBAL_AS uint8_t* p) {  // NL status bytes
+  if constexpr (NL == 1)
+    return ld_stream(p);
+  else if constexpr (NL == 2)
+    return ld_stream((const GLOBAL_AS uint16_t*)p);
+  else
+    return ld_stream((const GLOBAL_AS uint32_t*)p);
+}
+template <int NL>
+__device__ __forceinline__ void st_lane(GLOBAL_AS double* p, const double (&v)[NL]) {
+  if constexpr (NL == 1) {
+    st_stream(p, v[0]);
+  } else {
+#pragma unroll
+    for (int h = 0; h < NL / 2; ++h) {
+      f64x2 w;
+      w[0] = v[2 * h];
+      w[1] = v[2 * h + 1];
+      st_stream((GLOBAL_AS f64x2*)p + h, w);
+    }
+  }
+}
+template <int NL>
+__device__ __forceinline__ void st_status(GLOBAL_AS uint8_t* p, uint32_t v) {
+  if constexpr (NL == 1)
+    *p = (uint8_t)v;
+  else if constexpr (NL == 2)
+    *(GLOBAL_AS uint16_t*)p = (uint16_t)v;
+  else
+    *(GLOBAL_AS uint32_t*)p = v;
+}
+
+// The particle pass.  The unit of work is a sub-chunk of SUB = 64*NL slots;
+// lane l holds the slots j*64 + l (ILV: every field access of the wave is a
+// contiguous 64-slot run) or NL*l + j (16-byte accesses per lane).  A
+// sub-chunk lists its movers and its division requests in its own segment of
+// the tile's lists, so sub-chunks are independent: the block's waves take
+// the sub-chunks of one group (one per wave) and groups are dealt to blocks G
+// at a time (b.tile_group; 0: one contiguous range per block).  Which
+// requests a tile accepts (the first `budget` in slot order) is decided
+// after the pass (k_resolve).  With PRIV the block accumulates the
+// contributions of one run in LDS (ds_add_f64), switching runs only between
+// groups; sub-chunks of another run add to S directly.
+//
+// Memory pipeline of a wave, per sub-chunk: issue the next sub-chunk's loads;
+// compute on the current one with its compartment records (gathered one step
+// earlier); gather the records of the next sub-chunk; store the current one.
+// Every load is issued for every lane (slots past the count read valid
+// memory of the tile and are masked where used), so the waits the compiler
+// places for a load's operands do not depend on the data.
+template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE, bool WIN>
+__global__ void __launch_bounds__(BLK)
+    k_particle(BatchDev b, const RunDev* __restrict__ runs, const RunState* __restrict__ rsr,
+               const uint32_t* __restrict__ tile_off) {
+  constexpr int NV = M::NV, NS = M::NS, REC = rec_stride(M::NS);
+  constexpr int NQ = NS + 2;             // record fields: M::record terms r_0..r_{n_s-1}, p_move, q_exit
+  constexpr int NP = M::NP > 0 ? M::NP : 1;
+  constexpr uint32_t WPB = BLK / 64;     // waves per block = sub-chunks per group
+  constexpr uint32_t SUB = 64 * NL;       // slots per sub-chunk
+  static_assert(NL == 1 || NL == 2 || NL == 4, "1, 2 or 4 slots per lane");
+  constexpr uint32_t NSUB = TILE / SUB;  // sub-chunks per tile (b.nsub is set to the same)
+  extern __shared__ double s_acc[];      // [n_s][acc_win] of run acc_run (PRIV)
+  __shared__ uint32_t s_w0;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = uniform(tid >> 6);  // wave-uniform: scalar bookkeeping
+  const uint32_t stages = b.stages;
+  const uint64_t P = b.P;
+  const uint32_t R = b.R;
+  const uint32_t T = tile_off[R];
+  const uint32_t n_sc = T * NSUB;        // sub-chunks of the pass
+  const uint32_t n_groups = (n_sc + WPB - 1) / WPB;
+  const uint32_t chunk = (n_groups + gridDim.x - 1) / gridDim.x;
+  const uint32_t G = b.tile_group ? b.tile_group : (chunk ? chunk : 1u);
+  uint32_t grp = blockIdx.x * G;
+  if (grp >= n_groups) return;  // the whole block
+  auto next_group = [&](uint32_t g) -> uint32_t {
+    const uint32_t g0 = g - g % G;
+    return (g + 1 < g0 + G) ? g + 1 : g0 + gridDim.x * G;
+  };
+  const bool do_model = (stages & STAGE_MODEL) != 0;
+  const bool do_contrib = (stages & STAGE_CONTRIB) != 0;
+
+  // one sub-chunk as the wave sees it; sc >= n_sc is a phantom: no slots,
+  // no stores.  cnt counts the sub-chunk's slots below the run's count.
+  struct SubRef {
+    uint32_t sc, t, r, slot0, cnt;  // slot0: run-relative slot of the sub-chunk's first slot
+    uint64_t base;                   // its batch-wide slot
+  };
+  auto sub_ref = [&](uint32_t sc) -> SubRef {
+    SubRef q{sc, 0u, 0u, 0u, 0u, 0ull};
+    if (sc < n_sc) {
+      q.t = sc / NSUB;
+      q.r = find_run(tile_off, R, q.t);
+      q.slot0 = (q.t - tile_off[q.r]) * TILE + (sc % NSUB) * SUB;
+      const uint32_t n = rsr[q.r].n;
+      q.cnt = n > q.slot0 ? (n - q.slot0 < SUB ? n - q.slot0 : SUB) : 0u;
+      q.base = runs[q.r].p_off + q.slot0;
+    }
+    return q;
+  };
+  // sub-chunk-relative slot of the lane's j-th slot: interleaved (64 apart,
+  // every field access a contiguous 64-lane run) or NL consecutive slots
+  auto off = [&](int j) -> uint32_t { return ILV ? (uint32_t)j * 64u + lane : NL * lane + (uint32_t)j; };
+  // the NL slots of a lane in one sub-chunk, as loaded
+  struct Sub {
+    uint32_t st;       // NL status bytes
+    uint32_t pos[NL];
+    double x[NV][NL];
+    double ad[NL], ah[NL];
+  };
+  auto load_sub = [&](const SubRef& q, Sub& a) {
+    const bool leave = runs[q.r].enable_leave && (stages & STAGE_LEAVE);
+    if constexpr (ILV) {
+      uint32_t st = 0;
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        const uint64_t ij = q.base + (q.cnt ? off(j) : 0u);  // an empty sub-chunk reads its first slot
+        st |= (uint32_t)ld_stream(gp(b.status) + ij) << (8 * j);
+        a.pos[j] = ld_stream(gp(b.pos) + ij);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.x[v][j] = ld_stream(gp(b.state) + (uint64_t)v * P + ij);
+        a.ad[j] = ld_stream(gp(b.age_div) + ij);
+        a.ah[j] = ld_stream(gp(b.age_hyd) + (leave ? ij : q.base));
+      }
+      a.st = st;
+    } else {
+      const uint64_t i0 = q.base + (q.cnt ? NL * lane : 0u);
+      a.st = ld_lane<NL>(gp(b.status) + i0);
+      ld_lane<NL>(gp(b.pos) + i0, a.pos);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ld_lane<NL>(gp(b.state) + (uint64_t)v * P + i0, a.x[v]);
+      ld_lane<NL>(gp(b.age_div) + i0, a.ad);
+      ld_lane<NL>(gp(b.age_hyd) + (leave ? i0 : q.base), a.ah);
+    }
+  };
+  auto gather = [&](const SubRef& q, const Sub& a, double (&rr)[NL][NQ]) {
+    const GLOBAL_AS double* __restrict__ rec = gp(runs[q.r].rec);
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      // slots past the count hold no particle: their position may be anything
+      const uint32_t pj = off(j) < q.cnt ? a.pos[j] : 0u;
+      const GLOBAL_AS double* rp = rec + (size_t)pj * REC;
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) rr[j][f] = rp[f];
+    }
+  };
+
+  // per-wave counters of the run cnt_run, flushed when the wave changes runs
+  TileCounts cnt_acc;
+  uint32_t cnt_run = 0xFFFFFFFFu;
+  // the LDS accumulator holds compartments [acc_w0, acc_w0 + acc_win) of run
+  // acc_run: the whole run when it fits (acc_win = n_c), else a window
+  // centred on the block's first slot of the run (particles are stored in
+  // compartment order, so a block's slots see few compartments); particles
+  // outside the window add to S directly
+  uint32_t acc_run = 0xFFFFFFFFu;
+  GLOBAL_AS double* acc_S = nullptr;
+  uint32_t acc_ns = 0, acc_nc = 0, acc_win = 0, acc_w0 = 0;
+
+  SubRef cur = sub_ref(grp * WPB + wave);
+  Sub A;
+  double rr[NL][NQ];
+  load_sub(cur, A);
+  gather(cur, A, rr);
+  // Later sub-chunks enter with their slots copied from the prefetch and
+  // their records in flight; taking the first one's registers through empty
+  // asm statements makes the entry path match.
+  asm volatile("" : "+v"(A.st));
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    asm volatile("" : "+v"(A.ad[j]), "+v"(A.ah[j]), "+v"(A.pos[j]));
+#pragma unroll
+    for (int v = 0; v < NV; ++v) asm volatile("" : "+v"(A.x[v][j]));
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) asm volatile("" : "+v"(rr[j][f]));
+  }
+
+  for (;;) {
+    // ---- block-uniform: the run whose contributions the LDS accumulates
+    if (PRIV) {
+      const uint32_t r0 = find_run(tile_off, R, (grp * WPB) / NSUB);
+      if (r0 != acc_run) {
+        __syncthreads();
+        flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
+        acc_run = r0;
+        acc_S = gp(runs[r0].S);
+        acc_ns = runs[r0].n_s;
+        acc_nc = runs[r0].n_c;
+        acc_win = acc_nc;
+        if constexpr (WIN) {
+          acc_win = b.acc_w < acc_nc ? b.acc_w : acc_nc;
+          if (tid == 0) {
+            // wave 0 holds the group's first sub-chunk; its lane 0 the first slot
+            const uint32_t c0 = cur.cnt ? A.pos[0] : 0u;
+            const uint32_t half = acc_win / 2;
+            const uint32_t lo = c0 > half ? c0 - half : 0u;
+            s_w0 = lo + acc_win <= acc_nc ? lo : acc_nc - acc_win;
+          }
+          __syncthreads();
+          acc_w0 = s_w0;
+        }
+        for (uint32_t i = tid; i < acc_ns * acc_win; i += BLK) s_acc[i] = 0.0;
+        __syncthreads();
+      } else if constexpr (WIN) {
+        // the window follows the block's slots: when the group's first
+        // compartment leaves the window's lower half (slots are in
+        // compartment order, newborns at the end of the run in parent
+        // order), flush it and re-centre a quarter ahead
+        if (acc_win < acc_nc) {
+          if (tid == 0) s_w0 = cur.cnt ? A.pos[0] : acc_w0;
+          __syncthreads();
+          const uint32_t c0 = s_w0;
+          __syncthreads();  // every wave has read s_w0
+          if (c0 - acc_w0 >= acc_win / 2) {  // wraps when below the window
+            flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
+            const uint32_t q = acc_win / 4;
+            const uint32_t lo = c0 > q ? c0 - q : 0u;
+            acc_w0 = lo + acc_win <= acc_nc ? lo : acc_nc - acc_win;
+            __syncthreads();  // the flush has read every entry
+            for (uint32_t i = tid; i < acc_ns * acc_win; i += BLK) s_acc[i] = 0.0;
+            __syncthreads();
+          }
+        }
+      }
+    }
+    // ---- the wave's sub-chunk of this group
+    const uint32_t sc = cur.sc;
+    const uint32_t r = cur.r;
+    const RunDev& rd = runs[r];
+    if (cur.cnt && r != cnt_run) {  // wave moves to another run
+      if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
+      cnt_acc = TileCounts{};
+      cnt_run = r;
+    }
+    const uint32_t step = rsr[r].step;
+    const double dt = rd.dt;
+    const double w = rd.weight;
+    const uint32_t n_c = rd.n_c;
+    const uint64_t seed = rd.seed;
+    GLOBAL_AS double* __restrict__ S = gp(rd.S);
+    const bool to_lds = PRIV && r == acc_run;
+    double p[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) p[q] = rd.params[q];
+    const bool do_move = rd.enable_move && (stages & STAGE_MOVE);
+    const bool do_leave = rd.enable_leave && (stages & STAGE_LEAVE);
+    const uint32_t g_next = next_group(grp);
+
+    // ---- prefetch: the wave's sub-chunk of the next group (a phantom past
+    // the last group)
+    const SubRef nxt = sub_ref(g_next < n_groups ? g_next * WPB + wave : n_sc);
+    __builtin_amdgcn_sched_barrier(0);
+    Sub B;
+    load_sub(nxt, B);
+    __builtin_amdgcn_sched_barrier(0);
+
+    // ---- per-particle work: arithmetic only
+    const uint32_t i0 = NL * lane;  // contiguous layout: sub-chunk-relative slot of j = 0
+    const bool pair_valid = i0 < cur.cnt;
+    bool valid[NL], idle[NL], want[NL], mover[NL], exited[NL], dead[NL];
+    double x[NL][NV], ad[NL], ah[NL];
+    double contrib[NL][NS];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      const uint32_t stj = (A.st >> (8 * j)) & 0xFFu;
+      valid[j] = off(j) < cur.cnt;
+      idle[j] = valid[j] && stj == IDLE;
+      want[j] = mover[j] = exited[j] = dead[j] = false;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[j][v] = A.x[v][j];
+      ad[j] = A.ad[j];
+      ah[j] = A.ah[j];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) contrib[j][q] = 0.0;
+      if (!idle[j]) continue;
+      int code = IDLE;
+      if (do_model) {
+        ad[j] += dt;
+        code = M::update(p, dt, x[j], rr[j], contrib[j]);
+      }
+      if constexpr (M::CAN_DIE) {
+        if (code == DEAD) {  // a dead cell stops contributing, moving and leaving
+          dead[j] = true;
+          continue;
+        }
+      }
+      want[j] = code == DIVISION;
+      const U4 u = draw4(seed, cur.slot0 + off(j), step, TAG_TRANSPORT, 0u);
+      // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:313-325)
+      // is listed; k_resolve finds its neighbour and runs its exit test
+      mover[j] = do_move && rr[j][NS] > u01(u.x);
+      if (do_leave) {
+        ah[j] += dt;
+        const double qx = rr[j][NS + 1];
+        if (!mover[j] && qx < 1.0 && u01(u.z) > qx) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
+          if constexpr (PROBE) {
+            if (rd.probe_exit) {
+              const unsigned long long e = atomicAdd(&b.rs[r].probe_exit_n, 1ull);
+              if (e < rd.probe_cap) rd.probe_exit[e] = ah[j];
+            }
+          }
+          ah[j] = 0.0;
+          exited[j] = true;
+        }
+      }
+    }
+    // contributions of the Idle particles (a dead cell contributes nothing)
+    if (do_contrib) {
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        if (!idle[j] || dead[j]) continue;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const double c = w * contrib[j][q];
+          if constexpr (WIN) {
+            const uint32_t k = A.pos[j] - acc_w0;  // wraps above the window when below it
+            if (to_lds && k < acc_win)
+              atomicAdd(s_acc + q * acc_win + k, c);
+            else
+              unsafeAtomicAdd((double*)(S + (size_t)q * n_c + A.pos[j]), c);
+          } else {
+            if (to_lds)
+              atomicAdd(s_acc + q * n_c + A.pos[j], c);
+            else
+              unsafeAtomicAdd((double*)(S + (size_t)q * n_c + A.pos[j]), c);
+          }
+        }
+      }
+    }
+
+    // ---- the records of the next sub-chunk (its positions have arrived)
+    double rn[NL][NQ];
+    gather(nxt, B, rn);
+
+    // ---- stores: the lane's final state, status changes, and the
+    // sub-chunk's movers and division requests in slot order
+    const uint64_t gi = cur.base + i0;
+    if constexpr (ILV) {
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        if (!valid[j]) continue;
+        const uint64_t gj = cur.base + off(j);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) st_stream(gp(b.state) + (uint64_t)v * P + gj, x[j][v]);
+        st_stream(gp(b.age_div) + gj, ad[j]);
+        if (do_leave) st_stream(gp(b.age_hyd) + gj, ah[j]);
+        if (exited[j] || dead[j]) gp(b.status)[gj] = dead[j] ? (uint8_t)DEAD : (uint8_t)EXIT;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double xv[NL];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) xv[j] = x[j][v];
+        if (pair_valid) st_lane<NL>(gp(b.state) + (uint64_t)v * P + gi, xv);
+      }
+      if (pair_valid) st_lane<NL>(gp(b.age_div) + gi, ad);
+      if (do_leave && pair_valid) st_lane<NL>(gp(b.age_hyd) + gi, ah);
+      uint32_t stn = A.st;
+      bool changed = false;
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        const uint32_t nv = dead[j] ? (uint32_t)DEAD : exited[j] ? (uint32_t)EXIT : ((A.st >> (8 * j)) & 0xFFu);
+        stn = (stn & ~(0xFFu << (8 * j))) | (nv << (8 * j));
+        changed = changed || exited[j] || dead[j];
+      }
+      if (changed && pair_valid) st_status<NL>(gp(b.status) + gi, stn);
+    }
+    const uint64_t lt = lanemask_lt(lane);
+    const uint64_t seg = (uint64_t)sc * SUB;  // the sub-chunk's list segment
+    const uint32_t slot_t = cur.slot0 % TILE;  // tile-relative slot of the sub-chunk's first slot
+    uint32_t n_mov = 0, n_div = 0;
+    {
+      uint64_t mv[NL], wd[NL];
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        mv[j] = __ballot(mover[j]);
+        wd[j] = __ballot(want[j]);
+      }
+      // rank of slot off(j) among the flagged slots of the sub-chunk
+      uint32_t rm[NL], rd_[NL];
+      if constexpr (ILV) {
+        uint32_t am = 0, adv = 0;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          rm[j] = am + (uint32_t)__popcll(mv[j] & lt);
+          rd_[j] = adv + (uint32_t)__popcll(wd[j] & lt);
+          am += (uint32_t)__popcll(mv[j]);
+          adv += (uint32_t)__popcll(wd[j]);
+        }
+        n_mov = am;
+        n_div = adv;
+      } else {
+        uint32_t bm = 0, bd = 0;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          bm += (uint32_t)__popcll(mv[j] & lt);
+          bd += (uint32_t)__popcll(wd[j] & lt);
+          n_mov += (uint32_t)__popcll(mv[j]);
+          n_div += (uint32_t)__popcll(wd[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          rm[j] = bm;
+          rd_[j] = bd;
+          bm += mover[j] ? 1u : 0u;
+          bd += want[j] ? 1u : 0u;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        if (mover[j]) {
+          gp(b.move_slot)[seg + rm[j]] = (uint16_t)(slot_t + off(j));
+          gp(b.move_pos)[seg + rm[j]] = A.pos[j];
+        }
+        if (n_div && want[j]) {  // rare
+          gp(b.div_slot)[seg + rd_[j]] = (uint16_t)(slot_t + off(j));
+          gp(b.div_pos)[seg + rd_[j]] = A.pos[j];
+        }
+      }
+    }
+    if (lane == 0 && sc < n_sc) {
+      gp(b.sub_moves)[sc] = n_mov;
+      gp(b.sub_divs)[sc] = n_div;
+    }
+    // ---- counters of the sub-chunk (wave-uniform)
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      cnt_acc.processed += (uint32_t)__popcll(__ballot(idle[j]));
+      cnt_acc.exits += (uint32_t)__popcll(__ballot(exited[j]));
+      cnt_acc.deaths += (uint32_t)__popcll(__ballot(dead[j]));
+    }
+    cnt_acc.moves += n_mov;
+
+    __builtin_amdgcn_sched_barrier(0);
+    cur = nxt;
+    A = B;
+#pragma unroll
+    for (int j = 0; j < NL; ++j)
+#pragma unroll
+      for (int f = 0; f < NQ; ++f) rr[j][f] = rn[j][f];
+    grp = g_next;
+    if (grp >= n_groups) break;
+  }
+  if (PRIV) {
+    __syncthreads();
+    flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
+  }
+  if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
+}
+
+// After the particle pass, per group of RES_SC sub-chunks (one per lane of
+// the first wave): (1) the movers the pass listed take their neighbour, the
+// first CDF column at or above u, and run the exit test in the new
+// compartment (move_kernel.hpp:392-437, 586-648); (2) each tile accepts the
+// first `budget` division requests in slot order (the reference's division
+// buffer, model_kernel.hpp:163-268), halves the mother and stages the
+// newborn at its rank.  Every mover and every request is independent, so
+// the processing order is free.
+constexpr uint32_t RES_SC = 64;
+
+template <class M>
+__global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
+                                                 const MapDev* __restrict__ maps,
+                                                 const uint32_t* __restrict__ tile_off) {
+  constexpr int NV = M::NV;
+  constexpr int NP = M::NP > 0 ? M::NP : 1;
+  __shared__ uint32_t s_pre[RES_SC + 1];  // movers: prefix over the group's sub-chunks
+  __shared__ uint32_t s_run[RES_SC];
+  __shared__ uint32_t s_dcnt[RES_SC];     // division requests per sub-chunk
+  __shared__ uint32_t s_dpre[RES_SC];     // ... their prefix within the tile
+  __shared__ uint32_t s_births[RES_SC];   // accepted requests per tile of the group
+  __shared__ uint32_t s_any;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t R = b.R;
+  const uint32_t nsub = b.nsub;          // sub-chunks per tile (divides RES_SC)
+  const uint32_t sub = TILE / nsub;      // slots per sub-chunk
+  const uint32_t T = tile_off[R];
+  const uint32_t n_sc = T * nsub;
+  const uint32_t tpg = RES_SC / nsub;    // tiles per group
+  const uint32_t n_groups = (n_sc + RES_SC - 1) / RES_SC;
+  const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
+  for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+    const uint32_t sc0 = grp * RES_SC;
+    const uint32_t t0 = sc0 / nsub;
+    if (tid < 64) {  // the group's sub-chunks, one lane each
+      const uint32_t lane = tid, sc = sc0 + lane;
+      uint32_t cm = 0, cd = 0, r = 0;
+      if (sc < n_sc) {
+        cm = b.sub_moves[sc];
+        cd = b.sub_divs[sc];
+        r = find_run(tile_off, R, sc / nsub);
+      }
+      const uint32_t inc = wave_incl_scan(cm, lane);
+      s_pre[lane] = inc - cm;
+      if (lane == RES_SC - 1) s_pre[RES_SC] = inc;
+      s_run[lane] = r;
+      s_dcnt[lane] = cd;
+      // requests: prefix within each tile (segments of nsub lanes)
+      const uint32_t incd = wave_incl_scan(cd, lane);
+      const uint32_t first = lane - lane % nsub;
+      const uint32_t before = __shfl(incd, first ? first - 1 : 0, 64);
+      const uint32_t seg_base = first ? before : 0u;
+      s_dpre[lane] = incd - cd - seg_base;
+      if (lane % nsub == nsub - 1) {  // the tile's births: the first `budget` requests
+        const uint32_t t = t0 + lane / nsub;
+        const uint32_t tot = incd - seg_base;
+        uint32_t births = 0;
+        if (t < T) {
+          RunState* rs = b.rs + r;
+          const uint32_t budget = rs->budget;
+          births = tot < budget ? tot : budget;
+          b.tile_births[t] = births;
+          if (tot > births) atomicAdd(&rs->step_overflow, (unsigned long long)(tot - births));
+        }
+        s_births[lane / nsub] = births;
+      }
+      const uint64_t any = __ballot(cd != 0);
+      if (lane == 0) s_any = any != 0 ? 1u : 0u;
+    }
+    __syncthreads();
+    // ---- movers
+    const uint32_t total = s_pre[RES_SC];
+    for (uint32_t e = tid; e < total; e += blockDim.x) {
+      uint32_t lo = 0, hi = RES_SC;  // the sub-chunk j with s_pre[j] <= e < s_pre[j+1]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= e) lo = mid; else hi = mid;
+      }
+      const uint32_t j = lo;
+      const uint32_t sc = sc0 + j;
+      const uint32_t t = sc / nsub;
+      const uint32_t r = s_run[j];
+      const RunDev& rd = runs[r];
+      RunState* rs = b.rs + r;
+      const uint64_t q = (uint64_t)sc * sub + (e - s_pre[j]);
+      const uint32_t slot = (t - tile_off[r]) * TILE + b.move_slot[q];
+      uint32_t pos = b.move_pos[q];
+      const uint64_t g = rd.p_off + slot;
+      const uint32_t step = rs->step;
+      const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
+      const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
+      const uint32_t m = mp.m;
+      const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
+      pos = gp(mp.nbr)[(size_t)pos * m + col];
+      b.pos[g] = pos;
+      if (rd.enable_leave && leave_stage) {
+        const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + rd.n_s + 1];
+        if (qx < 1.0 && u01(u.z) > qx) {
+          const double age = b.age_hyd[g];
+          if (rd.probe_exit) {
+            const unsigned long long k = atomicAdd(&rs->probe_exit_n, 1ull);
+            if (k < rd.probe_cap) rd.probe_exit[k] = age;
+          }
+          b.age_hyd[g] = 0.0;
+          b.status[g] = EXIT;
+          atomicAdd(&rs->step_exits, 1ull);
+        }
+      }
+    }
+    // ---- divisions (rare)
+    if (s_any) {
+      for (uint32_t k = 0; k < tpg; ++k) {
+        const uint32_t t = t0 + k;
+        const uint32_t births = s_births[k];
+        if (t >= T || births == 0) continue;
+        const uint32_t r = s_run[k * nsub];
+        const RunDev& rd = runs[r];
+        RunState* rs = b.rs + r;
+        const uint32_t step = rs->step;
+        double p[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) p[i] = rd.params[i];
+        for (uint32_t e = tid; e < births; e += blockDim.x) {
+          uint32_t j = k * nsub;  // the tile's sub-chunk holding request e
+          while (j + 1 < (k + 1) * nsub && s_dpre[j + 1] <= e) ++j;
+          const uint64_t q = (uint64_t)(sc0 + j) * sub + (e - s_dpre[j]);
+          const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
+          const uint64_t g = rd.p_off + slot;
+          double x[NV], xd[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
+          if (rd.probe_div) {
+            const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
+            if (i < rd.probe_cap) rd.probe_div[i] = b.age_div[g];
+          }
+          M::division(p, x, xd, RngKey{rd.seed, slot, step});
+          const uint64_t sidx = (uint64_t)t * b.stage_stride + e;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            b.state[(uint64_t)v * b.P + g] = x[v];
+            b.st_state[(uint64_t)v * b.ST + sidx] = xd[v];
+          }
+          b.age_div[g] = 0.0;
+          b.st_pos[sidx] = b.div_pos[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ layout
+// Tile -> run table and per-tile division budget for the next pass:
+// budget = min(stage_cap, (cap - n) / tiles), so staged newborns always fit.
+template <uint32_t NT>
+__device__ void layout_pass(const BatchDev& b) {
+  constexpr uint32_t NWV = NT / 64;
+  __shared__ uint32_t s_w[NWV];
+  __shared__ uint32_t s_carry;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t r0 = 0; r0 < b.R; r0 += NT) {
+    const uint32_t r = r0 + tid;
+    uint32_t tiles = 0;
+    if (r < b.R) {
+      RunState* rs = b.rs + r;
+      const RunDev& rd = b.runs[r];
+      const uint32_t active = rs->step < rd.step_target ? 1u : 0u;
+      rs->active = active;
+      tiles = active ? (rs->n + TILE - 1u) / TILE : 0u;
+      uint32_t budget = 0;
+      if (tiles) {
+        const uint32_t q = (rd.cap - rs->n) / tiles;
+        budget = q < rd.stage_cap ? q : rd.stage_cap;
+      }
+      rs->budget = budget;
+      rs->n_tiles = tiles;
+    }
+    const uint32_t inc = wave_incl_scan(tiles, lane);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (uint32_t w = 0; w < NWV; ++w) {
+      if (w < wave) wpre += s_w[w];
+      tot += s_w[w];
+    }
+    const uint32_t carry = s_carry;
+    if (r < b.R) {
+      b.tile_off[r] = carry + wpre + inc - tiles;
+      b.rs[r].tile_begin = carry + wpre + inc - tiles;
+    }
+    __syncthreads();
+    if (tid == 0) s_carry = carry + tot;
+    __syncthreads();
+  }
+  if (tid == 0) b.tile_off[b.R] = s_carry;
+}
+
+
+// ------------------------------------------------------------------ properties
+// Per-particle export values of run r (get_properties, post_process.hpp:54-98):
+// for every Idle slot the model state and M::mass, row-major [n_var+1][n]
+// (zero for inactive slots), and their sums per compartment [n_var+1][n_c].
+template <class M>
+__global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint64_t n, double* values,
+                                                    double* spatial) {
+  constexpr int NV = M::NV;
+  const RunDev& rd = b.runs[r];
+  const uint32_t n_c = rd.n_c;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = rd.p_off + s;
+    if (b.status[g] != IDLE) {
+      for (int v = 0; v <= NV; ++v) values[(uint64_t)v * n + s] = 0.0;
+      continue;
+    }
+    const uint32_t c = b.pos[g];
+    double x[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      x[v] = b.state[(uint64_t)v * b.P + g];
+      values[(uint64_t)v * n + s] = x[v];
+      unsafeAtomicAdd(spatial + (uint64_t)v * n_c + c, x[v]);
+    }
+    const double m = M::mass(rd.params, x);
+    values[(uint64_t)NV * n + s] = m;
+    unsafeAtomicAdd(spatial + (uint64_t)NV * n_c + c, m);
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <class M, int BLK, bool PRIV, bool PROBE, bool WIN>
+static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((k_particle<M, BLK, M::NL, LANE_INTERLEAVE, PRIV, PROBE, WIN>), dim3(grid), dim3(BLK), lds, s, b,
+                     b.runs, (const RunState*)b.rs, (const uint32_t*)b.tile_off);
+}
+
+// The variants: direct global atomics (256 threads); an LDS accumulator of
+// whole runs (256/512/1024) or of a compartment window (256/512); and the
+// 256-thread probe variants of each accumulator kind.
+template <class M>
+static const void* kernel_ptr_t(int blk, bool priv, bool probe, bool win) {
+  if (probe) {
+    if (!priv) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, true, false>;
+    return win ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, true>
+               : (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, false>;
+  }
+  if (!priv) return blk == 256 ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, false, false> : nullptr;
+  if (win) {
+    if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, true>;
+    if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, true>;
+    return nullptr;
+  }
+  if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, false>;
+  if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, false>;
+  if (blk == 1024) return (const void*)&k_particle<M, 1024, M::NL, LANE_INTERLEAVE, true, false, false>;
+  return nullptr;
+}
+
+// Runs with probes take a 256-thread variant that records exit and division
+// ages; the atomics it needs (with returned values) are kept out of the
+// production variants, where they would make every wait for a load in the
+// tile loop a wait for all outstanding memory operations.
+template <class M>
+static void launch_t(const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
+  if (L.probe) {
+    if (!L.priv)
+      launch_particle_t<M, 256, false, true, false>(b, L.grid, 0, s);
+    else if (L.win)
+      launch_particle_t<M, 256, true, true, true>(b, L.grid, L.lds, s);
+    else
+      launch_particle_t<M, 256, true, true, false>(b, L.grid, L.lds, s);
+  } else if (!L.priv) {
+    launch_particle_t<M, 256, false, false, false>(b, L.grid, 0, s);
+  } else if (L.win) {
+    if (L.block == 512)
+      launch_particle_t<M, 512, true, false, true>(b, L.grid, L.lds, s);
+    else
+      launch_particle_t<M, 256, true, false, true>(b, L.grid, L.lds, s);
+  } else if (L.block == 256) {
+    launch_particle_t<M, 256, true, false, false>(b, L.grid, L.lds, s);
+  } else if (L.block == 512) {
+    launch_particle_t<M, 512, true, false, false>(b, L.grid, L.lds, s);
+  } else {
+    launch_particle_t<M, 1024, true, false, false>(b, L.grid, L.lds, s);
+  }
+}
+
+template <class M>
+void transport_t(const BatchDev& b, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_transport<M>, dim3(grid), dim3(256), 0, s, b);
+}
+template <class M>
+void resolve_t(const BatchDev& b, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve<M>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, (const uint32_t*)b.tile_off);
+}
+template <class M>
+void properties_t(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s) {
+  hipLaunchKernelGGL(k_properties<M>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial);
+}
+
+// The launchers of model M's kernels.
+template <class M>
+ModelOps model_ops() {
+  return ModelOps{&transport_t<M>, &launch_t<M>, &kernel_ptr_t<M>, &resolve_t<M>, &properties_t<M>};
+}
+
+}  // namespace mcst

@@ -35,7 +35,7 @@ EXPORTS = [
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
     "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
-    "mcst_engine_get_steps",
+    "mcst_engine_get_steps", "mcst_model_load",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -64,6 +64,7 @@ def lib():
             "mcst_model_lookup": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)],
             "mcst_model_default_params": [C.c_int, _f64p, C.c_int],
+            "mcst_model_load": [C.c_char_p, C.POINTER(C.c_int)],
             "mcst_engine_create": [C.c_char_p, C.c_int, C.POINTER(_vp)],
             "mcst_engine_destroy": [C.POINTER(_vp)],
             "mcst_run_create": [_vp, u32, u32, dbl, u64, C.POINTER(i32)],
@@ -128,6 +129,17 @@ def model_info(name):
     _ck(lib().mcst_model_lookup(name.encode(), C.byref(mid), C.byref(nv), C.byref(ns), C.byref(npar)),
         "model_lookup")
     return mid.value, nv.value, ns.value, npar.value
+
+
+def load_model(path):
+    """Load a model plugin (built by `build.py --udf`); returns its model id.
+    Its model is then known by the name its struct declares."""
+    mid = C.c_int()
+    _ck(lib().mcst_model_load(os.fsencode(os.path.abspath(path)), C.byref(mid)), "model_load")
+    return mid.value
+
+
+EXAMPLE_PLUGIN = os.path.join(_PKG, "examples", "libudf_minimal.so")
 
 
 def default_params(name):

@@ -85,9 +85,20 @@ const char* mcst_last_error(void);
  * (apps/libs/models/public/models/{monod,fixed_length,simple_acetate}.hpp)
  * compiled for gfx950, plus the mass-based models of the benchmark cases.
  * Names: "monod_mass", "dual_monod", "monod", "fixed_length", "simple_acetate",
- * "quad_monod". */
+ * "quad_monod", and those of loaded plugins (mcst_model_load). */
 int mcst_model_lookup(const char* name, int* model_id, int* n_var, int* n_species, int* n_params);
 int mcst_model_default_params(int model_id, double* params, int n);
+/* Loads a model plugin (a shared library built by `build.py --udf` from a
+ * model struct; the dlopen'ed udf_model module of the reference,
+ * apps/api/src/udf_handle.cpp:21-38 and udf_includes.cpp:57-78) and registers
+ * its model under the struct's NAME; *model_id receives its id (100, 101, ...
+ * in load order).  Loading the same path again returns the same id.  The
+ * name "udf_model" (mcst_model_lookup, mcst_engine_create) loads the plugin
+ * named by the environment variable BIOMC_LIB_UDF on first use, as the
+ * reference does.  MCST_ERR_ARG for a file that is not a plugin, a plugin
+ * built against other device structures, or a name already taken.  Plugins
+ * stay loaded until the process ends. */
+int mcst_model_load(const char* path, int* model_id);
 
 /* Engine lifetime: binds HIP device `device` and creates the engine's stream. */
 int mcst_engine_create(const char* model_name, int device, mcst_engine** out);

@@ -26,7 +26,8 @@
  *   repartition ........... apps/libs/mc/src/unit.cpp:67-100,190-230
  *   models ................ apps/libs/models/public/models/monod.hpp:80-193,
  *                           fixed_length.hpp:360-413, simple_acetate.hpp:132-248, utils.hpp:63-67
- *                           (monod_mass, dual_monod, quad_monod: the benchmark cases' models)
+ *                           (monod_mass, dual_monod, quad_monod: the benchmark cases' models;
+ *                           udf_minimal: apps/udf_model/minimal.cpp:23-107, the example plugin)
  *   gas-liquid transfer ... apps/libs/simulation/src/implScalar.cpp:228-248 (liquid side, fixed gas)
  *   export properties ..... apps/core/public/core/post_process.hpp:54-98
  *
@@ -179,9 +180,9 @@ static double trunc_normal(double mu, double sd, double lo, double hi, double u)
 /* Models.  State is an FP64 vector x[n_var]; c[] the local concentrations;
  * contrib[] the per-particle source term (kg/s) for each species. */
 
-static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9, 8};
-static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2, 4};
-static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0, 15};
+static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9, 8, 2};
+static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2, 4, 1};
+static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0, 15, 5};
 
 int orc_model_info(int model, int* n_var, int* n_species, int* n_params) {
   if (model < 0 || model >= ORC_MODEL_COUNT) return -1;
@@ -210,6 +211,9 @@ int orc_model_default_params(int model, double* p, int n) {
     case ORC_MODEL_QUAD_MONOD: /* qs_max, Ks, Ko, Ka, qa_max, Yxs, Yxa, Yos, Yoa, Yas, tau, Yco2, m_div, m0 mean, m0 sd */
       p[0] = 2.0; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.01; p[4] = 0.5; p[5] = 0.5; p[6] = 0.3; p[7] = 0.3;
       p[8] = 0.4; p[9] = 0.3; p[10] = 0.2; p[11] = 0.4; p[12] = 2e-15; p[13] = 1.4e-15; p[14] = 0.2e-15;
+      break;
+    case ORC_MODEL_UDF_MINIMAL: /* Y, l_dot_max, l_max, k, d_m (udf_model/minimal.cpp:26-35) */
+      p[0] = 0.5; p[1] = 2e-6 / 3600.0; p[2] = 2e-6; p[3] = 1e-3; p[4] = 0.6e-6;
       break;
     default:
       break;
@@ -259,6 +263,10 @@ static void model_init(int model, const double* p, double* x, uint64_t seed, uin
       x[0] = p[1] * (0.5 + 0.5 * orc_u01(w[0]));
       x[1] = p[1];
       break;
+    case ORC_MODEL_UDF_MINIMAL: /* minimal.cpp:55-63: length uniform on (l_max/2, l_max) (:121-140) */
+      x[0] = p[2] * (0.5 + 0.5 * orc_u01(w[0]));
+      x[1] = p[2];
+      break;
     case ORC_MODEL_SIMPLE_ACETATE: { /* simple_acetate.hpp:132-152 */
       x[0] = trunc_normal(SA_L_MAX * 0.75, SA_L_MAX / 10.0, 0.7 * SA_L_MIN, SA_L_MAX * 1.3, orc_u01_52(w[0], w[1]));
       x[1] = trunc_normal(SA_L_MAX * 0.75, SA_L_MAX / 10.0, 0.7 * SA_L_MIN, SA_L_MAX * 1.3, orc_u01_52(w[2], w[3]));
@@ -292,6 +300,8 @@ static double model_mass(int model, const double* p, const double* x) {
       return x[0] * lin_density(p[4]);
     case ORC_MODEL_FIXED_LENGTH:
       return x[0] * lin_density(p[3]);
+    case ORC_MODEL_UDF_MINIMAL: /* minimal.cpp:101-105 */
+      return x[0] * lin_density(p[4]);
     case ORC_MODEL_SIMPLE_ACETATE:
       return x[0] * lin_density(SA_D_M);
     default:
@@ -347,6 +357,16 @@ static int model_update(int model, const double* p, double dt, double* x, const 
       const double ldot = p[0] * g;
       x[0] = x[0] + dt * ldot;
       contrib[0] = -phi;
+      return (x[0] >= x[1]) ? ORC_DIVISION : ORC_IDLE;
+    }
+    case ORC_MODEL_UDF_MINIMAL: { /* minimal.cpp:65-83 */
+      const double s = c[0];
+      const double g = s / (p[3] + s);
+      const double phi_s_max = (p[1] * lin_density(p[4])) / p[0];
+      const double ldot = p[1] * g;
+      const double d_length = dt * ldot;
+      x[0] = x[0] + d_length / (1.0 + dt * ldot);
+      contrib[0] = -(phi_s_max * g);
       return (x[0] >= x[1]) ? ORC_DIVISION : ORC_IDLE;
     }
     case ORC_MODEL_SIMPLE_ACETATE: { /* simple_acetate.hpp:154-204 */
@@ -442,6 +462,14 @@ static void model_division(int model, const double* p, double* x, double* xd, ui
       x[1] = p[1];
       xd[0] = h;
       xd[1] = p[1];
+      break;
+    }
+    case ORC_MODEL_UDF_MINIMAL: { /* minimal.cpp:85-99 */
+      const double h = x[0] / 2.0;
+      x[0] = h;
+      x[1] = p[2];
+      xd[0] = h;
+      xd[1] = p[2];
       break;
     }
     case ORC_MODEL_SIMPLE_ACETATE: { /* simple_acetate.hpp:206-248 */

@@ -324,6 +324,42 @@ def test_gas_liquid_transfer_closed_form(oracle_lib):
         assert C[0, 0] == 1.5
 
 
+def test_udf_minimal_closed_form(oracle_lib):
+    """The example plugin's model (minimal.cpp:65-99) at constant substrate:
+    the length grows by the same dt*ldot/(1 + dt*ldot) every step, so it is
+    linear in the step count; each particle takes phi_s_max*g; at l_max the
+    cell divides into two halves that both keep l_max."""
+    p = [0.5, 2e-7, 2e-6, 1e-3, 0.6e-6]
+    r = one_box(oracle_lib, "udf_minimal")
+    r.set_params(p)
+    s = 0.7
+    r.set_concentrations(np.full((1, 1), s))
+    n, w = 30, 2.0
+    l0 = np.linspace(1.0e-6, 1.2e-6, n)
+    r.set_particles(2 * n, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([l0, np.full(n, 1.0)]),
+                    np.zeros(n), np.zeros(n), w)  # l_max = 1 m: no division
+    g = s / (p[3] + s)
+    ldot = p[1] * g
+    dl = 0.01 * ldot / (1 + 0.01 * ldot)
+    for k in range(1, 21):
+        r.step_stages(MODEL | CONTRIB)
+        np.testing.assert_allclose(r.particles()["state"][0], l0 + k * dl, rtol=1e-12)
+    lin = 1000 * math.pi * p[4] ** 2 / 4
+    np.testing.assert_allclose(r.contributions()[0, 0], -w * n * (p[1] * lin / p[0]) * g, rtol=1e-12)
+    # division: lengths just below l_max cross it in one step and halve
+    r2 = one_box(oracle_lib, "udf_minimal")
+    r2.set_params(p)
+    r2.set_concentrations(np.full((1, 1), s))
+    x0 = np.full(n, p[2] - 0.5 * dl)
+    r2.set_particles(2 * n, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([x0, np.full(n, p[2])]),
+                     np.zeros(n), np.zeros(n), w)
+    r2.step_stages(MODEL)
+    st = r2.particles()["state"]
+    assert r2.n == 2 * n
+    np.testing.assert_allclose(st[0], np.full(2 * n, (x0[0] + dl) / 2), rtol=1e-14)
+    np.testing.assert_array_equal(st[1], np.full(2 * n, p[2]))
+
+
 def test_time_dependent_feed_closed_form(oracle_lib):
     """One box, no particles, no outlet: a feed of flow q(t) at concentration
     c adds dt*q(k*dt)*c of mass at step k (t is the step's start), so
