@@ -5,7 +5,7 @@ model plugins for it.
     python biocma-mcst_amd/build.py --udf model.hpp --model Struct -o plugin.so
 
 The first builds the library, the C++ example driver and the example model
-plugin (examples/libudf_minimal.so); the second builds a plugin from a model
+plugins (examples/libudf_*.so); the second builds a plugin from a model
 struct (the contract of csrc/models.hpp), for mcst_model_load.
 
 Flags: -ffp-contract=off keeps every FP64 expression rounded as written (the
@@ -94,9 +94,10 @@ def build(verbose: bool = True) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("example driver failed to build:\n" + r.stderr)
-    # the example model plugin
-    build_udf(os.path.join(HERE, "examples", "udf_minimal.hpp"), "UdfMinimal",
-              os.path.join(HERE, "examples", "libudf_minimal.so"), verbose=False)
+    # the example model plugins
+    for hdr, model in (("udf_minimal.hpp", "UdfMinimal"), ("udf_starvation.hpp", "UdfStarvation")):
+        build_udf(os.path.join(HERE, "examples", hdr), model,
+                  os.path.join(HERE, "examples", "lib" + hdr.replace(".hpp", ".so")), verbose=False)
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -4,7 +4,7 @@
 // ModelType concept, apps/libs/mc/public/mc/traits.hpp:66-153):
 //   NV, NS, NP           state variables, liquid species, parameters
 //   init(p, x, seed, slot)                 host, initial state      (Model::init)
-//   record(p, c, r)                        device+host, per compartment
+//   record(p, c, r)                        device+host, per compartment: writes r[0..NS)
 //   update(p, dt, x, r, contrib) -> Status device+host               (Model::update)
 //   division(p, x, xd, key)                device+host              (Model::division)
 //   mass(p, x)                             device+host              (Model::mass)

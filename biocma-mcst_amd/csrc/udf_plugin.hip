@@ -8,6 +8,8 @@
 // MCST_UDF_MODEL the struct.  The struct follows the contract of models.hpp
 // (NAME, NV, NS, NP, NL, CAN_DIE, defaults, init, record, update, division,
 // mass); its ID member is not used: the library numbers loaded models itself.
+// record() writes exactly NS terms (r[0..NS)): the record row holds the move
+// probability and exit threshold right after them.
 #include "step_kernels.hpp"
 #include "udf.hpp"
 

@@ -140,6 +140,7 @@ def load_model(path):
 
 
 EXAMPLE_PLUGIN = os.path.join(_PKG, "examples", "libudf_minimal.so")
+EXAMPLE_PLUGIN_DEATH = os.path.join(_PKG, "examples", "libudf_starvation.so")
 
 
 def default_params(name):

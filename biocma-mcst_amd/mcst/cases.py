@@ -232,7 +232,7 @@ def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, s
     that divisions, moves and exits all happen within a few steps."""
     rng = np.random.default_rng(seed)
     n_s = {"monod_mass": 1, "dual_monod": 2, "monod": 1, "fixed_length": 1, "simple_acetate": 2,
-           "quad_monod": 4, "udf_minimal": 1}[model]
+           "quad_monod": 4, "udf_minimal": 1, "udf_starvation": 1}[model]
     if n_c > 1:
         edges = ring_with_chords(n_c, min(4, max(0, n_c * (n_c - 3) // 2)), rng) if n_c > 3 else \
             np.array([[i, (i + 1) % n_c] for i in range(n_c)])
@@ -259,8 +259,10 @@ def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, s
         p = np.array([0.3, 1e-3, 2.0, 2e-6, 0.6e-6, 3.0]) if fast else None
     elif model == "fixed_length":
         p = np.array([2e-7, 2e-6, 1e-3, 0.6e-6]) if fast else None
-    elif model == "udf_minimal":  # the example model plugin
+    elif model == "udf_minimal":  # the example model plugins
         p = np.array([0.5, 2e-7, 2e-6, 1e-3, 0.6e-6]) if fast else None
+    elif model == "udf_starvation":
+        p = np.array([0.5, 1.0, 0.5, 2.0, 10.0, 1e-15, 0.1]) if fast else None
     elif model == "quad_monod":
         p = np.array([20.0, 0.01, 1e-4, 0.01, 5.0, 0.5, 0.3, 0.3, 0.4, 0.3, 0.05, 0.4, 2e-15, 1.4e-15,
                       0.2e-15]) if fast else None

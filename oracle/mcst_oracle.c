@@ -27,7 +27,8 @@
  *   models ................ apps/libs/models/public/models/monod.hpp:80-193,
  *                           fixed_length.hpp:360-413, simple_acetate.hpp:132-248, utils.hpp:63-67
  *                           (monod_mass, dual_monod, quad_monod: the benchmark cases' models;
- *                           udf_minimal: apps/udf_model/minimal.cpp:23-107, the example plugin)
+ *                           udf_minimal: apps/udf_model/minimal.cpp:23-107, and udf_starvation,
+ *                           the example plugins; the latter's Dead status: mc/alias.hpp:124-130)
  *   gas-liquid transfer ... apps/libs/simulation/src/implScalar.cpp:228-248 (liquid side, fixed gas)
  *   export properties ..... apps/core/public/core/post_process.hpp:54-98
  *
@@ -180,9 +181,9 @@ static double trunc_normal(double mu, double sd, double lo, double hi, double u)
 /* Models.  State is an FP64 vector x[n_var]; c[] the local concentrations;
  * contrib[] the per-particle source term (kg/s) for each species. */
 
-static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9, 8, 2};
-static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2, 4, 1};
-static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0, 15, 5};
+static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9, 8, 2, 3};
+static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2, 4, 1, 1};
+static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0, 15, 5, 7};
 
 int orc_model_info(int model, int* n_var, int* n_species, int* n_params) {
   if (model < 0 || model >= ORC_MODEL_COUNT) return -1;
@@ -214,6 +215,9 @@ int orc_model_default_params(int model, double* p, int n) {
       break;
     case ORC_MODEL_UDF_MINIMAL: /* Y, l_dot_max, l_max, k, d_m (udf_model/minimal.cpp:26-35) */
       p[0] = 0.5; p[1] = 2e-6 / 3600.0; p[2] = 2e-6; p[3] = 1e-3; p[4] = 0.6e-6;
+      break;
+    case ORC_MODEL_UDF_STARVATION: /* mu_max, Ks, Y, division ratio, kd, m0 mean, m0 cv */
+      p[0] = 0.5 / 3600.0; p[1] = 0.01; p[2] = 0.5; p[3] = 2.0; p[4] = 1.0 / 3600.0; p[5] = 1e-15; p[6] = 0.1;
       break;
     default:
       break;
@@ -267,6 +271,13 @@ static void model_init(int model, const double* p, double* x, uint64_t seed, uin
       x[0] = p[2] * (0.5 + 0.5 * orc_u01(w[0]));
       x[1] = p[2];
       break;
+    case ORC_MODEL_UDF_STARVATION: {
+      const double m = trunc_normal(p[5], p[6] * p[5], 0.0, INFINITY, orc_u01_52(w[0], w[1]));
+      x[0] = m;
+      x[1] = m / (1.0 + orc_u01(w[2]));
+      x[2] = 0.2 + 0.8 * orc_u01(w[3]);
+      break;
+    }
     case ORC_MODEL_SIMPLE_ACETATE: { /* simple_acetate.hpp:132-152 */
       x[0] = trunc_normal(SA_L_MAX * 0.75, SA_L_MAX / 10.0, 0.7 * SA_L_MIN, SA_L_MAX * 1.3, orc_u01_52(w[0], w[1]));
       x[1] = trunc_normal(SA_L_MAX * 0.75, SA_L_MAX / 10.0, 0.7 * SA_L_MIN, SA_L_MAX * 1.3, orc_u01_52(w[2], w[3]));
@@ -295,6 +306,7 @@ static double model_mass(int model, const double* p, const double* x) {
     case ORC_MODEL_MONOD_MASS:
     case ORC_MODEL_DUAL_MONOD:
     case ORC_MODEL_QUAD_MONOD:
+    case ORC_MODEL_UDF_STARVATION:
       return x[0];
     case ORC_MODEL_MONOD:
       return x[0] * lin_density(p[4]);
@@ -358,6 +370,16 @@ static int model_update(int model, const double* p, double dt, double* x, const 
       x[0] = x[0] + dt * ldot;
       contrib[0] = -phi;
       return (x[0] >= x[1]) ? ORC_DIVISION : ORC_IDLE;
+    }
+    case ORC_MODEL_UDF_STARVATION: { /* viability falls at kd (1 - g); Dead at zero */
+      const double s = c[0] > 0.0 ? c[0] : 0.0;
+      const double g = s / (p[1] + s);
+      x[2] = x[2] - dt * (p[4] * (1.0 - g));
+      if (x[2] <= 0.0) return ORC_DEAD;
+      const double gm = (p[0] * g) * x[0];
+      contrib[0] = -(gm / p[2]);
+      x[0] = x[0] + dt * gm;
+      return (x[0] >= p[3] * x[1]) ? ORC_DIVISION : ORC_IDLE;
     }
     case ORC_MODEL_UDF_MINIMAL: { /* minimal.cpp:65-83 */
       const double s = c[0];
@@ -462,6 +484,16 @@ static void model_division(int model, const double* p, double* x, double* xd, ui
       x[1] = p[1];
       xd[0] = h;
       xd[1] = p[1];
+      break;
+    }
+    case ORC_MODEL_UDF_STARVATION: {
+      const double h = x[0] * 0.5;
+      x[0] = h;
+      x[1] = h;
+      x[2] = 1.0;
+      xd[0] = h;
+      xd[1] = h;
+      xd[2] = 1.0;
       break;
     }
     case ORC_MODEL_UDF_MINIMAL: { /* minimal.cpp:85-99 */

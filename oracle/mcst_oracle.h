@@ -25,7 +25,8 @@ enum orc_model {
   ORC_MODEL_FIXED_LENGTH = 3,
   ORC_MODEL_SIMPLE_ACETATE = 4,
   ORC_MODEL_QUAD_MONOD = 5,
-  ORC_MODEL_UDF_MINIMAL = 6, /* the example model plugin (the reference's minimal udf_model) */
+  ORC_MODEL_UDF_MINIMAL = 6,    /* example model plugins: the reference's minimal udf_model, */
+  ORC_MODEL_UDF_STARVATION = 7, /* and Monod growth with starvation death */
   ORC_MODEL_COUNT
 };
 

@@ -23,6 +23,7 @@ MODEL_IDS = {
     "simple_acetate": 4,
     "quad_monod": 5,
     "udf_minimal": 6,
+    "udf_starvation": 7,
 }
 FEED_TYPES = {"constant": 0, "linear": 1, "exponential": 2}
 COUNTERS = ["steps", "births", "exits", "deaths", "moves", "overflow", "compactions", "n", "inactive", "processed"]

@@ -42,11 +42,12 @@ def test_python_binding_covers_header():
 def test_model_registry_matches_oracle(lib, oracle_lib):
     import mcst
 
-    mcst.load_model(mcst.EXAMPLE_PLUGIN)  # udf_minimal: a plugin here, built in the oracle
+    mcst.load_model(mcst.EXAMPLE_PLUGIN)  # the example plugins' models: built into the oracle
+    mcst.load_model(mcst.EXAMPLE_PLUGIN_DEATH)
     for name in oracle_lib.MODEL_IDS:
         mid, nv, ns, npar = mcst.model_info(name)
         assert (nv, ns, npar) == oracle_lib.model_info(name)
-        if name != "udf_minimal":
+        if not name.startswith("udf_"):
             assert mid == oracle_lib.MODEL_IDS[name]
         np.testing.assert_array_equal(mcst.default_params(name), oracle_lib.default_params(name))
 

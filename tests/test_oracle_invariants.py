@@ -360,6 +360,36 @@ def test_udf_minimal_closed_form(oracle_lib):
     np.testing.assert_array_equal(st[1], np.full(2 * n, p[2]))
 
 
+def test_udf_starvation_death_times_and_contributions(oracle_lib):
+    """The dying example plugin's model at constant substrate: viability falls
+    by delta = dt*kd*(1 - g) per step, so a cell starting at (j + 1/2)*delta
+    dies at step j + 1; dead cells stop contributing, and the living ones
+    contribute w*mu*m/Y with m growing by (1 + dt*mu) per step."""
+    p = [0.5, 0.01, 0.5, 2.0, 3.0, 1e-15, 0.1]
+    s, dt, w = 1e-3, 0.01, 4.0
+    g = s / (p[1] + s)
+    mu, delta = p[0] * g, dt * (p[4] * (1 - g))
+    n = 40
+    j = np.arange(n) % 8
+    v0 = (j + 0.5) * delta
+    m0 = np.linspace(1e-15, 1.5e-15, n)
+    r = one_box(oracle_lib, "udf_starvation", dt=dt)
+    r.set_params(p)
+    r.set_concentrations(np.full((1, 1), s))
+    r.set_particles(2 * n, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([m0, np.ones(n), v0]),
+                    np.zeros(n), np.zeros(n), w)
+    r.set_compaction(1.0, 0)  # keep dead slots in place
+    for k in range(1, 10):
+        r.step_stages(MODEL | CONTRIB)
+        alive = j >= k
+        assert r.counters()["deaths"] == int(np.sum(~alive))
+        st = r.particles()["status"]
+        np.testing.assert_array_equal(st == 3, ~alive)
+        m_prev = m0 * (1 + dt * mu) ** (k - 1)
+        want = -w * np.sum(mu * m_prev[alive] / p[2]) if alive.any() else 0.0
+        np.testing.assert_allclose(r.contributions()[0, 0], want, rtol=1e-12)
+
+
 def test_time_dependent_feed_closed_form(oracle_lib):
     """One box, no particles, no outlet: a feed of flow q(t) at concentration
     c adds dt*q(k*dt)*c of mass at step k (t is the step's start), so

@@ -1,7 +1,8 @@
-"""A model loaded from a plugin (the example udf_minimal, the reference's
-minimal udf_model) runs the same kernels as a built-in one: parity with the
-oracle's restatement of the model, and the same results when the plugin is
-named through BIOMC_LIB_UDF as "udf_model"."""
+"""Models loaded from plugins (the example udf_minimal, the reference's
+minimal udf_model, and udf_starvation, whose cells die) run the same kernels
+as built-in ones: parity with the oracle's restatements of the models, and
+the same results when a plugin is named through BIOMC_LIB_UDF as
+"udf_model"."""
 import os
 import subprocess
 import sys
@@ -28,6 +29,26 @@ def test_plugin_model_parity():
         assert_parity(g, o)
     cnt = g.counters()
     assert cnt["moves"] > 0 and cnt["exits"] > 0 and cnt["births"] > 0
+    eng.close()
+
+
+def test_plugin_model_with_deaths_parity():
+    """The dying example plugin (udf_starvation) on the small network with a
+    third of the compartments starved: deaths, removal of dead and exited
+    cells, divisions and moves, against the oracle."""
+    import mcst
+    from mcst import cases
+
+    mcst.load_model(mcst.EXAMPLE_PLUGIN_DEATH)
+    c = cases.small_case("udf_starvation", n_particles=6000, seed=5)
+    c.C0[0, ::3] = 0.0
+    eng, g, o = make_pair(c)
+    for _ in range(6):
+        eng.step(5)
+        o.step(5)
+        assert_parity(g, o)
+    cnt = g.counters()
+    assert cnt["deaths"] > 0 and cnt["compactions"] > 0 and cnt["births"] > 0 and cnt["moves"] > 0
     eng.close()
 
 
