@@ -95,8 +95,11 @@ UdfRegistry& udf_registry() {
 // again returns the entry of the first load.
 int load_plugin(UdfRegistry& u, const char* path, const ModelEntry** out) {
   if (!path || !*path) return fail(MCST_ERR_ARG, "empty plugin path");
+  char* real = realpath(path, nullptr);  // one file under several names is one plugin
+  const std::string key = real ? real : path;
+  std::free(real);
   for (size_t k = 0; k < u.paths.size(); ++k)
-    if (u.paths[k] == path) {
+    if (u.paths[k] == key) {
       *out = &u.entries[k];
       return MCST_OK;
     }
@@ -139,7 +142,7 @@ int load_plugin(UdfRegistry& u, const char* path, const ModelEntry** out) {
     return fail(MCST_ERR_ARG, std::string("model plugin ") + path + " " + why);
   }
   u.names.emplace_back(d.name);
-  u.paths.emplace_back(path);
+  u.paths.emplace_back(key);
   u.ops.push_back(d.ops);
   u.entries.push_back(ModelEntry{u.names.back().c_str(), UDF_ID_BASE + (int)u.entries.size(), d.nv, d.ns, d.np,
                                  d.nl, d.can_die, d.defaults, d.init, d.mass, &u.ops.back()});
@@ -1591,11 +1594,18 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
   HIP_TRY(hipSetDevice(e->device));
   constexpr int K = 6;
-  std::vector<hipEvent_t> ev((size_t)n_steps * (K + 1));
-  for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
-  hipEvent_t t0, t1;
-  HIP_TRY(hipEventCreate(&t0));
-  HIP_TRY(hipEventCreate(&t1));
+  // every event created here is destroyed on every return path
+  struct Events {
+    std::vector<hipEvent_t> v;
+    ~Events() {
+      for (auto& x : v)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } events;
+  events.v.assign((size_t)n_steps * (K + 1) + 2, nullptr);
+  for (auto& x : events.v) HIP_TRY(hipEventCreate(&x));
+  hipEvent_t* ev = events.v.data();
+  hipEvent_t t0 = events.v[events.v.size() - 2], t1 = events.v.back();
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipEventRecord(t0, e->stream));
   const uint32_t nv = (uint32_t)e->model->nv;
@@ -1639,9 +1649,6 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
       per_kernel_ms[k] = n_steps ? acc / n_steps : 0.0;
     }
   }
-  for (auto& x : ev) (void)hipEventDestroy(x);
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   return MCST_OK;
 }
 
