@@ -35,12 +35,20 @@ def _same(a, b):
         close(np.sort(getattr(a, f)()), np.sort(getattr(b, f)()), RTOL, f)
 
 
-@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "simple_acetate"])
+@pytest.mark.parametrize("model", ["monod_mass", "dual_monod", "simple_acetate", "udf_starvation", "timefeed"])
 def test_restored_run_continues_exactly(model):
+    """Also a plugin model whose cells die, and a run whose outlet is led by a
+    linear feed (its flows follow the restored step count)."""
     import mcst
     from mcst import cases
 
-    c = cases.small_case(model, n_c=12, n_particles=6000, seed=21)
+    if model == "udf_starvation":
+        mcst.load_model(mcst.EXAMPLE_PLUGIN_DEATH)
+    c = cases.small_case("dual_monod" if model == "timefeed" else model, n_c=12, n_particles=6000, seed=21)
+    if model == "udf_starvation":
+        c.C0[0, ::3] = 0.0
+    if model == "timefeed":
+        c.feeds = [("linear", 1e-3, 0.02, 3, 9, [0, 1], [8.0, 0.5]), ("constant", 2e-3, 0.0, 0, 6, [0], [10.0])]
     c.capacity = 60000
     eng, g, o = make_pair(c)
     eng.step(31)  # mid flowmap cycle (3 steps per map, 2 maps), after divisions and exits
