@@ -327,6 +327,8 @@ struct HostRun {
   uint32_t probe_cap = 0;
   uint32_t step_target = 0xFFFFFFFFu;
   uint32_t sort_interval = 0;    // reorder the particles by compartment every k steps (0: never)
+  uint32_t grow_every = 0;       // capacity check every k steps (0: fixed capacity)
+  double grow_factor = 2.5;      // ... new capacity = ceil(n * factor)
   uint32_t host_step = 0;        // the run's step count as the device will have it
   // initial particles
   bool has_particles = false;
@@ -1039,6 +1041,162 @@ int sort_run(mcst_engine* e, uint32_t r) {
   return MCST_OK;
 }
 
+// Replaces the engine-wide array *p (in e->allocs) by one of new_bytes
+// holding the first copy_bytes of the old one; the rest is zero.
+int dev_realloc(mcst_engine* e, void** p, size_t new_bytes, size_t copy_bytes) {
+  void* q = nullptr;
+  const hipError_t err = hipMalloc(&q, new_bytes);
+  if (err != hipSuccess) return fail(MCST_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(err));
+  HIP_TRY(hipMemsetAsync(q, 0, new_bytes, e->stream));
+  if (copy_bytes) HIP_TRY(hipMemcpyAsync(q, *p, copy_bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (auto& a : e->allocs)
+    if (a == *p) a = q;
+  HIP_TRY(hipFree(*p));
+  *p = q;
+  return MCST_OK;
+}
+
+// Grows the particle pool to at least `min_tiles` tiles: the particle arrays
+// keep their slots (the state's field stride becomes the new pool size),
+// the per-step lists and staging rows are reallocated empty.
+int grow_pool(mcst_engine* e, uint64_t min_tiles) {
+  BatchDev& b = e->bd;
+  const uint64_t T0 = b.T_max, P0 = b.P;
+  const uint64_t T1 = std::max<uint64_t>(min_tiles, 2 * T0), P1 = T1 * TILE;
+  if (P1 >= 0xFFFFFFFFull) return fail(MCST_ERR_MEMORY, "more than 2^32-1 particle slots in one engine");
+  const uint32_t nv = (uint32_t)e->model->nv;
+  int rc;
+  if ((rc = dev_realloc(e, (void**)&b.pos, P1 * sizeof(uint32_t), P0 * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.status, P1, P0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.age_hyd, P1 * sizeof(double), P0 * sizeof(double)))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.age_div, P1 * sizeof(double), P0 * sizeof(double)))) return rc;
+  {  // state [nv][P]: re-strided field by field
+    double* q = nullptr;
+    const hipError_t err = hipMalloc((void**)&q, P1 * nv * sizeof(double));
+    if (err != hipSuccess) return fail(MCST_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(err));
+    HIP_TRY(hipMemsetAsync(q, 0, P1 * nv * sizeof(double), e->stream));
+    for (uint32_t v = 0; v < nv; ++v)
+      HIP_TRY(hipMemcpyAsync(q + (uint64_t)v * P1, b.state + (uint64_t)v * P0, P0 * sizeof(double),
+                             hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (auto& a : e->allocs)
+      if (a == (void*)b.state) a = q;
+    HIP_TRY(hipFree(b.state));
+    b.state = q;
+  }
+  const uint64_t ST1 = T1 * b.stage_stride;
+  if ((rc = dev_realloc(e, (void**)&b.st_state, ST1 * nv * sizeof(double), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.st_pos, ST1 * sizeof(uint32_t), 0))) return rc;
+  uint32_t** tile_arrays[] = {&b.tile_births, &b.tile_birth_off, &b.tile_gap, &b.tile_fill, &b.tile_gap_off,
+                              &b.tile_fill_off};
+  for (uint32_t** a : tile_arrays)
+    if ((rc = dev_realloc(e, (void**)a, T1 * sizeof(uint32_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.fill_src, P1 * sizeof(uint32_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.move_slot, P1 * sizeof(uint16_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.move_pos, P1 * sizeof(uint32_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.div_slot, P1 * sizeof(uint16_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.div_pos, P1 * sizeof(uint32_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.sub_moves, T1 * b.nsub * sizeof(uint32_t), 0))) return rc;
+  if ((rc = dev_realloc(e, (void**)&b.sub_divs, T1 * b.nsub * sizeof(uint32_t), 0))) return rc;
+  b.P = P1;
+  b.T_max = (uint32_t)T1;
+  b.ST = ST1;
+  give_tiles(e, (uint32_t)T0, (uint32_t)(T1 - T0));
+  return MCST_OK;
+}
+
+// Takes `n` free tiles starting exactly at tile `at`; false if they are not free.
+bool take_tiles_at(mcst_engine* e, uint32_t at, uint32_t n) {
+  if (n == 0) return true;
+  for (size_t i = 0; i < e->free_tiles.size(); ++i) {
+    auto& f = e->free_tiles[i];
+    if (f.first != at || f.second < n) continue;
+    f.first += n;
+    f.second -= n;
+    if (f.second == 0) e->free_tiles.erase(e->free_tiles.begin() + (long)i);
+    return true;
+  }
+  return false;
+}
+
+// Gives run r (n particles) capacity new_cap: in place when the tiles after
+// its range are free, else in a new range (taken from the pool, grown if
+// needed) to which its particles are copied.
+int regrow_run(mcst_engine* e, uint32_t r, uint64_t n, uint64_t new_cap) {
+  HostRun& h = e->runs[r];
+  const uint32_t start0 = (uint32_t)(h.p_off / TILE), tiles0 = h.tiles;
+  const uint32_t tiles1 = (uint32_t)((new_cap + TILE - 1) / TILE);
+  uint32_t start1 = start0;
+  if (tiles1 > tiles0 && !take_tiles_at(e, start0 + tiles0, tiles1 - tiles0)) {
+    if (!take_tiles(e, tiles1, start1)) {
+      int rc = grow_pool(e, (uint64_t)e->bd.T_max + tiles1);
+      if (rc) return rc;
+      if (!take_tiles(e, tiles1, start1)) return fail(MCST_ERR_MEMORY, "particle pool full after growing it");
+    }
+    BatchDev& b = e->bd;
+    const uint64_t o0 = h.p_off, o1 = (uint64_t)start1 * TILE;
+    HIP_TRY(hipMemcpyAsync(b.pos + o1, b.pos + o0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(b.status + o1, b.status + o0, n, hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(b.age_hyd + o1, b.age_hyd + o0, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(b.age_div + o1, b.age_div + o0, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    for (uint32_t v = 0; v < (uint32_t)e->model->nv; ++v)
+      HIP_TRY(hipMemcpyAsync(b.state + (uint64_t)v * b.P + o1, b.state + (uint64_t)v * b.P + o0, n * sizeof(double),
+                             hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    give_tiles(e, start0, tiles0);
+  }
+  h.p_off = (uint64_t)start1 * TILE;
+  h.tiles = std::max(tiles0, tiles1);
+  h.cap = new_cap;
+  RunDev& d = e->h_runs[r];
+  d.p_off = h.p_off;
+  d.cap = (uint32_t)new_cap;
+  d.tile_cap_off = start1;
+  HIP_TRY(hipMemcpy(e->d_runs + r, &d, sizeof(RunDev), hipMemcpyHostToDevice));
+  return MCST_OK;
+}
+
+// Container growth (merge_buffer / _resize, particles_container.hpp:575-643,
+// whose allocation factor is 2.5): before every grow_every-th step, a run
+// with fewer free slots than particles gets capacity ceil(n * grow_factor).
+// The checks fall on fixed steps, so results do not depend on timing (the
+// division budget depends on the free capacity); the oracle applies the
+// same rule.
+int grow_runs(mcst_engine* e) {
+  bool any = false;
+  for (auto& h : e->runs)
+    if (h.phase == 1 && h.grow_every && h.host_step < h.step_target && h.host_step % h.grow_every == 0) any = true;
+  if (!any) return MCST_OK;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  bool grown = false;
+  for (uint32_t r = 0; r < e->runs.size(); ++r) {
+    HostRun& h = e->runs[r];
+    if (h.phase != 1 || !h.grow_every || h.host_step >= h.step_target || h.host_step % h.grow_every) continue;
+    RunState st;
+    HIP_TRY(hipMemcpy(&st, e->d_rs + r, sizeof(RunState), hipMemcpyDeviceToHost));
+    const uint64_t n = st.n;
+    if (h.cap - n >= n) continue;
+    const uint64_t new_cap = (uint64_t)std::ceil((double)n * h.grow_factor);
+    if (new_cap <= h.cap) continue;
+    if (new_cap >= 0xFFFFFFFFull) return fail(MCST_ERR_MEMORY, "run capacity above 2^32-1 slots");
+    const int rc = regrow_run(e, r, n, new_cap);
+    if (rc) return rc;
+    grown = true;
+  }
+  if (grown) {
+    drop_graph(e);
+    return refresh_tables(e);
+  }
+  return MCST_OK;
+}
+
+bool has_growth(const mcst_engine* e) {
+  for (const auto& h : e->runs)
+    if (h.phase == 1 && h.grow_every) return true;
+  return false;
+}
+
 // The host's mirror of the device step counters: k steps of the batch
 // advance every live run that has not reached its step target.
 void count_steps(mcst_engine* e, uint64_t k) {
@@ -1079,6 +1237,8 @@ bool has_time_feeds(const mcst_engine* e) {
 
 // One step of the batch, then the reorders that fall due.
 int step_once(mcst_engine* e, uint32_t stages) {
+  const int rcg = grow_runs(e);
+  if (rcg) return rcg;
   const int rcf = prepare_feeds(e);
   if (rcf) return rcf;
   std::vector<uint32_t> before(e->runs.size());
@@ -1296,6 +1456,17 @@ int mcst_run_sort_particles(mcst_engine* e, int32_t run) {
   CHECK_READY(e, run);
   HIP_TRY(hipSetDevice(e->device));
   return sort_run(e, (uint32_t)run);
+}
+
+int mcst_run_set_growth(mcst_engine* e, int32_t run, uint32_t every, double factor) {
+  CHECK_ENGINE(e);
+  HostRun* h = get_run(e, run);
+  if (!h) return fail(MCST_ERR_ARG, "bad run id");
+  if (h->phase == 2) return fail(MCST_ERR_STATE, "run retired");
+  if (every && !(factor > 1.0 && factor <= 16.0)) return fail(MCST_ERR_ARG, "growth factor must be in (1, 16]");
+  h->grow_every = every;
+  if (every) h->grow_factor = factor;
+  return MCST_OK;
 }
 
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap) {
@@ -1516,6 +1687,7 @@ int mcst_engine_capture(mcst_engine* e, uint32_t n_steps) {
   if (n_steps == 0) return fail(MCST_ERR_ARG, "capture at least one step");
   if (has_time_feeds(e))
     return fail(MCST_ERR_STATE, "a run has time-dependent feeds: their flows are set by the host before each step");
+  if (has_growth(e)) return fail(MCST_ERR_STATE, "a run grows its capacity: the host checks it between steps");
   HIP_TRY(hipSetDevice(e->device));
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (e->graph_exec) HIP_TRY(hipGraphExecDestroy(e->graph_exec));

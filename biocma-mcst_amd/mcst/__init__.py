@@ -35,7 +35,7 @@ EXPORTS = [
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
     "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
-    "mcst_engine_get_steps", "mcst_model_load",
+    "mcst_engine_get_steps", "mcst_model_load", "mcst_run_set_growth",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -65,6 +65,7 @@ def lib():
                                   C.POINTER(C.c_int)],
             "mcst_model_default_params": [C.c_int, _f64p, C.c_int],
             "mcst_model_load": [C.c_char_p, C.POINTER(C.c_int)],
+            "mcst_run_set_growth": [_vp, i32, u32, dbl],
             "mcst_engine_create": [C.c_char_p, C.c_int, C.POINTER(_vp)],
             "mcst_engine_destroy": [C.POINTER(_vp)],
             "mcst_run_create": [_vp, u32, u32, dbl, u64, C.POINTER(i32)],
@@ -278,6 +279,11 @@ class Run:
 
     def set_sort_interval(self, steps):
         _ck(lib().mcst_run_set_sort_interval(self._h, self.id, steps), "set_sort_interval")
+
+    def set_growth(self, every, factor=2.5):
+        """Grow the capacity to ceil(n*factor) when fewer slots than particles
+        are free, checked before every `every`-th step (0: fixed capacity)."""
+        _ck(lib().mcst_run_set_growth(self._h, self.id, every, factor), "set_growth")
 
     def sort_particles(self):
         _ck(lib().mcst_run_sort_particles(self._h, self.id), "sort_particles")

@@ -152,6 +152,17 @@ int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, co
  * statistics do not depend on it. */
 int mcst_run_set_sort_interval(mcst_engine* e, int32_t run, uint32_t steps);
 int mcst_run_sort_particles(mcst_engine* e, int32_t run);
+/* Container growth (merge_buffer / _resize, particles_container.hpp:575-643,
+ * whose allocation factor is 2.5): before every `every`-th step of the run
+ * (counted from its step 0), a run with fewer free slots than particles gets
+ * capacity ceil(n * factor), factor in (1, 16]; its particles keep their
+ * slots, moving to a new range of the engine's pool (grown if needed) when
+ * the slots after them are taken.  The checks fall on fixed steps, so
+ * results do not depend on timing; each check waits for the stream.  Between
+ * checks a full run refuses divisions (Overflow), as with a fixed capacity.
+ * every = 0 (the default) keeps the capacity fixed.  An engine with a growing
+ * run cannot capture a graph; mcst_engine_profile steps skip the checks. */
+int mcst_run_set_growth(mcst_engine* e, int32_t run, uint32_t every, double factor);
 /* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
 /* Keep up to `cap` exit ages and division ages (the reference's probes). */

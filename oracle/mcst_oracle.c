@@ -807,6 +807,8 @@ struct orc_run {
   double ratio;
   uint32_t stage_cap;
   uint32_t sort_interval; /* reorder by compartment every k steps (0: never) */
+  uint32_t grow_every;    /* capacity check every k steps (0: fixed capacity) */
+  double grow_factor;     /* new capacity = ceil(n * grow_factor) */
   uint64_t counters[ORC_C_COUNT];
   dvec exit_ages, div_ages;
 };
@@ -980,6 +982,40 @@ int orc_sort_particles(orc_run* r) { return sort_by_compartment(r); }
 
 int orc_set_sort_interval(orc_run* r, uint32_t steps) {
   r->sort_interval = steps;
+  return 0;
+}
+
+/* Container growth (merge_buffer / _resize, particles_container.hpp:575-643,
+ * allocation factor 2.5), checked on fixed steps: before every
+ * grow_every-th step, a run with fewer free slots than particles gets
+ * capacity ceil(n * grow_factor).  Slots keep their particles. */
+int orc_set_growth(orc_run* r, uint32_t every, double factor) {
+  if (every && !(factor > 1.0 && factor <= 16.0)) return -1;
+  r->grow_every = every;
+  r->grow_factor = factor;
+  return 0;
+}
+
+static int grow_capacity(orc_run* r, uint64_t cap) {
+  if (cap > 0xFFFFFFFFull) return -1;
+  const uint64_t n = r->n, old = r->cap;
+  uint32_t* pos = (uint32_t*)calloc(cap, sizeof(uint32_t));
+  uint8_t* st = (uint8_t*)calloc(cap, 1);
+  double* x = (double*)calloc((size_t)cap * r->n_var, sizeof(double));
+  double* ah = (double*)calloc(cap, sizeof(double));
+  double* ad = (double*)calloc(cap, sizeof(double));
+  if (!pos || !st || !x || !ah || !ad) {
+    free(pos), free(st), free(x), free(ah), free(ad);
+    return -2;
+  }
+  memcpy(pos, r->pos, n * sizeof(uint32_t));
+  memcpy(st, r->status, n);
+  memcpy(ah, r->age_hyd, n * sizeof(double));
+  memcpy(ad, r->age_div, n * sizeof(double));
+  for (int v = 0; v < r->n_var; ++v) memcpy(x + (size_t)v * cap, r->state + (size_t)v * old, n * sizeof(double));
+  free(r->pos), free(r->status), free(r->state), free(r->age_hyd), free(r->age_div);
+  r->pos = pos, r->status = st, r->state = x, r->age_hyd = ah, r->age_div = ad;
+  r->cap = cap;
   return 0;
 }
 
@@ -1176,6 +1212,10 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
 #endif
   for (uint32_t it = 0; it < n_steps; ++it) {
     const uint64_t step = r->counters[ORC_C_STEPS];
+    if (r->grow_every && step % r->grow_every == 0 && r->cap - r->n < r->n) {
+      const uint64_t cap = (uint64_t)ceil((double)r->n * r->grow_factor);
+      if (cap > r->cap && grow_capacity(r, cap)) return -2;
+    }
     const flowmap* f = &r->maps[(step / r->steps_per_map) % r->n_maps];
     const double t = (double)step * r->dt;
     /* update_feed (simulation.model.cpp:72-122).  Feed terms of one

@@ -105,6 +105,7 @@ int orc_set_stage_cap(orc_run* r, uint32_t cap);
 int orc_set_gas_transfer(orc_run* r, const double* kla, const double* cstar);
 int orc_set_sort_interval(orc_run* r, uint32_t steps);
 int orc_sort_particles(orc_run* r);
+int orc_set_growth(orc_run* r, uint32_t every, double factor);
 int orc_init_particles(orc_run* r, uint64_t n, uint64_t capacity, double x0);
 int orc_set_particles(orc_run* r, uint64_t n, uint64_t capacity, const uint32_t* pos,
                       const uint8_t* status, const double* state, const double* age_hyd,

@@ -96,6 +96,7 @@ def lib():
         L.orc_set_gas_transfer.argtypes = [C.c_void_p, _f64p, _f64p]
         L.orc_set_sort_interval.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_sort_particles.argtypes = [C.c_void_p]
+        L.orc_set_growth.argtypes = [C.c_void_p, C.c_uint32, C.c_double]
         for name in ("orc_get_exit_ages", "orc_get_division_ages"):
             getattr(L, name).restype = C.c_uint64
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
@@ -212,6 +213,9 @@ class OracleRun:
 
     def set_sort_interval(self, steps):
         _check(lib().orc_set_sort_interval(self.h, steps), "set_sort_interval")
+
+    def set_growth(self, every, factor=2.5):
+        _check(lib().orc_set_growth(self.h, every, factor), "set_growth")
 
     def sort_particles(self):
         _check(lib().orc_sort_particles(self.h), "sort_particles")

@@ -390,6 +390,30 @@ def test_udf_starvation_death_times_and_contributions(oracle_lib):
         np.testing.assert_allclose(r.contributions()[0, 0], want, rtol=1e-12)
 
 
+def test_container_growth_doubles_an_all_dividing_population(oracle_lib):
+    """Every cell divides every step (division ratio 0, a 1024-division cap
+    per tile): with a capacity check before every step (grow to 2.5 n when
+    fewer slots than cells are free) nothing is refused and n doubles each
+    step; with the capacity fixed at the initial count nothing is born."""
+    for every in (1, 0):
+        r = one_box(oracle_lib)
+        r.set_params([0.5, 0.01, 0.5, 0.0, 1e-15, 0.1])
+        r.set_concentrations(np.full((1, 1), 1.0))
+        r.init_particles(1000, 1000, 1.0)
+        r.set_stage_cap(1024)
+        if every:
+            r.set_growth(every, 2.5)
+        for k in range(1, 7):
+            r.step_stages(MODEL)
+            c = r.counters()
+            if every:
+                assert r.n == 1000 * 2 ** k and c["overflow"] == 0
+            else:
+                assert r.n == 1000 and c["overflow"] == 1000 * k
+    p = r.particles()
+    assert p["state"].shape[1] == 1000
+
+
 def test_time_dependent_feed_closed_form(oracle_lib):
     """One box, no particles, no outlet: a feed of flow q(t) at concentration
     c adds dt*q(k*dt)*c of mass at step k (t is the step's start), so
