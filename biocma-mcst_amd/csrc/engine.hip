@@ -971,18 +971,31 @@ int retire(mcst_engine* e, uint32_t r) {
   return rc;
 }
 
-void launch_step(mcst_engine* e, uint32_t stages) {
+// The kernels of one step of the batch.  With `ev`, records ev[0] before
+// the transport and ev[k] after the k-th stage (transport, particle pass,
+// resolve, bookkeeping, removal, merge) for mcst_engine_profile.
+void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr) {
   if (e->bd.R == 0) return;  // no run admitted yet
   BatchDev b = e->bd;
   b.stages = stages;
   const uint32_t nv = (uint32_t)e->model->nv;
   const ModelOps& ops = *e->model->ops;
+  auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev[k], e->stream);
+  };
+  mark(0);
   ops.transport(b, e->grid_transport, e->stream);
+  mark(1);
   ops.particle(b, e->plaunch, e->stream);
+  mark(2);
   ops.resolve(b, e->grid_move, e->stream);
+  mark(3);
   launch_bookkeep(b, e->stream);
+  mark(4);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
+  mark(5);
   launch_merge(b, nv, e->grid_particle, e->stream);
+  mark(6);
 }
 
 int read_state(mcst_engine* e, int32_t run, RunState& s) {
@@ -1236,14 +1249,14 @@ bool has_time_feeds(const mcst_engine* e) {
 }
 
 // One step of the batch, then the reorders that fall due.
-int step_once(mcst_engine* e, uint32_t stages) {
+int step_once(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr) {
   const int rcg = grow_runs(e);
   if (rcg) return rcg;
   const int rcf = prepare_feeds(e);
   if (rcf) return rcf;
   std::vector<uint32_t> before(e->runs.size());
   for (size_t r = 0; r < e->runs.size(); ++r) before[r] = e->runs[r].host_step;
-  launch_step(e, stages);
+  launch_step(e, stages, ev);
   count_steps(e, 1);
   for (size_t r = 0; r < e->runs.size(); ++r) {
     const HostRun& h = e->runs[r];
@@ -1780,32 +1793,15 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   hipEvent_t t0 = events.v[events.v.size() - 2], t1 = events.v.back();
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipEventRecord(t0, e->stream));
-  const uint32_t nv = (uint32_t)e->model->nv;
-  for (uint32_t s = 0; s < n_steps && e->bd.R; ++s) {
-    hipEvent_t* x = &ev[(size_t)s * (K + 1)];
-    if (has_time_feeds(e)) {
-      const int rc = prepare_feeds(e);
-      if (rc) return rc;
-      count_steps(e, 1);  // the next step's flows need the count of this one
-    }
-    BatchDev b = e->bd;
-    HIP_TRY(hipEventRecord(x[0], e->stream));
-    e->model->ops->transport(b, e->grid_transport, e->stream);
-    HIP_TRY(hipEventRecord(x[1], e->stream));
-    e->model->ops->particle(b, e->plaunch, e->stream);
-    HIP_TRY(hipEventRecord(x[2], e->stream));
-    e->model->ops->resolve(b, e->grid_move, e->stream);
-    HIP_TRY(hipEventRecord(x[3], e->stream));
-    launch_bookkeep(b, e->stream);
-    HIP_TRY(hipEventRecord(x[4], e->stream));
-    if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
-    HIP_TRY(hipEventRecord(x[5], e->stream));
-    launch_merge(b, nv, e->grid_particle, e->stream);
-    HIP_TRY(hipEventRecord(x[6], e->stream));
+  // the same steps as mcst_engine_step (capacity checks, time-dependent
+  // flows, reorders), with events between the kernels
+  const uint32_t steps = e->bd.R ? n_steps : 0;
+  for (uint32_t s = 0; s < steps; ++s) {
+    const int rc = step_once(e, STAGE_ALL, &ev[(size_t)s * (K + 1)]);
+    if (rc) return rc;
   }
   HIP_TRY(hipEventRecord(t1, e->stream));
   HIP_TRY(hipGetLastError());
-  if (!has_time_feeds(e)) count_steps(e, n_steps);
   HIP_TRY(hipEventSynchronize(t1));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
@@ -1813,12 +1809,12 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   if (per_kernel_ms) {
     for (int k = 0; k < K; ++k) {
       double acc = 0.0;
-      for (uint32_t s = 0; s < n_steps; ++s) {
+      for (uint32_t s = 0; s < steps; ++s) {
         float d = 0.f;
         HIP_TRY(hipEventElapsedTime(&d, ev[(size_t)s * (K + 1) + k], ev[(size_t)s * (K + 1) + k + 1]));
         acc += d;
       }
-      per_kernel_ms[k] = n_steps ? acc / n_steps : 0.0;
+      per_kernel_ms[k] = steps ? acc / steps : 0.0;
     }
   }
   return MCST_OK;

@@ -161,7 +161,7 @@ int mcst_run_sort_particles(mcst_engine* e, int32_t run);
  * results do not depend on timing; each check waits for the stream.  Between
  * checks a full run refuses divisions (Overflow), as with a fixed capacity.
  * every = 0 (the default) keeps the capacity fixed.  An engine with a growing
- * run cannot capture a graph; mcst_engine_profile steps skip the checks. */
+ * run cannot capture a graph. */
 int mcst_run_set_growth(mcst_engine* e, int32_t run, uint32_t every, double factor);
 /* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
@@ -207,12 +207,14 @@ int mcst_engine_step(mcst_engine* e, uint32_t n_steps);
 /* One step restricted to the MCST_STAGE_* bits in `stages`. */
 int mcst_engine_step_stages(mcst_engine* e, uint32_t stages);
 /* Capture `n_steps` steps into a HIP graph (replaces any earlier one).
- * MCST_ERR_STATE while a live run has a time-dependent feed. */
+ * MCST_ERR_STATE while a live run has a time-dependent feed or grows its
+ * capacity (mcst_run_set_growth). */
 int mcst_engine_capture(mcst_engine* e, uint32_t n_steps);
 /* Replay the captured graph `times` times (times * n_steps steps). */
 int mcst_engine_replay(mcst_engine* e, uint32_t times);
 int mcst_engine_synchronize(mcst_engine* e);
-/* Time n steps with HIP events on the engine stream: total ms and the mean
+/* Take n steps, as mcst_engine_step, timed with HIP events on the engine
+ * stream: total ms and the mean
  * duration of each step kernel, in the order transport (with the tile
  * layout), particle pass, mover and division resolution, bookkeeping, removal (4 kernels
  * together, skipped when no particle can become inactive), merge
