@@ -161,6 +161,23 @@ def test_rgg5000_full_size_few_steps():
     assert_parity(g, o, probes=False)
 
 
+def test_rgg5000_all_steps():
+    """The 5000-compartment network at 1e7 particles as specified: all 1000
+    steps (20 flowmaps switched every 100 steps), compared every 250 steps
+    (the oracle on 16 threads)."""
+    from mcst import cases
+
+    c = cases.rgg5000(n_particles=10_000_000)
+    eng, g, o = make_pair(c)
+    for _ in range(4):
+        eng.step(250)
+        o.step(250, threads=16)
+        assert_parity(g, o, probes=False)
+    cnt = g.counters()
+    assert cnt["steps"] == 1000 and cnt["births"] > 0 and cnt["moves"] > 0
+    eng.close()
+
+
 def test_batch_of_runs_equals_runs_alone():
     """A ragged batch: every run of the batch matches its own oracle."""
     import mcst
