@@ -326,6 +326,25 @@ def test_lattice50k_reduced():
     eng.close()
 
 
+def test_lattice50k_1e7_all_steps():
+    """The lattice tank at 1e7 particles (growing to about 2.3e7) for all 200
+    steps: the windowed LDS accumulator (the run's 4 x 50 000 contributions
+    do not fit in LDS) at a realistic density, compared at steps 100 and 200
+    (the oracle on 16 threads)."""
+    from mcst import cases
+
+    c = cases.lattice50k(n_particles=10_000_000)
+    c.capacity = 30_000_000
+    eng, g, o = make_pair(c)
+    for _ in range(2):
+        eng.step(100)
+        o.step(100, threads=16)
+        assert_parity(g, o, probes=False)
+    cnt = g.counters()
+    assert cnt["steps"] == 200 and cnt["exits"] > 0 and cnt["births"] > 0 and cnt["compactions"] > 0
+    eng.close()
+
+
 @pytest.mark.parametrize("model", ["monod_mass", "quad_monod"])
 def test_periodic_reorder_parity(model):
     """Reordering by compartment every 5 steps on the device (radix sort of
