@@ -1054,64 +1054,72 @@ int sort_run(mcst_engine* e, uint32_t r) {
   return MCST_OK;
 }
 
-// Replaces the engine-wide array *p (in e->allocs) by one of new_bytes
-// holding the first copy_bytes of the old one; the rest is zero.
-int dev_realloc(mcst_engine* e, void** p, size_t new_bytes, size_t copy_bytes) {
-  void* q = nullptr;
-  const hipError_t err = hipMalloc(&q, new_bytes);
-  if (err != hipSuccess) return fail(MCST_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(err));
-  HIP_TRY(hipMemsetAsync(q, 0, new_bytes, e->stream));
-  if (copy_bytes) HIP_TRY(hipMemcpyAsync(q, *p, copy_bytes, hipMemcpyDeviceToDevice, e->stream));
-  HIP_TRY(hipStreamSynchronize(e->stream));
-  for (auto& a : e->allocs)
-    if (a == *p) a = q;
-  HIP_TRY(hipFree(*p));
-  *p = q;
-  return MCST_OK;
-}
-
 // Grows the particle pool to at least `min_tiles` tiles: the particle arrays
 // keep their slots (the state's field stride becomes the new pool size),
-// the per-step lists and staging rows are reallocated empty.
+// the per-step lists and staging rows are reallocated empty.  Every new
+// array is allocated before any old one is touched, so a failed allocation
+// leaves the engine as it was.
 int grow_pool(mcst_engine* e, uint64_t min_tiles) {
   BatchDev& b = e->bd;
   const uint64_t T0 = b.T_max, P0 = b.P;
   const uint64_t T1 = std::max<uint64_t>(min_tiles, 2 * T0), P1 = T1 * TILE;
   if (P1 >= 0xFFFFFFFFull) return fail(MCST_ERR_MEMORY, "more than 2^32-1 particle slots in one engine");
-  const uint32_t nv = (uint32_t)e->model->nv;
-  int rc;
-  if ((rc = dev_realloc(e, (void**)&b.pos, P1 * sizeof(uint32_t), P0 * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.status, P1, P0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.age_hyd, P1 * sizeof(double), P0 * sizeof(double)))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.age_div, P1 * sizeof(double), P0 * sizeof(double)))) return rc;
-  {  // state [nv][P]: re-strided field by field
-    double* q = nullptr;
-    const hipError_t err = hipMalloc((void**)&q, P1 * nv * sizeof(double));
-    if (err != hipSuccess) return fail(MCST_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(err));
-    HIP_TRY(hipMemsetAsync(q, 0, P1 * nv * sizeof(double), e->stream));
-    for (uint32_t v = 0; v < nv; ++v)
-      HIP_TRY(hipMemcpyAsync(q + (uint64_t)v * P1, b.state + (uint64_t)v * P0, P0 * sizeof(double),
-                             hipMemcpyDeviceToDevice, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    for (auto& a : e->allocs)
-      if (a == (void*)b.state) a = q;
-    HIP_TRY(hipFree(b.state));
-    b.state = q;
+  const uint64_t nv = (uint64_t)e->model->nv, ST1 = T1 * b.stage_stride;
+  struct Move {
+    void** slot;     // the BatchDev field
+    size_t bytes;    // new size
+    size_t keep;     // bytes copied from the old array (per field for the state)
+    uint64_t fields; // > 0: re-strided [fields][P] array
+    void* fresh;
+  };
+  std::vector<Move> mv = {
+      {(void**)&b.pos, P1 * 4, P0 * 4, 0, nullptr},
+      {(void**)&b.status, P1, P0, 0, nullptr},
+      {(void**)&b.age_hyd, P1 * 8, P0 * 8, 0, nullptr},
+      {(void**)&b.age_div, P1 * 8, P0 * 8, 0, nullptr},
+      {(void**)&b.state, P1 * nv * 8, P0 * 8, nv, nullptr},
+      {(void**)&b.st_state, ST1 * nv * 8, 0, 0, nullptr},
+      {(void**)&b.st_pos, ST1 * 4, 0, 0, nullptr},
+      {(void**)&b.tile_births, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.tile_birth_off, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.tile_gap, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.tile_fill, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.tile_gap_off, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.tile_fill_off, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.fill_src, P1 * 4, 0, 0, nullptr},
+      {(void**)&b.move_slot, P1 * 2, 0, 0, nullptr},
+      {(void**)&b.move_pos, P1 * 4, 0, 0, nullptr},
+      {(void**)&b.div_slot, P1 * 2, 0, 0, nullptr},
+      {(void**)&b.div_pos, P1 * 4, 0, 0, nullptr},
+      {(void**)&b.sub_moves, T1 * b.nsub * 4, 0, 0, nullptr},
+      {(void**)&b.sub_divs, T1 * b.nsub * 4, 0, 0, nullptr},
+  };
+  for (auto& m : mv) {
+    const hipError_t err = hipMalloc(&m.fresh, m.bytes);
+    if (err != hipSuccess) {
+      for (auto& q : mv)
+        if (q.fresh) (void)hipFree(q.fresh);
+      return fail(MCST_ERR_MEMORY, std::string("growing the particle pool: ") + hipGetErrorString(err));
+    }
   }
-  const uint64_t ST1 = T1 * b.stage_stride;
-  if ((rc = dev_realloc(e, (void**)&b.st_state, ST1 * nv * sizeof(double), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.st_pos, ST1 * sizeof(uint32_t), 0))) return rc;
-  uint32_t** tile_arrays[] = {&b.tile_births, &b.tile_birth_off, &b.tile_gap, &b.tile_fill, &b.tile_gap_off,
-                              &b.tile_fill_off};
-  for (uint32_t** a : tile_arrays)
-    if ((rc = dev_realloc(e, (void**)a, T1 * sizeof(uint32_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.fill_src, P1 * sizeof(uint32_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.move_slot, P1 * sizeof(uint16_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.move_pos, P1 * sizeof(uint32_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.div_slot, P1 * sizeof(uint16_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.div_pos, P1 * sizeof(uint32_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.sub_moves, T1 * b.nsub * sizeof(uint32_t), 0))) return rc;
-  if ((rc = dev_realloc(e, (void**)&b.sub_divs, T1 * b.nsub * sizeof(uint32_t), 0))) return rc;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (auto& m : mv) {
+    HIP_TRY(hipMemsetAsync(m.fresh, 0, m.bytes, e->stream));
+    if (m.fields) {
+      for (uint64_t v = 0; v < m.fields; ++v)
+        HIP_TRY(hipMemcpyAsync((char*)m.fresh + v * P1 * 8, (char*)*m.slot + v * P0 * 8, m.keep,
+                               hipMemcpyDeviceToDevice, e->stream));
+    } else if (m.keep) {
+      HIP_TRY(hipMemcpyAsync(m.fresh, *m.slot, m.keep, hipMemcpyDeviceToDevice, e->stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (auto& m : mv) {
+    for (auto& a : e->allocs)
+      if (a == *m.slot) a = m.fresh;
+    (void)hipFree(*m.slot);
+    *m.slot = m.fresh;
+  }
   b.P = P1;
   b.T_max = (uint32_t)T1;
   b.ST = ST1;
