@@ -47,7 +47,7 @@ int fail(int code, const std::string& msg) {
 // ------------------------------------------------------------ model registry
 struct ModelEntry {
   const char* name;
-  int id, nv, ns, np, nl;
+  int id, nv, ns, np, nl, nr;
   bool can_die;
   void (*defaults)(double*);
   void (*init)(const double*, double*, uint64_t, uint32_t);
@@ -62,8 +62,8 @@ double host_mass(const double* p, const double* x) {
 
 template <class M>
 ModelEntry entry() {
-  return ModelEntry{M::NAME,  M::ID,    M::NV,          M::NS,   M::NP, M::NL, M::CAN_DIE, &M::defaults,
-                    &M::init, &host_mass<M>, builtin_model_ops(M::ID)};
+  return ModelEntry{M::NAME,   M::ID,         M::NV, M::NS, M::NP, M::NL, record_terms<M>(), M::CAN_DIE,
+                    &M::defaults, &M::init, &host_mass<M>, builtin_model_ops(M::ID)};
 }
 
 const std::vector<ModelEntry>& registry() {
@@ -124,7 +124,7 @@ int load_plugin(UdfRegistry& u, const char* path, const ModelEntry** out) {
   std::string why;
   if (!d.name || !*d.name) why = "has no name";
   else if (d.nv < 1 || d.nv > MAX_VAR || d.ns < 1 || d.ns > MAX_SPECIES || d.np < 0 || d.np > MAX_PARAMS ||
-           (d.nl != 2 && d.nl != 4))
+           (d.nl != 2 && d.nl != 4) || d.nr < 1 || d.nr > 14)
     why = "has sizes out of range";
   else if (!d.defaults || !d.init || !d.mass || !d.ops.transport || !d.ops.particle || !d.ops.particle_kernel ||
            !d.ops.resolve || !d.ops.properties)
@@ -145,7 +145,7 @@ int load_plugin(UdfRegistry& u, const char* path, const ModelEntry** out) {
   u.paths.emplace_back(key);
   u.ops.push_back(d.ops);
   u.entries.push_back(ModelEntry{u.names.back().c_str(), UDF_ID_BASE + (int)u.entries.size(), d.nv, d.ns, d.np,
-                                 d.nl, d.can_die, d.defaults, d.init, d.mass, &u.ops.back()});
+                                 d.nl, d.nr, d.can_die, d.defaults, d.init, d.mass, &u.ops.back()});
   *out = &u.entries.back();
   return MCST_OK;
 }
@@ -700,7 +700,7 @@ int upload_run(mcst_engine* e, uint32_t r) {
     for (uint32_t j = 0; j < h.n_c; ++j) m0[(size_t)sp * h.n_c + j] = C0[(size_t)sp * h.n_c + j] * h.maps[0].V[j];
   std::vector<double> F, sink;
   feed_tables(h, F, sink);
-  const uint32_t rw = (uint32_t)rec_stride((int)h.n_s);
+  const uint32_t rw = (uint32_t)rec_stride(e->model->nr);
   std::vector<double> rec((size_t)h.n_c * rw, 0.0);
   std::vector<double> zeros(nsc, 0.0);
   if ((rc = run_upload(h, &d.C[0], C0.data(), nsc))) return rc;

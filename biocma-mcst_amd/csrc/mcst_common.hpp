@@ -105,7 +105,7 @@ struct RunDev {
   uint32_t cap;             // slots reserved for the run
   uint32_t n_c;
   uint32_t n_s;
-  uint32_t rec_w;           // record stride in doubles: {M::record(C)_0..n_s-1, p_move, q_exit, pad}
+  uint32_t rec_w;           // record stride in doubles: {M::record(C)_0..NR-1, p_move, q_exit, pad}
   uint32_t enable_move;     // n_c > 1                        (kernels.hpp:54-55)
   uint32_t enable_leave;    // at least one leaving flow      (kernels.hpp:56)
   uint32_t n_maps;

@@ -4,7 +4,8 @@
 // ModelType concept, apps/libs/mc/public/mc/traits.hpp:66-153):
 //   NV, NS, NP           state variables, liquid species, parameters
 //   init(p, x, seed, slot)                 host, initial state      (Model::init)
-//   record(p, c, r)                        device+host, per compartment: writes r[0..NS)
+//   record(p, c, r)                        device+host, per compartment: writes r[0..NR)
+//   NR (optional)                          record terms, NS when absent
 //   update(p, dt, x, r, contrib) -> Status device+host               (Model::update)
 //   division(p, x, xd, key)                device+host              (Model::division)
 //   mass(p, x)                             device+host              (Model::mass)
@@ -14,7 +15,7 @@
 // State is FP64; contributions (kg/s per particle) are returned in
 // registers instead of a per-particle contribs row, because the step kernel
 // scatters them immediately.  record() maps a compartment's concentrations
-// c[0..NS) to the NS terms r[] that update() reads: the parts of the
+// c[0..NS) to the NR terms r[] that update() reads: the parts of the
 // reference's update that depend on the liquid alone (e.g. the Monod factor
 // mu_max*S/(Ks+S)), evaluated once per compartment and step by the transport
 // kernel instead of once per particle, with the same operations in the same
@@ -133,7 +134,23 @@ struct RngKey {
   uint32_t step;
 };
 
-constexpr int rec_stride(int ns) { return (ns + 2) <= 4 ? 4 : ((ns + 2) <= 8 ? 8 : 16); }
+// Record row: the model's NR terms, the move probability and the exit
+// threshold, padded to 4, 8 or 16 doubles.
+constexpr int rec_stride(int nr) { return (nr + 2) <= 4 ? 4 : ((nr + 2) <= 8 ? 8 : 16); }
+
+// NR of a model: its own NR member, else NS.
+template <class M, class = void>
+struct RecordTerms {
+  static constexpr int value = M::NS;
+};
+template <class M>
+struct RecordTerms<M, decltype((void)M::NR)> {
+  static constexpr int value = M::NR;
+};
+template <class M>
+constexpr int record_terms() {
+  return RecordTerms<M>::value;
+}
 
 // ---------------------------------------------------------------- monod_mass
 // Mass-based Monod uptake-growth with division at `ratio` x birth mass.

@@ -112,11 +112,12 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
     const uint32_t n_c = rd.n_c;
     const uint32_t step = rs->step;
     const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
+    constexpr int NR = record_terms<M>();
     double* rec = rd.rec + (size_t)j * rd.rec_w;
-    rec[NS] = mp.p_move[j];
+    rec[NR] = mp.p_move[j];
     double qx = mp.q_exit[j];
     if (rd.n_vf && rd.vx_slot[j] >= 0) qx = rd.vf_q[rd.n_vf + (uint32_t)rd.vx_slot[j]];
-    rec[NS + 1] = qx;
+    rec[NR + 1] = qx;
     const double* c = rd.C[step & 1u];
     double* Cn = rd.C[(step + 1u) & 1u];
     double cn[NS];
@@ -179,13 +180,14 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
         cn[s] = mm * inv_v;
       }
     }
-    double rt[NS];
+    double rt[NR];
     M::record(rd.params, cn, rt);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) rec[k] = rt[k];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const size_t o = (size_t)s * n_c + j;
       Cn[o] = cn[s];
-      rec[s] = rt[s];
       rd.S[o] = 0.0;
     }
   }
@@ -342,12 +344,13 @@ template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE, bool WIN>
 __global__ void __launch_bounds__(BLK)
     k_particle(BatchDev b, const RunDev* __restrict__ runs, const RunState* __restrict__ rsr,
                const uint32_t* __restrict__ tile_off) {
-  constexpr int NV = M::NV, NS = M::NS, REC = rec_stride(M::NS);
-  constexpr int NQ = NS + 2;             // record fields: M::record terms r_0..r_{n_s-1}, p_move, q_exit
+  constexpr int NV = M::NV, NS = M::NS, NR = record_terms<M>(), REC = rec_stride(NR);
+  constexpr int NQ = NR + 2;             // record fields: M::record terms r_0..r_{NR-1}, p_move, q_exit
   constexpr int NP = M::NP > 0 ? M::NP : 1;
   constexpr uint32_t WPB = BLK / 64;     // waves per block = sub-chunks per group
   constexpr uint32_t SUB = 64 * NL;       // slots per sub-chunk
   static_assert(NL == 1 || NL == 2 || NL == 4, "1, 2 or 4 slots per lane");
+  static_assert(NR >= 1 && NR + 2 <= 16, "1..14 record terms");
   constexpr uint32_t NSUB = TILE / SUB;  // sub-chunks per tile (b.nsub is set to the same)
   extern __shared__ double s_acc[];      // [n_s][acc_win] of run acc_run (PRIV)
   __shared__ uint32_t s_w0;
@@ -577,10 +580,10 @@ __global__ void __launch_bounds__(BLK)
       const U4 u = draw4(seed, cur.slot0 + off(j), step, TAG_TRANSPORT, 0u);
       // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:313-325)
       // is listed; k_resolve finds its neighbour and runs its exit test
-      mover[j] = do_move && rr[j][NS] > u01(u.x);
+      mover[j] = do_move && rr[j][NR] > u01(u.x);
       if (do_leave) {
         ah[j] += dt;
-        const double qx = rr[j][NS + 1];
+        const double qx = rr[j][NR + 1];
         if (!mover[j] && qx < 1.0 && u01(u.z) > qx) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
           if constexpr (PROBE) {
             if (rd.probe_exit) {
@@ -834,7 +837,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       pos = gp(mp.nbr)[(size_t)pos * m + col];
       b.pos[g] = pos;
       if (rd.enable_leave && leave_stage) {
-        const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + rd.n_s + 1];
+        const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + record_terms<M>() + 1];
         if (qx < 1.0 && u01(u.z) > qx) {
           const double age = b.age_hyd[g];
           if (rd.probe_exit) {

@@ -15,13 +15,13 @@
 
 namespace mcst {
 
-constexpr uint32_t UDF_ABI_VERSION = 1;
+constexpr uint32_t UDF_ABI_VERSION = 2;
 
 struct UdfModel {
   uint32_t abi_version;
   uint32_t batch_bytes, run_bytes, map_bytes, state_bytes;  // sizeof BatchDev, RunDev, MapDev, RunState
   const char* name;
-  int nv, ns, np, nl;
+  int nv, ns, np, nl, nr;  // nr: record terms (record_terms<M>())
   bool can_die;
   void (*defaults)(double* p);
   void (*init)(const double* p, double* x, uint64_t seed, uint32_t slot);

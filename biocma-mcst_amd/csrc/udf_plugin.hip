@@ -8,8 +8,9 @@
 // MCST_UDF_MODEL the struct.  The struct follows the contract of models.hpp
 // (NAME, NV, NS, NP, NL, CAN_DIE, defaults, init, record, update, division,
 // mass); its ID member is not used: the library numbers loaded models itself.
-// record() writes exactly NS terms (r[0..NS)): the record row holds the move
-// probability and exit threshold right after them.
+// record() writes exactly NR terms (r[0..NR); NR is NS unless the struct
+// declares it): the record row holds the move probability and the exit
+// threshold right after them.
 #include "step_kernels.hpp"
 #include "udf.hpp"
 
@@ -32,6 +33,7 @@ static_assert(Model::NV >= 1 && Model::NV <= mcst::MAX_VAR, "state variables: 1.
 static_assert(Model::NS >= 1 && Model::NS <= mcst::MAX_SPECIES, "species: 1..MAX_SPECIES");
 static_assert(Model::NP >= 0 && Model::NP <= mcst::MAX_PARAMS, "parameters: 0..MAX_PARAMS");
 static_assert(Model::NL == 2 || Model::NL == 4, "slots per lane: 2 or 4");
+static_assert(mcst::record_terms<Model>() >= 1 && mcst::record_terms<Model>() <= 14, "record terms: 1..14");
 
 }  // namespace
 
@@ -46,6 +48,7 @@ extern "C" __attribute__((visibility("default"))) int mcst_udf_describe(mcst::Ud
   u.ns = Model::NS;
   u.np = Model::NP;
   u.nl = Model::NL;
+  u.nr = mcst::record_terms<Model>();
   u.can_die = Model::CAN_DIE;
   u.defaults = &Model::defaults;
   u.init = &Model::init;

@@ -19,6 +19,7 @@
 // division at ratio x birth mass into halves, both fully viable.
 struct UdfStarvation {
   static constexpr int ID = -1, NV = 3, NS = 1, NP = 7;
+  static constexpr int NR = 2;  // record terms: mu and g (more than the one species)
   static constexpr bool CAN_DIE = true;
   static constexpr int NL = 4;
   static constexpr const char* NAME = "udf_starvation";
@@ -33,16 +34,18 @@ struct UdfStarvation {
     x[2] = 0.2 + 0.8 * mcst::u01(w.w);
   }
   MCST_HD static double mass(const double*, const double* x) { return x[0]; }
-  // one term per species (NS): the saturation g = S/(Ks+S)
+  // NR = 2 terms per compartment: the growth rate mu = mu_max g and the
+  // saturation g = S/(Ks+S)
   MCST_HD static void record(const double* p, const double* c, double* r) {
     const double s = c[0] > 0.0 ? c[0] : 0.0;
-    r[0] = s / (p[1] + s);
+    const double g = s / (p[1] + s);
+    r[0] = p[0] * g;
+    r[1] = g;
   }
   MCST_HD static int update(const double* p, double dt, double* x, const double* r, double* contrib) {
-    const double g = r[0];
-    x[2] = x[2] - dt * (p[4] * (1.0 - g));
+    x[2] = x[2] - dt * (p[4] * (1.0 - r[1]));
     if (x[2] <= 0.0) return mcst::DEAD;
-    const double gm = (p[0] * g) * x[0];
+    const double gm = r[0] * x[0];
     contrib[0] = -(gm / p[2]);
     x[0] = x[0] + dt * gm;
     return (x[0] >= p[3] * x[1]) ? mcst::DIVISION : mcst::IDLE;
