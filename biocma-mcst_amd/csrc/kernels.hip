@@ -261,11 +261,20 @@ __global__ void __launch_bounds__(BLOCK) k_gap_fill(BatchDev b, uint32_t n_var) 
 __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) { layout_pass<SCAN_THREADS>(b); }
 
 // ------------------------------------------------------------------ merge
-// Appends staged newborns in tile order.
+// Appends staged newborns in tile order.  One lane per tile: it reads the
+// tile's birth count and copies the tile's newborns (few per tile), so tiles
+// without births cost one load.  A newborn's fields are all loaded before
+// any is stored, two newborns at a time, so a lane waits for memory once per
+// pair rather than once per field.
 __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
-  // one lane per tile: it reads the tile's birth count and copies the
-  // tile's newborns (few per tile), so tiles without births cost one load
   const uint32_t T = b.tile_off[b.R];
+  const uint32_t* __restrict__ st_pos = b.st_pos;
+  const double* __restrict__ st_state = b.st_state;
+  uint32_t* __restrict__ pos = b.pos;
+  uint8_t* __restrict__ status = b.status;
+  double* __restrict__ state = b.state;
+  double* __restrict__ age_hyd = b.age_hyd;
+  double* __restrict__ age_div = b.age_div;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
     const uint32_t nb = b.tile_births[t];
     if (nb == 0) continue;
@@ -274,13 +283,35 @@ __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
     const RunState* rs = b.rs + r;
     const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
     const uint64_t src0 = (uint64_t)t * b.stage_stride;
-    for (uint32_t j = 0; j < nb; ++j) {
-      const uint64_t d = dst0 + j, s = src0 + j;
-      b.pos[d] = b.st_pos[s];
-      b.status[d] = IDLE;
-      b.age_hyd[d] = 0.0;
-      b.age_div[d] = 0.0;
-      for (uint32_t v = 0; v < n_var; ++v) b.state[(uint64_t)v * b.P + d] = b.st_state[(uint64_t)v * b.ST + s];
+    for (uint32_t j = 0; j < nb; j += 2) {
+      const bool two = j + 1 < nb;
+      const uint64_t s0 = src0 + j, s1 = two ? s0 + 1 : s0;
+      double x0[MAX_VAR], x1[MAX_VAR];
+#pragma unroll
+      for (uint32_t v = 0; v < (uint32_t)MAX_VAR; ++v)
+        if (v < n_var) {
+          x0[v] = st_state[(uint64_t)v * b.ST + s0];
+          x1[v] = st_state[(uint64_t)v * b.ST + s1];
+        }
+      const uint32_t p0 = st_pos[s0], p1 = st_pos[s1];
+      const uint64_t d0 = dst0 + j;
+#pragma unroll
+      for (uint32_t v = 0; v < (uint32_t)MAX_VAR; ++v)
+        if (v < n_var) state[(uint64_t)v * b.P + d0] = x0[v];
+      pos[d0] = p0;
+      status[d0] = IDLE;
+      age_hyd[d0] = 0.0;
+      age_div[d0] = 0.0;
+      if (two) {
+        const uint64_t d1 = d0 + 1;
+#pragma unroll
+        for (uint32_t v = 0; v < (uint32_t)MAX_VAR; ++v)
+          if (v < n_var) state[(uint64_t)v * b.P + d1] = x1[v];
+        pos[d1] = p1;
+        status[d1] = IDLE;
+        age_hyd[d1] = 0.0;
+        age_div[d1] = 0.0;
+      }
     }
   }
 }

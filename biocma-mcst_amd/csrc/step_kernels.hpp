@@ -762,6 +762,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   __shared__ uint32_t s_dcnt[RES_SC];     // division requests per sub-chunk
   __shared__ uint32_t s_dpre[RES_SC];     // ... their prefix within the tile
   __shared__ uint32_t s_births[RES_SC];   // accepted requests per tile of the group
+  __shared__ uint32_t s_bpre[RES_SC + 1]; // ... their prefix over the group's tiles
   __shared__ uint32_t s_any;
   const uint32_t tid = threadIdx.x;
   const uint32_t R = b.R;
@@ -850,12 +851,24 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         }
       }
     }
-    // ---- divisions (rare)
+    // ---- divisions: every accepted request of the group's tiles at once
+    // (one thread each, found through the prefix of the tiles' births)
     if (s_any) {
-      for (uint32_t k = 0; k < tpg; ++k) {
+      if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < tpg; ++k) {
+          s_bpre[k] = acc;
+          acc += s_births[k];
+        }
+        s_bpre[tpg] = acc;
+      }
+      __syncthreads();
+      const uint32_t n_births = s_bpre[tpg];
+      for (uint32_t f = tid; f < n_births; f += blockDim.x) {
+        uint32_t k = 0;  // the tile of birth f: the last k with s_bpre[k] <= f
+        while (k + 1 < tpg && s_bpre[k + 1] <= f) ++k;
+        const uint32_t e = f - s_bpre[k];  // request e of tile t, in slot order
         const uint32_t t = t0 + k;
-        const uint32_t births = s_births[k];
-        if (t >= T || births == 0) continue;
         const uint32_t r = s_run[k * nsub];
         const RunDev& rd = runs[r];
         RunState* rs = b.rs + r;
@@ -863,29 +876,27 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         double p[NP];
 #pragma unroll
         for (int i = 0; i < NP; ++i) p[i] = rd.params[i];
-        for (uint32_t e = tid; e < births; e += blockDim.x) {
-          uint32_t j = k * nsub;  // the tile's sub-chunk holding request e
-          while (j + 1 < (k + 1) * nsub && s_dpre[j + 1] <= e) ++j;
-          const uint64_t q = (uint64_t)(sc0 + j) * sub + (e - s_dpre[j]);
-          const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
-          const uint64_t g = rd.p_off + slot;
-          double x[NV], xd[NV];
+        uint32_t j = k * nsub;  // the tile's sub-chunk holding request e
+        while (j + 1 < (k + 1) * nsub && s_dpre[j + 1] <= e) ++j;
+        const uint64_t q = (uint64_t)(sc0 + j) * sub + (e - s_dpre[j]);
+        const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
+        const uint64_t g = rd.p_off + slot;
+        double x[NV], xd[NV];
 #pragma unroll
-          for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
-          if (rd.probe_div) {
-            const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
-            if (i < rd.probe_cap) rd.probe_div[i] = b.age_div[g];
-          }
-          M::division(p, x, xd, RngKey{rd.seed, slot, step});
-          const uint64_t sidx = (uint64_t)t * b.stage_stride + e;
-#pragma unroll
-          for (int v = 0; v < NV; ++v) {
-            b.state[(uint64_t)v * b.P + g] = x[v];
-            b.st_state[(uint64_t)v * b.ST + sidx] = xd[v];
-          }
-          b.age_div[g] = 0.0;
-          b.st_pos[sidx] = b.div_pos[q];
+        for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
+        if (rd.probe_div) {
+          const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
+          if (i < rd.probe_cap) rd.probe_div[i] = b.age_div[g];
         }
+        M::division(p, x, xd, RngKey{rd.seed, slot, step});
+        const uint64_t sidx = (uint64_t)t * b.stage_stride + e;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          b.state[(uint64_t)v * b.P + g] = x[v];
+          b.st_state[(uint64_t)v * b.ST + sidx] = xd[v];
+        }
+        b.age_div[g] = 0.0;
+        b.st_pos[sidx] = b.div_pos[q];
       }
     }
     __syncthreads();
