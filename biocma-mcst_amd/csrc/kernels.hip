@@ -29,36 +29,80 @@ namespace {
 
 // Exclusive scan of in[0..n) into out (optional) by one block of SCAN_THREADS
 // threads, in ascending or descending index order; returns the total.
+//  - up to RB items per thread: each thread owns a contiguous run, loaded in
+//    one round; one block-wide scan of the per-thread sums;
+//  - more: the items are read in coalesced chunks of SCAN_THREADS (item
+//    base + j*SCAN_THREADS + tid), RB chunks at a time with all their loads
+//    in flight; the waves scan each chunk, publish their totals for all RB
+//    chunks at once, and every thread adds up the totals before it (two
+//    barriers per RB chunks).  One CU streams the 2e5 tile counts of a
+//    2e8-particle run in about 0.1 ms; contiguous runs of 200 items per
+//    thread took 0.45 ms (each load instruction touched 64 lines).
 __device__ uint32_t block_scan(const uint32_t* in, uint32_t* out, uint32_t n, bool descending) {
-  // each thread owns a contiguous run of `per` items: one read pass, one
-  // block-wide scan of the per-thread sums, one write pass
-  __shared__ uint32_t s_w[SCAN_WAVES];
+  constexpr uint32_t RB = 16;
+  __shared__ uint32_t s_w[RB][SCAN_WAVES];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-  const uint32_t k0 = tid * per;
-  const uint32_t k1 = (k0 + per < n) ? k0 + per : n;
-  uint32_t sum = 0;
-  for (uint32_t k = k0; k < k1; ++k) sum += in[descending ? n - 1u - k : k];
-  const uint32_t inc = wave_incl_scan(sum, lane);
-  __syncthreads();  // earlier readers of s_w are done
-  if (lane == 63) s_w[wave] = inc;
-  __syncthreads();
-  uint32_t wpre = 0, tot = 0;
-  for (uint32_t w = 0; w < SCAN_WAVES; ++w) {
-    const uint32_t x = s_w[w];
-    if (w < wave) wpre += x;
-    tot += x;
+  if (n <= RB * SCAN_THREADS) {
+    const uint32_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+    const uint32_t k0 = tid * per;
+    const uint32_t k1 = (k0 + per < n) ? k0 + per : n;
+    uint32_t v[RB], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < RB; ++j) {
+      const uint32_t k = k0 + j;
+      v[j] = k < k1 ? in[descending ? n - 1u - k : k] : 0u;
+      sum += v[j];
+    }
+    const uint32_t inc = wave_incl_scan(sum, lane);
+    __syncthreads();  // earlier readers of s_w are done
+    if (lane == 63) s_w[0][wave] = inc;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (uint32_t w = 0; w < SCAN_WAVES; ++w) {
+      const uint32_t x = s_w[0][w];
+      if (w < wave) wpre += x;
+      tot += x;
+    }
+    if (out) {
+      uint32_t run = wpre + inc - sum;
+#pragma unroll
+      for (uint32_t j = 0; j < RB; ++j) {
+        const uint32_t k = k0 + j;
+        if (k < k1) out[descending ? n - 1u - k : k] = run;
+        run += v[j];
+      }
+    }
+    return tot;
   }
-  if (out) {
-    uint32_t run = wpre + inc - sum;
-    for (uint32_t k = k0; k < k1; ++k) {
-      const uint32_t idx = descending ? n - 1u - k : k;
-      const uint32_t v = in[idx];
-      out[idx] = run;
-      run += v;
+  uint32_t carry = 0;  // the items before the current chunks (the same in every thread)
+  for (uint32_t base = 0; base < n; base += RB * SCAN_THREADS) {
+    uint32_t v[RB], inc[RB];
+#pragma unroll
+    for (uint32_t j = 0; j < RB; ++j) {
+      const uint32_t k = base + j * SCAN_THREADS + tid;
+      v[j] = k < n ? in[descending ? n - 1u - k : k] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < RB; ++j) inc[j] = wave_incl_scan(v[j], lane);
+    __syncthreads();  // earlier readers of s_w are done
+    if (lane == 63)
+#pragma unroll
+      for (uint32_t j = 0; j < RB; ++j) s_w[j][wave] = inc[j];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < RB; ++j) {
+      uint32_t wpre = 0, tot = 0;
+      for (uint32_t w = 0; w < SCAN_WAVES; ++w) {
+        const uint32_t x = s_w[j][w];
+        if (w < wave) wpre += x;
+        tot += x;
+      }
+      const uint32_t k = base + j * SCAN_THREADS + tid;
+      if (out && k < n) out[descending ? n - 1u - k : k] = carry + wpre + inc[j] - v[j];
+      carry += tot;
     }
   }
-  return tot;
+  return carry;
 }
 
 }  // namespace
