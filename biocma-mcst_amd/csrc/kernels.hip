@@ -310,7 +310,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) { layout_pa
 // without births cost one load.  A newborn's fields are all loaded before
 // any is stored, two newborns at a time, so a lane waits for memory once per
 // pair rather than once per field.
-__global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
+template <uint32_t NV>
+__global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b) {
   const uint32_t T = b.tile_off[b.R];
   const uint32_t* __restrict__ st_pos = b.st_pos;
   const double* __restrict__ st_state = b.st_state;
@@ -330,18 +331,16 @@ __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
     for (uint32_t j = 0; j < nb; j += 2) {
       const bool two = j + 1 < nb;
       const uint64_t s0 = src0 + j, s1 = two ? s0 + 1 : s0;
-      double x0[MAX_VAR], x1[MAX_VAR];
+      double x0[NV], x1[NV];
 #pragma unroll
-      for (uint32_t v = 0; v < (uint32_t)MAX_VAR; ++v)
-        if (v < n_var) {
-          x0[v] = st_state[(uint64_t)v * b.ST + s0];
-          x1[v] = st_state[(uint64_t)v * b.ST + s1];
-        }
+      for (uint32_t v = 0; v < NV; ++v) {
+        x0[v] = st_state[(uint64_t)v * b.ST + s0];
+        x1[v] = st_state[(uint64_t)v * b.ST + s1];
+      }
       const uint32_t p0 = st_pos[s0], p1 = st_pos[s1];
       const uint64_t d0 = dst0 + j;
 #pragma unroll
-      for (uint32_t v = 0; v < (uint32_t)MAX_VAR; ++v)
-        if (v < n_var) state[(uint64_t)v * b.P + d0] = x0[v];
+      for (uint32_t v = 0; v < NV; ++v) state[(uint64_t)v * b.P + d0] = x0[v];
       pos[d0] = p0;
       status[d0] = IDLE;
       age_hyd[d0] = 0.0;
@@ -349,8 +348,7 @@ __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b, uint32_t n_var) {
       if (two) {
         const uint64_t d1 = d0 + 1;
 #pragma unroll
-        for (uint32_t v = 0; v < (uint32_t)MAX_VAR; ++v)
-          if (v < n_var) state[(uint64_t)v * b.P + d1] = x1[v];
+        for (uint32_t v = 0; v < NV; ++v) state[(uint64_t)v * b.P + d1] = x1[v];
         pos[d1] = p1;
         status[d1] = IDLE;
         age_hyd[d1] = 0.0;
@@ -393,8 +391,20 @@ void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) 
   hipLaunchKernelGGL(k_gap_fill, dim3(grid), dim3(BLOCK), 0, s, b, n_var);
 }
 
+// k_merge for the model's state size (its register arrays hold two newborns)
+template <uint32_t NV>
+void launch_merge_t(const BatchDev& b, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_merge<NV>, dim3(grid), dim3(BLOCK), 0, s, b);
+}
 void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_merge, dim3(grid), dim3(BLOCK), 0, s, b, n_var);
+  static_assert(MAX_VAR == 16, "one instance per state size");
+  using L = void (*)(const BatchDev&, int, hipStream_t);
+  static const L table[MAX_VAR] = {
+      launch_merge_t<1>,  launch_merge_t<2>,  launch_merge_t<3>,  launch_merge_t<4>,
+      launch_merge_t<5>,  launch_merge_t<6>,  launch_merge_t<7>,  launch_merge_t<8>,
+      launch_merge_t<9>,  launch_merge_t<10>, launch_merge_t<11>, launch_merge_t<12>,
+      launch_merge_t<13>, launch_merge_t<14>, launch_merge_t<15>, launch_merge_t<16>};
+  if (n_var >= 1 && n_var <= (uint32_t)MAX_VAR) table[n_var - 1](b, grid, s);
 }
 
 void launch_layout(const BatchDev& b, hipStream_t s) {
