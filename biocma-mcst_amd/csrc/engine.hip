@@ -620,8 +620,10 @@ int choose_launch(mcst_engine* e) {
   e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
   // one thread per compartment, plus the block that lays out the tiles
   e->grid_transport = (int)std::min<uint64_t>(((uint64_t)b.n_comp_total + 255) / 256, 8192) + 1;
-  e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8,
-                                                              ((uint64_t)b.T_max * b.nsub + 63) / 64));
+  // resolve: enough blocks for groups of 32 sub-chunks, up to a wave of
+  // resident blocks (the kernel picks 32 or 64 from the step's tile count)
+  const uint64_t resident = (uint64_t)prop.multiProcessorCount * 8, n_sc = (uint64_t)b.T_max * b.nsub;
+  e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>(resident, (n_sc + 31) / 32));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u (lds cap %zu)\n",
                  best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, lds_cap);

@@ -19,6 +19,7 @@ constexpr uint32_t TILE = BLOCK * PPT;     // 1024 slots per tile (the reference
 constexpr bool LANE_INTERLEAVE = true;     // particle pass: a lane's slots lie 64 apart (else consecutive)
 constexpr uint32_t NSUB_MAX = TILE / 128;  // sub-chunks (the pass's unit of work) per tile: TILE / (64 * M::NL)
 constexpr uint32_t WAVES = BLOCK / 64;
+constexpr uint32_t RESOLVE_GROUP_MAX = 64;  // sub-chunks per group of the resolve kernel: 32 or 64
 constexpr int MAX_PARAMS = 16;
 constexpr int MAX_VAR = 16;
 constexpr int MAX_SPECIES = 8;

@@ -741,7 +741,7 @@ __global__ void __launch_bounds__(BLK)
   if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
 }
 
-// After the particle pass, per group of RES_SC sub-chunks (one per lane of
+// After the particle pass, per group of 32 or 64 sub-chunks (one per lane of
 // the first wave): (1) the movers the pass listed take their neighbour, the
 // first CDF column at or above u, and run the exit test in the new
 // compartment (move_kernel.hpp:392-437, 586-648); (2) each tile accepts the
@@ -749,7 +749,8 @@ __global__ void __launch_bounds__(BLK)
 // buffer, model_kernel.hpp:163-268), halves the mother and stages the
 // newborn at its rank.  Every mover and every request is independent, so
 // the processing order is free.
-constexpr uint32_t RES_SC = 64;
+constexpr uint32_t RES_SC = RESOLVE_GROUP_MAX;  // shared arrays; a group is GS <= RES_SC sub-chunks
+static_assert(RES_SC <= 64 && 32 % NSUB_MAX == 0, "a group is at most one wave of whole tiles");
 
 template <class M>
 __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
@@ -766,36 +767,42 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   __shared__ uint32_t s_any;
   const uint32_t tid = threadIdx.x;
   const uint32_t R = b.R;
-  const uint32_t nsub = b.nsub;          // sub-chunks per tile (divides RES_SC)
+  const uint32_t nsub = b.nsub;          // sub-chunks per tile (divides the group)
   const uint32_t sub = TILE / nsub;      // slots per sub-chunk
   const uint32_t T = tile_off[R];
   const uint32_t n_sc = T * nsub;
-  const uint32_t tpg = RES_SC / nsub;    // tiles per group
-  const uint32_t n_groups = (n_sc + RES_SC - 1) / RES_SC;
+  // groups of 32 sub-chunks while they fit the grid (one group per block:
+  // shorter chains), else 64 (fewer, fuller groups per block); lanes
+  // GS..63 of the first wave stay idle
+  const uint32_t GS = (n_sc + 31) / 32 <= gridDim.x ? 32u : RES_SC;
+  const uint32_t tpg = GS / nsub;        // tiles per group
+  const uint32_t n_groups = (n_sc + GS - 1) / GS;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
   for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
-    const uint32_t sc0 = grp * RES_SC;
+    const uint32_t sc0 = grp * GS;
     const uint32_t t0 = sc0 / nsub;
     if (tid < 64) {  // the group's sub-chunks, one lane each
       const uint32_t lane = tid, sc = sc0 + lane;
       uint32_t cm = 0, cd = 0, r = 0;
-      if (sc < n_sc) {
+      if (lane < GS && sc < n_sc) {
         cm = b.sub_moves[sc];
         cd = b.sub_divs[sc];
         r = find_run(tile_off, R, sc / nsub);
       }
       const uint32_t inc = wave_incl_scan(cm, lane);
-      s_pre[lane] = inc - cm;
-      if (lane == RES_SC - 1) s_pre[RES_SC] = inc;
-      s_run[lane] = r;
-      s_dcnt[lane] = cd;
+      if (lane < GS) {
+        s_pre[lane] = inc - cm;
+        s_run[lane] = r;
+        s_dcnt[lane] = cd;
+      }
+      if (lane == GS - 1) s_pre[GS] = inc;
       // requests: prefix within each tile (segments of nsub lanes)
       const uint32_t incd = wave_incl_scan(cd, lane);
       const uint32_t first = lane - lane % nsub;
       const uint32_t before = __shfl(incd, first ? first - 1 : 0, 64);
       const uint32_t seg_base = first ? before : 0u;
-      s_dpre[lane] = incd - cd - seg_base;
-      if (lane % nsub == nsub - 1) {  // the tile's births: the first `budget` requests
+      if (lane < GS) s_dpre[lane] = incd - cd - seg_base;
+      if (lane < GS && lane % nsub == nsub - 1) {  // the tile's births: the first `budget` requests
         const uint32_t t = t0 + lane / nsub;
         const uint32_t tot = incd - seg_base;
         uint32_t births = 0;
@@ -813,9 +820,9 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
     }
     __syncthreads();
     // ---- movers
-    const uint32_t total = s_pre[RES_SC];
+    const uint32_t total = s_pre[GS];
     for (uint32_t e = tid; e < total; e += blockDim.x) {
-      uint32_t lo = 0, hi = RES_SC;  // the sub-chunk j with s_pre[j] <= e < s_pre[j+1]
+      uint32_t lo = 0, hi = GS;  // the sub-chunk j with s_pre[j] <= e < s_pre[j+1]
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (s_pre[mid] <= e) lo = mid; else hi = mid;
