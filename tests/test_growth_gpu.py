@@ -77,3 +77,23 @@ def test_batch_runs_relocate_and_the_pool_grows():
     for (g, o), n in zip(pairs, (2000, 2500)):
         assert g.n > n + 100
     eng.close()
+
+
+def test_growth_arguments_and_profiled_steps():
+    """A factor outside (1, 16] is refused; steps taken by the profiler are
+    ordinary steps (capacity checks included) and match the oracle."""
+    import mcst
+    from mcst import cases
+
+    c = cases.small_case("monod_mass", n_particles=3000, seed=9)
+    eng, g, o = make_pair(c, capacity=3100)
+    for bad in (1.0, 0.5, 17.0):
+        with pytest.raises(mcst.MCSTError):
+            g.set_growth(1, bad)
+    for r in (g, o):
+        r.set_growth(2, 2.5)
+    eng.profile(10)
+    o.step(10)
+    assert_parity(g, o)
+    assert g.counters()["steps"] == 10 and g.n > 3100
+    eng.close()
