@@ -44,35 +44,67 @@ def _subset(p, seed):
     return np.sort(np.random.default_rng(seed).choice(idle, SUBSET, replace=False))
 
 
-@pytest.mark.parametrize("model", ["monod_mass", "dual_monod"])
-def test_population_distributions_match_under_independent_streams(model):
+def _oracle(case, seed, steps):
+    from oracle.oracle import OracleRun
+
+    o = OracleRun(case.model, case.n_c, case.n_s, case.dt, seed)
+    case.apply(o)
+    o.set_stage_cap(1024)
+    o.set_compaction(0.01, 0)
+    o.step(steps, threads=8)
+    return o
+
+
+@pytest.mark.parametrize("model, steps", [("monod_mass", 200), ("dual_monod", 200), ("quad_monod", 200),
+                                          ("simple_acetate", 200), ("udf_starvation", 60)])
+def test_population_distributions_match_under_independent_streams(model, steps):
+    """Also the 8-state model, the model whose division redistributes its
+    state with random draws, and the plugin model whose cells die.
+
+    Some variables follow the liquid state of the realization (an uptake
+    rate that tracks a nearly depleted substrate): two oracle runs with
+    different seeds already differ in them beyond what KS allows.  A second
+    oracle run (a third seed) calibrates: every quantity on which the two
+    oracle runs agree (p > 0.01) must also agree between the HIP run and the
+    oracle, and most quantities must be of that kind."""
+    import mcst
     from mcst import cases
 
+    if model == "udf_starvation":
+        mcst.load_model(mcst.EXAMPLE_PLUGIN_DEATH)
     c = cases.small_case(model, n_c=12, n_particles=20000, seed=5)
+    if model == "udf_starvation":
+        c.C0[0, ::3] = 0.0
     c.capacity = 400_000
-    eng, g, o = _pair(c, seed_gpu=1001, seed_cpu=2002, steps=200)
-    pg, po = g.particles(), o.particles()
-    ig, io = _subset(pg, 1), _subset(po, 2)
-    for v in range(g.n_var):
-        a, b = pg["state"][v][ig], po["state"][v][io]
-        p = stats.ks_2samp(a, b).pvalue
-        assert p > P_MIN, (model, "state", v, p)
-    for f in ("age_div", "age_hyd"):
-        p = stats.ks_2samp(pg[f][ig], po[f][io]).pvalue
-        assert p > P_MIN, (model, f, p)
+    eng, g, o = _pair(c, seed_gpu=1001, seed_cpu=2002, steps=steps)
+    o2 = _oracle(c, 3003, steps)
+    pg, po, p2 = g.particles(), o.particles(), o2.particles()
+    ig, io, i2 = _subset(pg, 1), _subset(po, 2), _subset(p2, 3)
+    assert len(ig) > 1000 and len(io) > 1000, (len(ig), len(io))
+    samples = [(f"state{v}", pg["state"][v][ig], po["state"][v][io], p2["state"][v][i2]) for v in range(g.n_var)]
+    samples += [(f, pg[f][ig], po[f][io], p2[f][i2]) for f in ("age_div", "age_hyd")]
     # residence times of the particles that left through the outlet, and
     # the ages at division
     for what in ("exit_ages", "division_ages"):
-        a, b = getattr(g, what)(), getattr(o, what)()
-        assert len(a) > 1000 and len(b) > 1000, (what, len(a), len(b))
+        a, b, r = getattr(g, what)(), getattr(o, what)(), getattr(o2, what)()
+        if min(len(a), len(b), len(r)) > 300:
+            samples.append((what, a, b, r))
+    checked = 0
+    for name, a, b, r in samples:
+        if stats.ks_2samp(b, r).pvalue <= P_MIN:
+            continue  # set by the realization's liquid state: not comparable
+        checked += 1
         p = stats.ks_2samp(a, b).pvalue
-        assert p > P_MIN, (model, what, p)
+        assert p > P_MIN, (model, name, p)
+    assert checked >= (len(samples) + 1) // 2, (model, checked, len(samples))
     # event rates per particle-step: a two-proportion z-test, |z| < 5 (the
     # raw counts also carry the spread of the population size)
     cg, co = g.counters(), o.counters()
     assert cg["overflow"] == co["overflow"] == 0
     ng, no = cg["processed"], co["processed"]
-    for k in ("births", "exits", "moves"):
+    for k in ("births", "exits", "moves", "deaths"):
+        if cg[k] + co[k] == 0:
+            continue
         pool = (cg[k] + co[k]) / (ng + no)
         sd = math.sqrt(pool * (1 - pool) * (1 / ng + 1 / no))
         z = (cg[k] / ng - co[k] / no) / sd
