@@ -497,8 +497,14 @@ __global__ void __launch_bounds__(BLK)
         // the window follows the block's slots: when the group's first
         // compartment leaves the window's lower half (slots are in
         // compartment order, newborns at the end of the run in parent
-        // order), flush it and re-centre a quarter ahead
-        if (acc_win < acc_nc) {
+        // order), flush it and re-centre a quarter ahead.  Checked on every
+        // other group (grp is the same in every wave): the check costs two
+        // block barriers, and the window only decides which contributions
+        // take the LDS and which take global atomics.  On the 50 000-
+        // compartment lattice at 1e8 particles, checking every group cost 6 %
+        // of the step; every 8th or 16th let the window lag behind the
+        // newborns and cost more
+        if (acc_win < acc_nc && (grp & 1u) == 0) {
           if (tid == 0) s_w0 = cur.cnt ? A.pos[0] : acc_w0;
           __syncthreads();
           const uint32_t c0 = s_w0;
