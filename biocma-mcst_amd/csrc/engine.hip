@@ -996,7 +996,7 @@ void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr
   mark(4);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   mark(5);
-  launch_merge(b, nv, e->grid_particle, e->stream);
+  launch_merge(b, nv, e->stream);
   mark(6);
 }
 

@@ -21,6 +21,8 @@
 // holds the others.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "step_kernels.hpp"
 
 namespace mcst {
@@ -305,11 +307,13 @@ __global__ void __launch_bounds__(BLOCK) k_gap_fill(BatchDev b, uint32_t n_var) 
 __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) { layout_pass<SCAN_THREADS>(b); }
 
 // ------------------------------------------------------------------ merge
-// Appends staged newborns in tile order.  One lane per tile: it reads the
-// tile's birth count and copies the tile's newborns (few per tile), so tiles
-// without births cost one load.  A newborn's fields are all loaded before
-// any is stored, two newborns at a time, so a lane waits for memory once per
-// pair rather than once per field.
+// Appends staged newborns in tile order.  MERGE_LANES lanes per tile: lane k
+// copies the tile's newborns k, k + MERGE_LANES, ... with all of a newborn's
+// fields loaded before any is stored.  A tile with the full budget of
+// newborns (up to 64) then costs a lane a few round trips instead of one
+// per newborn, which set the kernel's length when one lane took a whole tile.
+constexpr uint32_t MERGE_LANES = 8;
+
 template <uint32_t NV>
 __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b) {
   const uint32_t T = b.tile_off[b.R];
@@ -320,40 +324,29 @@ __global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b) {
   double* __restrict__ state = b.state;
   double* __restrict__ age_hyd = b.age_hyd;
   double* __restrict__ age_div = b.age_div;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+  const uint64_t n_lanes = (uint64_t)T * MERGE_LANES;
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_lanes;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t = (uint32_t)(q / MERGE_LANES), k = (uint32_t)(q % MERGE_LANES);
     const uint32_t nb = b.tile_births[t];
-    if (nb == 0) continue;
+    if (k >= nb) continue;
     const uint32_t r = find_run(b.tile_off, b.R, t);
     const RunDev& rd = b.runs[r];
     const RunState* rs = b.rs + r;
     const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
     const uint64_t src0 = (uint64_t)t * b.stage_stride;
-    for (uint32_t j = 0; j < nb; j += 2) {
-      const bool two = j + 1 < nb;
-      const uint64_t s0 = src0 + j, s1 = two ? s0 + 1 : s0;
-      double x0[NV], x1[NV];
+    for (uint32_t j = k; j < nb; j += MERGE_LANES) {
+      const uint64_t sj = src0 + j, d = dst0 + j;
+      double x[NV];
 #pragma unroll
-      for (uint32_t v = 0; v < NV; ++v) {
-        x0[v] = st_state[(uint64_t)v * b.ST + s0];
-        x1[v] = st_state[(uint64_t)v * b.ST + s1];
-      }
-      const uint32_t p0 = st_pos[s0], p1 = st_pos[s1];
-      const uint64_t d0 = dst0 + j;
+      for (uint32_t v = 0; v < NV; ++v) x[v] = st_state[(uint64_t)v * b.ST + sj];
+      const uint32_t p = st_pos[sj];
 #pragma unroll
-      for (uint32_t v = 0; v < NV; ++v) state[(uint64_t)v * b.P + d0] = x0[v];
-      pos[d0] = p0;
-      status[d0] = IDLE;
-      age_hyd[d0] = 0.0;
-      age_div[d0] = 0.0;
-      if (two) {
-        const uint64_t d1 = d0 + 1;
-#pragma unroll
-        for (uint32_t v = 0; v < NV; ++v) state[(uint64_t)v * b.P + d1] = x1[v];
-        pos[d1] = p1;
-        status[d1] = IDLE;
-        age_hyd[d1] = 0.0;
-        age_div[d1] = 0.0;
-      }
+      for (uint32_t v = 0; v < NV; ++v) state[(uint64_t)v * b.P + d] = x[v];
+      pos[d] = p;
+      status[d] = IDLE;
+      age_hyd[d] = 0.0;
+      age_div[d] = 0.0;
     }
   }
 }
@@ -392,19 +385,21 @@ void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) 
 }
 
 // k_merge for the model's state size (its register arrays hold two newborns)
+// one lane per (tile, k < MERGE_LANES) of the largest layout
 template <uint32_t NV>
-void launch_merge_t(const BatchDev& b, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_merge<NV>, dim3(grid), dim3(BLOCK), 0, s, b);
+void launch_merge_t(const BatchDev& b, hipStream_t s) {
+  const uint64_t blocks = ((uint64_t)b.T_max * MERGE_LANES + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_merge<NV>, dim3((uint32_t)std::min<uint64_t>(blocks, 1u << 20)), dim3(BLOCK), 0, s, b);
 }
-void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
+void launch_merge(const BatchDev& b, uint32_t n_var, hipStream_t s) {
   static_assert(MAX_VAR == 16, "one instance per state size");
-  using L = void (*)(const BatchDev&, int, hipStream_t);
+  using L = void (*)(const BatchDev&, hipStream_t);
   static const L table[MAX_VAR] = {
       launch_merge_t<1>,  launch_merge_t<2>,  launch_merge_t<3>,  launch_merge_t<4>,
       launch_merge_t<5>,  launch_merge_t<6>,  launch_merge_t<7>,  launch_merge_t<8>,
       launch_merge_t<9>,  launch_merge_t<10>, launch_merge_t<11>, launch_merge_t<12>,
       launch_merge_t<13>, launch_merge_t<14>, launch_merge_t<15>, launch_merge_t<16>};
-  if (n_var >= 1 && n_var <= (uint32_t)MAX_VAR) table[n_var - 1](b, grid, s);
+  if (n_var >= 1 && n_var <= (uint32_t)MAX_VAR) table[n_var - 1](b, s);
 }
 
 void launch_layout(const BatchDev& b, hipStream_t s) {
