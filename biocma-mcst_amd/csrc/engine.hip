@@ -127,7 +127,7 @@ int load_plugin(UdfRegistry& u, const char* path, const ModelEntry** out) {
            (d.nl != 2 && d.nl != 4) || d.nr < 1 || d.nr > 14)
     why = "has sizes out of range";
   else if (!d.defaults || !d.init || !d.mass || !d.ops.transport || !d.ops.particle || !d.ops.particle_kernel ||
-           !d.ops.resolve || !d.ops.properties)
+           !d.ops.resolve || !d.ops.resolve_kernel || !d.ops.properties)
     why = "misses a function";
   else if (std::strcmp(d.name, UDF_ALIAS) == 0)
     why = std::string("may not be named ") + UDF_ALIAS;
@@ -620,9 +620,14 @@ int choose_launch(mcst_engine* e) {
   e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
   // one thread per compartment, plus the block that lays out the tiles
   e->grid_transport = (int)std::min<uint64_t>(((uint64_t)b.n_comp_total + 255) / 256, 8192) + 1;
-  // resolve: enough blocks for groups of 32 sub-chunks, up to a wave of
-  // resident blocks (the kernel picks 32 or 64 from the step's tile count)
-  const uint64_t resident = (uint64_t)prop.multiProcessorCount * 8, n_sc = (uint64_t)b.T_max * b.nsub;
+  // resolve: enough blocks for groups of 32 sub-chunks, up to the blocks
+  // that are resident at once (a second round of blocks would start after
+  // the first had worked through their groups); the kernel picks 32 or 64
+  // from the step's tile count
+  int res_per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&res_per_cu, M.ops->resolve_kernel(), 256, 0));
+  const uint64_t resident = (uint64_t)prop.multiProcessorCount * (uint64_t)std::max(1, res_per_cu);
+  const uint64_t n_sc = (uint64_t)b.T_max * b.nsub;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>(resident, (n_sc + 31) / 32));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u (lds cap %zu)\n",

@@ -23,14 +23,15 @@ struct ParticleLaunch {
 
 // A model's kernels: transport (its record() per compartment), the particle
 // pass (particle_kernel: the kernel of a launch shape, for occupancy queries;
-// nullptr when the shape has no variant), resolve (its division()) and the
-// export values (its mass()).  The same table comes from the library for the
+// nullptr when the shape has no variant), resolve (its division();
+// resolve_kernel for occupancy queries) and the export values (its mass()).  The same table comes from the library for the
 // built-in models and from a plugin for a model loaded at run time.
 struct ModelOps {
   void (*transport)(const BatchDev& b, int grid, hipStream_t s);
   void (*particle)(const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
   const void* (*particle_kernel)(int block, bool priv, bool probe, bool win);
   void (*resolve)(const BatchDev& b, int grid, hipStream_t s);
+  const void* (*resolve_kernel)();  // for occupancy queries
   void (*properties)(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s);
 };
 const ModelOps* builtin_model_ops(int model_id);

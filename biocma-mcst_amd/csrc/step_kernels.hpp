@@ -1062,6 +1062,10 @@ void resolve_t(const BatchDev& b, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_resolve<M>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, (const uint32_t*)b.tile_off);
 }
 template <class M>
+const void* resolve_kernel_t() {
+  return (const void*)&k_resolve<M>;
+}
+template <class M>
 void properties_t(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s) {
   hipLaunchKernelGGL(k_properties<M>, dim3(1024), dim3(256), 0, s, b, run, n, values, spatial);
 }
@@ -1069,7 +1073,8 @@ void properties_t(const BatchDev& b, uint32_t run, uint64_t n, double* values, d
 // The launchers of model M's kernels.
 template <class M>
 ModelOps model_ops() {
-  return ModelOps{&transport_t<M>, &launch_t<M>, &kernel_ptr_t<M>, &resolve_t<M>, &properties_t<M>};
+  return ModelOps{&transport_t<M>, &launch_t<M>, &kernel_ptr_t<M>, &resolve_t<M>, &resolve_kernel_t<M>,
+                  &properties_t<M>};
 }
 
 }  // namespace mcst
