@@ -386,6 +386,7 @@ struct mcst_engine {
   size_t sort_temp = 0;
   int tuned_group = -1;
   int grid_move = 1;
+  int grid_merge = 1;
   bool removal_possible = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -629,6 +630,11 @@ int choose_launch(mcst_engine* e) {
   const uint64_t resident = (uint64_t)prop.multiProcessorCount * (uint64_t)std::max(1, res_per_cu);
   const uint64_t n_sc = (uint64_t)b.T_max * b.nsub;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>(resident, (n_sc + 31) / 32));
+  // merge: 8 lanes per tile of the pool, at most a wave of 8 blocks per CU
+  // (the kernel strides over the step's tiles; most steps of a large pool
+  // have few live tiles, and idle blocks still cost their launch)
+  e->grid_merge = (int)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8, ((uint64_t)b.T_max * 8 + BLOCK - 1) / BLOCK));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u (lds cap %zu)\n",
                  best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, lds_cap);
@@ -1001,7 +1007,7 @@ void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr
   mark(4);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   mark(5);
-  launch_merge(b, nv, e->stream);
+  launch_merge(b, nv, e->grid_merge, e->stream);
   mark(6);
 }
 

@@ -21,8 +21,6 @@
 // holds the others.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
-
 #include "step_kernels.hpp"
 
 namespace mcst {
@@ -385,21 +383,19 @@ void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) 
 }
 
 // k_merge for the model's state size (its register arrays hold two newborns)
-// one lane per (tile, k < MERGE_LANES) of the largest layout
 template <uint32_t NV>
-void launch_merge_t(const BatchDev& b, hipStream_t s) {
-  const uint64_t blocks = ((uint64_t)b.T_max * MERGE_LANES + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL(k_merge<NV>, dim3((uint32_t)std::min<uint64_t>(blocks, 1u << 20)), dim3(BLOCK), 0, s, b);
+void launch_merge_t(const BatchDev& b, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_merge<NV>, dim3(grid), dim3(BLOCK), 0, s, b);
 }
-void launch_merge(const BatchDev& b, uint32_t n_var, hipStream_t s) {
+void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
   static_assert(MAX_VAR == 16, "one instance per state size");
-  using L = void (*)(const BatchDev&, hipStream_t);
+  using L = void (*)(const BatchDev&, int, hipStream_t);
   static const L table[MAX_VAR] = {
       launch_merge_t<1>,  launch_merge_t<2>,  launch_merge_t<3>,  launch_merge_t<4>,
       launch_merge_t<5>,  launch_merge_t<6>,  launch_merge_t<7>,  launch_merge_t<8>,
       launch_merge_t<9>,  launch_merge_t<10>, launch_merge_t<11>, launch_merge_t<12>,
       launch_merge_t<13>, launch_merge_t<14>, launch_merge_t<15>, launch_merge_t<16>};
-  if (n_var >= 1 && n_var <= (uint32_t)MAX_VAR) table[n_var - 1](b, s);
+  if (n_var >= 1 && n_var <= (uint32_t)MAX_VAR) table[n_var - 1](b, grid, s);
 }
 
 void launch_layout(const BatchDev& b, hipStream_t s) {
