@@ -396,3 +396,29 @@ def test_lattice50k_batch_of_two_window_runs():
         assert_parity(g, o, probes=False)
         assert g.counters()["births"] > 0
     eng.close()
+
+
+@pytest.mark.parametrize("env", [{"MCST_PARTICLE_BLOCK": "512"}, {"MCST_PARTICLE_BLOCK": "1024"},
+                                 {"MCST_CONTRIB_LDS": "0"}, {"MCST_TILE_GROUP": "0"}, {"MCST_TILE_GROUP": "4"}])
+def test_forced_launch_variants(env, monkeypatch):
+    """Every particle-pass variant the autotuner or the overrides can select
+    (block size, global-atomic contributions, tile grouping) gives the same
+    events as the oracle: on the small network (whole-run LDS accumulator)
+    and on the lattice (compartment window)."""
+    from mcst import cases
+
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    c = cases.small_case("dual_monod", n_particles=6000, seed=4)
+    eng, g, o = make_pair(c)
+    eng.step(12)
+    o.step(12)
+    assert_parity(g, o)
+    eng.close()
+    c = cases.lattice50k(n_particles=100_000)
+    c.capacity = 300_000
+    eng, g, o = make_pair(c)
+    eng.step(8)
+    o.step(8, threads=8)
+    assert_parity(g, o, probes=False)
+    eng.close()
