@@ -610,11 +610,10 @@ int choose_launch(mcst_engine* e) {
   if (best.probe && best.lds > 64 * 1024)
     (void)hipFuncSetAttribute(M.ops->particle_kernel(256, best.priv, true, best.win),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)best.lds);
-  // one run: blocks take groups in turn, which keeps the wave front of the
-  // whole grid inside a narrow window of the particle arrays; a batch: each
-  // block takes one contiguous range, so it meets few runs
-  // one run whose accumulator fits: groups in turn; a batch, or a window of
-  // compartments: one contiguous range per block
+  // one run whose accumulator fits: blocks take groups in turn, which keeps
+  // the wave front of the whole grid inside a narrow window of the particle
+  // arrays; a batch, or a window of compartments: one contiguous range per
+  // block, so it meets few runs and few compartments
   b.tile_group = e->tuned_group >= 0 ? (uint32_t)e->tuned_group : (live <= 1 && !windowed ? 1u : 0u);
   if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
   e->plaunch = best;
