@@ -173,6 +173,13 @@ class Engine:
         except Exception:
             pass
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def add_run(self, n_c, dt, seed):
         rid = C.c_int32()
         _ck(lib().mcst_run_create(self.h, n_c, self.n_s, dt, seed, C.byref(rid)), "run_create")
