@@ -44,11 +44,9 @@ def read_traffic():
 
 
 def cpu_baseline(case_fn, seconds=12.0, max_steps=20000, n_particles=1_000_000):
-    # thread placement as the reference's own benchmark sets it
-    # (tools/benchs/benchmark.py:147-153); read when the OpenMP runtime of
-    # the oracle library starts, i.e. at its first load below
-    os.environ.setdefault("OMP_PROC_BIND", "spread")
-    os.environ.setdefault("OMP_PLACES", "threads")
+    # No OMP_PROC_BIND / OMP_PLACES here: binding also pins this process's
+    # initial thread, and the host work of the measurements after this one
+    # (particle initialisation threads) would then share its one core.
     from oracle.oracle import OracleRun
 
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -65,8 +63,7 @@ def cpu_baseline(case_fn, seconds=12.0, max_steps=20000, n_particles=1_000_000):
     ps = r.counters()["processed"] - p0
     return {"value": ps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle (C/OpenMP restatement) on the same 5000-compartment network, "
-                      f"{n_particles:.0e} particles, {done} steps, {el:.1f} s, "
-                      f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']} OMP_PLACES={os.environ['OMP_PLACES']}"}
+                      f"{n_particles:.0e} particles, {done} steps, {el:.1f} s"}
 
 
 def served_with_lattice(n_requests=40, rate=20.0, seed=13, chunk=20, lattice_particles=10_000_000):
