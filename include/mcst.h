@@ -225,7 +225,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
  * with each candidate shape (threads per block 256/512/1024 x consecutive
  * tiles per block turn 0/1/2/4/8), times the particle pass with HIP events,
  * and keeps the fastest for the engine's later steps (also across admit and
- * retire).  The runs advance by the trial steps; their results do not depend
+ * retire); the untuned shape stays unless another is at least 2 % faster.  The runs advance by the trial steps; their results do not depend
  * on the shape.  Drops a captured graph. */
 int mcst_engine_autotune(mcst_engine* e, uint32_t steps, int32_t* block, int32_t* group, double* particle_ms);
 
