@@ -117,23 +117,31 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_bookkeep(BatchDev b) {
   const RunDev& rd = b.runs[r];
   const uint32_t t0 = b.tile_off[r];
   const uint32_t nt = b.tile_off[r + 1] - t0;
+  // the run's counts, read before the scan (which changes none of them), so
+  // their loads overlap it instead of following its barriers
+  const uint32_t n = rs->n, inactive0 = rs->inactive, step0 = rs->step;
+  const unsigned long long exits = rs->step_exits, deaths = rs->step_deaths, moves = rs->step_moves,
+                           overflow = rs->step_overflow, processed = rs->step_processed;
+  unsigned long long cnt[C_COUNT];
+#pragma unroll
+  for (int k = 0; k < C_COUNT; ++k) cnt[k] = rs->counters[k];
+  const double ratio = rd.ratio;
+  const uint64_t min_removal = rd.min_removal;
   const uint32_t births = block_scan(b.tile_births + t0, b.tile_birth_off + t0, nt, false);
   if (threadIdx.x != 0) return;
-  const uint32_t n = rs->n;
-  const unsigned long long exits = rs->step_exits, deaths = rs->step_deaths;
-  rs->counters[C_BIRTHS] += births;
-  rs->counters[C_EXITS] += exits;
-  rs->counters[C_DEATHS] += deaths;
-  rs->counters[C_MOVES] += rs->step_moves;
-  rs->counters[C_OVERFLOW] += rs->step_overflow;
-  rs->counters[C_PROCESSED] += rs->step_processed;
+  cnt[C_BIRTHS] += births;
+  cnt[C_EXITS] += exits;
+  cnt[C_DEATHS] += deaths;
+  cnt[C_MOVES] += moves;
+  cnt[C_OVERFLOW] += overflow;
+  cnt[C_PROCESSED] += processed;
   rs->step_exits = rs->step_deaths = rs->step_moves = rs->step_overflow = rs->step_processed = 0;
-  uint64_t inactive = (uint64_t)rs->inactive + exits + deaths;
-  const uint64_t thr_ratio = (uint64_t)((double)n * rd.ratio);
-  const uint64_t thr = rd.min_removal > thr_ratio ? rd.min_removal : thr_ratio;
+  uint64_t inactive = (uint64_t)inactive0 + exits + deaths;
+  const uint64_t thr_ratio = (uint64_t)((double)n * ratio);
+  const uint64_t thr = min_removal > thr_ratio ? min_removal : thr_ratio;
   uint32_t n_mid = n, compact = 0;
   if ((b.stages & STAGE_REMOVE) && inactive > thr) {
-    rs->counters[C_COMPACTIONS] += 1;
+    cnt[C_COMPACTIONS] += 1;
     if (inactive >= n) {
       n_mid = 0;
       compact = 2;  // everything removed, nothing to move
@@ -149,10 +157,12 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_bookkeep(BatchDev b) {
   rs->births = births;
   rs->inactive = (uint32_t)inactive;
   rs->n = n_mid + births;
-  rs->step = rs->step + 1u;
-  rs->counters[C_STEPS] = rs->step;
-  rs->counters[C_N] = rs->n;
-  rs->counters[C_INACTIVE] = rs->inactive;
+  rs->step = step0 + 1u;
+  cnt[C_STEPS] = step0 + 1u;
+  cnt[C_N] = n_mid + births;
+  cnt[C_INACTIVE] = (uint32_t)inactive;
+#pragma unroll
+  for (int k = 0; k < C_COUNT; ++k) rs->counters[k] = cnt[k];
 }
 
 // ------------------------------------------------------------------ removal
