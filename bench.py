@@ -43,27 +43,74 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(case_fn, seconds=12.0, max_steps=20000, n_particles=1_000_000):
-    # No OMP_PROC_BIND / OMP_PLACES here: binding also pins this process's
-    # initial thread, and the host work of the measurements after this one
-    # (particle initialisation threads) would then share its one core.
+# Threads of the CPU baseline: the host-core share of one GPU on the
+# benchmark hosts (16), or fewer when the process may run on fewer cores.
+CPU_SHARE = 16
+
+
+def host_info():
+    """nproc, the cores this process may run on, and lscpu's model/topology."""
+    info = {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0))}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_baseline_child(n_particles, seconds, threads):
+    """Runs in a child process started before the parent touches the GPU:
+    the oracle (C/OpenMP restatement of the step) on the full
+    5000-compartment workload, timed over the step loop only."""
+    from mcst import cases
     from oracle.oracle import OracleRun
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    c = case_fn(n_particles=n_particles)
+    t_init = time.perf_counter()
+    c = cases.rgg5000(n_particles=n_particles)
     r = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
     c.apply(r)
-    r.step(2, threads=threads)  # warm
+    t_init = time.perf_counter() - t_init
+    r.step(2, threads=threads)  # warm: first touch of every page of the particle arrays
     done, t0 = 0, time.perf_counter()
     p0 = r.counters()["processed"]
-    while done < max_steps and time.perf_counter() - t0 < seconds:
-        r.step(10, threads=threads)
-        done += 10
+    while time.perf_counter() - t0 < seconds:
+        r.step(5, threads=threads)
+        done += 5
     el = time.perf_counter() - t0
     ps = r.counters()["processed"] - p0
-    return {"value": ps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C/OpenMP restatement) on the same 5000-compartment network, "
-                      f"{n_particles:.0e} particles, {done} steps, {el:.1f} s"}
+    print(json.dumps({"value": ps / el, "steps": done, "seconds": el, "init_s": t_init}), flush=True)
+
+
+def cpu_baseline(n_particles=10_000_000, seconds=15.0):
+    """The CPU baseline in a child process (so that the OpenMP placement of
+    the reference's bench, OMP_PROC_BIND=spread OMP_PLACES=threads
+    (tools/benchs/benchmark.py:147-153), pins the child's threads and never
+    this process), run to completion before any HIP call here."""
+    import subprocess
+
+    host = host_info()
+    threads = max(1, min(CPU_SHARE, host["affinity_cores"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="threads")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--particles", str(n_particles),
+           "--cpu-seconds", str(seconds), "--cpu-threads", str(threads)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError("CPU baseline child failed:\n" + out.stderr[-2000:])
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": res["value"], "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C/OpenMP restatement of the reference step) on the bench workload itself: "
+                      f"5000-compartment network, {n_particles:.0e} dual-substrate Monod particles, "
+                      f"{res['steps']} steps in {res['seconds']:.1f} s, {threads} threads "
+                      f"(OMP_PROC_BIND=spread, OMP_PLACES=threads)",
+            "host": host}
 
 
 def served_with_lattice(n_requests=40, rate=20.0, seed=13, chunk=20, lattice_particles=10_000_000):
@@ -258,7 +305,13 @@ def main():
                     help="reorder the lattice case's particles by compartment every K steps (0: never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of this many MiB (PMC counter calibration)")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_child:
+        cpu_baseline_child(a.particles, a.cpu_seconds, a.cpu_threads)
+        return
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
@@ -267,6 +320,12 @@ def main():
               "the step would shard by particles with one all-reduce of the contribution matrix per step",
               file=sys.stderr)
         sys.exit(2)
+
+    cpu = None
+    if not a.no_cpu:  # first, before this process makes any HIP call
+        log(f"timing the CPU baseline on {a.particles:.0e} particles (child process)")
+        cpu = cpu_baseline(a.particles)
+        log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
 
     import mcst
     from mcst import cases
@@ -333,11 +392,6 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_particle<DualMonod>", "bytes_per_particle_step": b_ps,
             "particles_per_launch": per_launch, "kernel_ms": per[1]}
-    cpu = None
-    if not a.no_cpu:
-        log("timing the CPU baseline")
-        cpu = cpu_baseline(cases.rgg5000)
-        log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
     served = None
     if not a.no_served:
         log(f"serving {a.served_requests} requests of the served stream, then one at a time")

@@ -213,7 +213,7 @@ struct Edge {
 // Transition data of one flowmap.  Off-diagonal flows F(i,j) (duplicates
 // summed in input order, non-positive totals dropped) give:
 //   out_flow[i] = sum_j F(i,j)                (diag of the transition matrix)
-//   p_move[i]   = dt*out_flow[i] / V[i]        (probability_leaving.hpp:313-325)
+//   p_move[i]   = dt*out_flow[i] / V[i]        (probability_leaving.hpp:33-46)
 //   cdf[i][k]   = sum_{k'<=k} F(i,nbr_k') / out_flow[i], neighbours ascending,
 //                 rows padded with cdf 1 and the last neighbour
 //   transport matrix by destination column: F(i,j) and -out_flow[j]

@@ -291,19 +291,28 @@ struct Monod {
 };
 
 // ---------------------------------------------------------------- fixed_length
-// apps/libs/models/public/models/fixed_length.hpp:271-413 in FP64.
-// x = {l, l_max}; p = {l_dot_max, l_max, k, d_m}.
+// apps/libs/models/public/models/fixed_length.hpp:108-161 in FP64.
+// x = {l, l_max}; p = {l_dot_max, l_max, k, d_m, lambda}.
 struct FixedLength {
-  static constexpr int ID = 3, NV = 2, NS = 1, NP = 4;
+  static constexpr int ID = 3, NV = 2, NS = 1, NP = 5;
   static constexpr bool CAN_DIE = false;  // update() never returns DEAD
   static constexpr int NL = 4;
   static constexpr const char* NAME = "fixed_length";
   static void defaults(double* p) {
-    p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6;
+    p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6; p[4] = 0.69314718055994530942 / 1e-6;
   }
+  // Initial length: the density lambda*exp(-lambda*l) truncated to
+  // [l_max/2, l_max], the target the reference samples by Metropolis
+  // (models/src/config_loader.cpp:13-36; lambda = ln2/1e-6 unless VLAMBDA
+  // sets it), drawn here exactly by inverting its CDF:
+  //   l = a - log1p(-u*(1 - exp(-lambda*(b - a))))/lambda.
+  // lambda = 0 gives the uniform density, the limit lambda -> 0.
   static void init(const double* p, double* x, uint64_t seed, uint32_t slot) {
     const U4 w = draw4(seed, slot, 0u, TAG_INIT, 0u);
-    x[0] = p[1] * (0.5 + 0.5 * u01(w.x));
+    const double a = p[1] / 2.0, b = p[1], lam = p[4];
+    const double u = u01_52(w.x, w.y);
+    double l = lam != 0.0 ? a - log1p(-u * -expm1(-lam * (b - a))) / lam : a + u * (b - a);
+    x[0] = l < a ? a : (l > b ? b : l);
     x[1] = p[1];
   }
   MCST_HD static double mass(const double* p, const double* x) { return x[0] * lin_density(p[3]); }

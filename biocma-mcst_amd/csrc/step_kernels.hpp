@@ -380,14 +380,19 @@ __global__ void __launch_bounds__(BLK)
   };
   auto sub_ref = [&](uint32_t sc) -> SubRef {
     SubRef q{sc, 0u, 0u, 0u, 0u, 0ull};
+    // a phantom stands at the first slot of the pass's last tile: the run of
+    // that tile is live, so the loads of the phantom (particle arrays, the
+    // run's compartment record) stay inside memory the run owns; the run of
+    // index 0 may have been retired and its arrays freed
+    const uint32_t t = sc < n_sc ? sc / NSUB : T - 1u;
+    q.t = t;
+    q.r = find_run(tile_off, R, t);
+    q.slot0 = (t - tile_off[q.r]) * TILE + (sc < n_sc ? (sc % NSUB) * SUB : 0u);
     if (sc < n_sc) {
-      q.t = sc / NSUB;
-      q.r = find_run(tile_off, R, q.t);
-      q.slot0 = (q.t - tile_off[q.r]) * TILE + (sc % NSUB) * SUB;
       const uint32_t n = rsr[q.r].n;
       q.cnt = n > q.slot0 ? (n - q.slot0 < SUB ? n - q.slot0 : SUB) : 0u;
-      q.base = runs[q.r].p_off + q.slot0;
     }
+    q.base = runs[q.r].p_off + q.slot0;
     return q;
   };
   // sub-chunk-relative slot of the lane's j-th slot: interleaved (64 apart,
@@ -584,7 +589,7 @@ __global__ void __launch_bounds__(BLK)
       }
       want[j] = code == DIVISION;
       const U4 u = draw4(seed, cur.slot0 + off(j), step, TAG_TRANSPORT, 0u);
-      // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:313-325)
+      // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:33-46)
       // is listed; k_resolve finds its neighbour and runs its exit test
       mover[j] = do_move && rr[j][NR] > u01(u.x);
       if (do_leave) {

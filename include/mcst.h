@@ -106,7 +106,7 @@ int mcst_engine_destroy(mcst_engine** engine);
 
 /* ---- run definition (before mcst_engine_commit) ---- */
 /* A run: compartments, species, time step, RNG seed.  (SimulationUnit ctor,
- * apps/libs/simulation/src/simulation.cpp:202-236) */
+ * apps/libs/simulation/src/simulation.cpp:47-81) */
 int mcst_run_create(mcst_engine* e, uint32_t n_compartments, uint32_t n_species, double dt, uint64_t seed,
                     int32_t* run_id);
 int mcst_run_set_model_params(mcst_engine* e, int32_t run, const double* params, int n);

@@ -25,7 +25,7 @@
  *   init + weight ......... apps/libs/mc/src/unit.cpp:102-163,232-257
  *   repartition ........... apps/libs/mc/src/unit.cpp:67-100,190-230
  *   models ................ apps/libs/models/public/models/monod.hpp:80-193,
- *                           fixed_length.hpp:360-413, simple_acetate.hpp:132-248, utils.hpp:63-67
+ *                           fixed_length.hpp:108-161, simple_acetate.hpp:132-248, utils.hpp:63-67
  *                           (monod_mass, dual_monod, quad_monod: the benchmark cases' models;
  *                           udf_minimal: apps/udf_model/minimal.cpp:23-107, and udf_starvation,
  *                           the example plugins; the latter's Dead status: mc/alias.hpp:124-130)
@@ -183,7 +183,7 @@ static double trunc_normal(double mu, double sd, double lo, double hi, double u)
 
 static const int MODEL_NVAR[ORC_MODEL_COUNT] = {2, 3, 6, 2, 9, 8, 2, 3};
 static const int MODEL_NS[ORC_MODEL_COUNT] = {1, 2, 1, 1, 2, 4, 1, 1};
-static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 4, 0, 15, 5, 7};
+static const int MODEL_NP[ORC_MODEL_COUNT] = {6, 10, 6, 5, 0, 15, 5, 7};
 
 int orc_model_info(int model, int* n_var, int* n_species, int* n_params) {
   if (model < 0 || model >= ORC_MODEL_COUNT) return -1;
@@ -206,8 +206,8 @@ int orc_model_default_params(int model, double* p, int n) {
     case ORC_MODEL_MONOD: /* mu_max, k_s, y_s_x, l_max, d_m, tau_meta */
       p[0] = 0.77 / 3600.0; p[1] = 1e-3; p[2] = 2.0; p[3] = 2e-6; p[4] = 0.6e-6; p[5] = 3600.0 / 0.77;
       break;
-    case ORC_MODEL_FIXED_LENGTH: /* l_dot_max, l_max, k, d_m */
-      p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6;
+    case ORC_MODEL_FIXED_LENGTH: /* l_dot_max, l_max, k, d_m, lambda (config_loader.cpp:16) */
+      p[0] = 2e-6 / 3600.0; p[1] = 2e-6; p[2] = 1e-3; p[3] = 0.6e-6; p[4] = 0.69314718055994530942 / 1e-6;
       break;
     case ORC_MODEL_QUAD_MONOD: /* qs_max, Ks, Ko, Ka, qa_max, Yxs, Yxa, Yos, Yoa, Yas, tau, Yco2, m_div, m0 mean, m0 sd */
       p[0] = 2.0; p[1] = 0.01; p[2] = 1e-4; p[3] = 0.01; p[4] = 0.5; p[5] = 0.5; p[6] = 0.3; p[7] = 0.3;
@@ -263,10 +263,16 @@ static void model_init(int model, const double* p, double* x, uint64_t seed, uin
       x[5] = 0.0;
       break;
     }
-    case ORC_MODEL_FIXED_LENGTH:
-      x[0] = p[1] * (0.5 + 0.5 * orc_u01(w[0]));
+    case ORC_MODEL_FIXED_LENGTH: { /* fixed_length.hpp:108-120 with config_loader.cpp:13-36: the
+                                    * Metropolis target lambda*exp(-lambda*l) on [l_max/2, l_max],
+                                    * sampled exactly by its inverse CDF (lambda = 0: uniform) */
+      const double a = p[1] / 2.0, b = p[1], lam = p[4];
+      const double u = orc_u01_52(w[0], w[1]);
+      const double l = lam != 0.0 ? a - log1p(-u * -expm1(-lam * (b - a))) / lam : a + u * (b - a);
+      x[0] = l < a ? a : (l > b ? b : l);
       x[1] = p[1];
       break;
+    }
     case ORC_MODEL_UDF_MINIMAL: /* minimal.cpp:55-63: length uniform on (l_max/2, l_max) (:121-140) */
       x[0] = p[2] * (0.5 + 0.5 * orc_u01(w[0]));
       x[1] = p[2];
@@ -1126,7 +1132,7 @@ static void particle_tile(orc_run* r, const flowmap* f, uint64_t step, uint32_t 
       for (int s = 0; s < ns; ++s) S[(size_t)s * n_c + pos] += r->weight * contrib[s];
     uint32_t w[4];
     draw4(r->seed, (uint32_t)p, (uint32_t)step, TAG_TRANSPORT, 0u, w);
-    /* move: move_kernel.hpp:392-437 with the linear test of probability_leaving.hpp:313-325 */
+    /* move: move_kernel.hpp:392-437 with the linear test of probability_leaving.hpp:33-46 */
     if (enable_move && (stages & ORC_STAGE_MOVE)) {
       const double u1 = orc_u01(w[0]);
       if (f->p_move[pos] > u1) {

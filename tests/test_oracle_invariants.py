@@ -172,7 +172,7 @@ def two_box(oracle_lib, F=(3e-3, 1e-3), V=(1e-3, 2e-3), n=100000, dt=0.01, seed=
 
 
 def test_move_probability_is_linear(oracle_lib):
-    """Move test dt*Q/V > u (probability_leaving.hpp:313-325)."""
+    """Move test dt*Q/V > u (probability_leaving.hpp:33-46)."""
     r = two_box(oracle_lib)
     before = r.particles()["pos"]
     r.step_stages(MOVE)

@@ -84,6 +84,40 @@ def test_admitted_run_starts_at_its_own_step_zero():
     eng.close()
 
 
+def test_retired_first_run_leaves_no_reads_behind():
+    """Run 0 is large (40 000 compartments: a 1.3 MB compartment record) and
+    retires first; its arrays are freed while the other runs keep stepping.
+    The particle pass's phantom sub-chunks (the prefetch past the last group)
+    stand at the last live tile, never at run 0, so the survivors step on
+    memory they own and still match their oracles over many steps, with
+    every launch shape of the pass."""
+    import os
+
+    import mcst
+    from mcst import cases
+
+    big = _fast(cases.ensemble_run(0, seed=31, n_c_range=(40000, 40000), n_range=(30000, 30000)))
+    small = [_fast(cases.ensemble_run(i, seed=31, n_c_range=(40, 300), n_range=(3000, 9000))) for i in (1, 2)]
+    for block in ("256", "512"):
+        os.environ["MCST_PARTICLE_BLOCK"] = block
+        try:
+            eng = mcst.Engine("monod_mass")
+            runs = []
+            for c in [big] + small:
+                r = eng.add_run(c.n_c, c.dt, c.seed)
+                c.apply(r)
+                runs.append(r)
+            eng.commit()
+            eng.step(3)
+            eng.retire(runs[0])
+            eng.step(40)
+            for r, c in zip(runs[1:], small):
+                assert_parity(r, _oracle(c, 43), probes=False)
+            eng.close()
+        finally:
+            del os.environ["MCST_PARTICLE_BLOCK"]
+
+
 def test_admit_reports_a_full_pool_and_retries():
     import mcst
     from mcst import cases
