@@ -6,7 +6,7 @@ cyclic flowmaps swapped every 100 steps, 2 liquid species, 1e7 particles of
 the dual-substrate Monod model (3 FP64 states + the FP64 division age; the
 network has no outlet, so no hydraulic age), dt = 0.01 s.
 A step is one pass of the whole pipeline (transport, fused particle pass,
-bookkeeping, removal, merge) over all particles, with every input resident
+mover/division resolution with the bookkeeping, removal) over all particles, with every input resident
 in HBM.  particle-steps = sum over steps of the Idle particles processed.
 
 Prints one JSON line.  --gpus above 1 is refused: this build targets one GPU.
@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "biocma-mcst_amd"))
 
 HBM_PEAK_GBS = 8000.0
+KERNELS = ["transport_layout", "particle", "resolve_bookkeep", "removal"]  # mcst_engine_profile's order
 METRIC = "particle-steps/s at 1e7 particles (% HBM peak); served p50/p99 run latency"
 
 
@@ -262,7 +263,7 @@ def lattice_measurement(n_particles=100_000_000, steps=200, profile_steps=5, sor
             "steps": steps, "particle_steps_per_s": ps / el, "ms_per_step": el * 1e3 / steps,
             "sort_interval": sort_interval,
             "births": int(c1["births"]), "exits": int(c1["exits"]), "compactions": int(c1["compactions"]),
-            "kernel_ms_per_step": dict(zip(["transport_layout", "particle", "resolve", "bookkeep", "removal", "merge"],
+            "kernel_ms_per_step": dict(zip(KERNELS,
                                            map(float, per))),
             "particle_pass": {"B_ps": b_ps, "achieved_GBps": achieved, "frac_of_8TBps": achieved / HBM_PEAK_GBS}}
 
@@ -384,7 +385,7 @@ def main():
     per_launch = (run.counters()["processed"] - p0) / a.profile_steps
     t_particle = per[1] * 1e-3
     achieved = per_launch * b_ps / t_particle / 1e9
-    names = ["transport_layout", "particle", "resolve", "bookkeep", "removal", "merge"]
+    names = KERNELS
     log("kernel ms/step: " + ", ".join(f"{n}={v:.4f}" for n, v in zip(names, per)))
 
     traffic = read_traffic()

@@ -323,7 +323,7 @@ struct HostRun {
   double* d_vf = nullptr;          // device: this step's flows of vf, then the outlets' thresholds
   double ratio = 0.01;
   uint64_t min_removal = 0;
-  uint32_t stage_cap = 64;
+  uint32_t div_cap = 0xFFFFFFFFu; // newborns per step at most (the reference's division buffer)
   uint32_t probe_cap = 0;
   uint32_t step_target = 0xFFFFFFFFu;
   uint32_t sort_interval = 0;    // reorder the particles by compartment every k steps (0: never)
@@ -386,7 +386,6 @@ struct mcst_engine {
   size_t sort_temp = 0;
   int tuned_group = -1;
   int grid_move = 1;
-  int grid_merge = 1;
   bool removal_possible = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -535,8 +534,6 @@ void exit_thresholds(const HostRun& r, HostMap& mp) {
 int check_definable(const mcst_engine* e, const HostRun& h, uint32_t r) {
   if (h.maps.empty()) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no flowmap");
   if (!h.has_particles) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no particles");
-  if (h.stage_cap > e->bd.stage_stride)
-    return fail(MCST_ERR_ARG, "division cap above the engine's staging rows");
   return MCST_OK;
 }
 
@@ -629,11 +626,6 @@ int choose_launch(mcst_engine* e) {
   const uint64_t resident = (uint64_t)prop.multiProcessorCount * (uint64_t)std::max(1, res_per_cu);
   const uint64_t n_sc = (uint64_t)b.T_max * b.nsub;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>(resident, (n_sc + 31) / 32));
-  // merge: 8 lanes per tile of the pool, at most a wave of 8 blocks per CU
-  // (the kernel strides over the step's tiles; most steps of a large pool
-  // have few live tiles, and idle blocks still cost their launch)
-  e->grid_merge = (int)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 8, ((uint64_t)b.T_max * 8 + BLOCK - 1) / BLOCK));
   if (std::getenv("MCST_VERBOSE"))
     std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u (lds cap %zu)\n",
                  best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, lds_cap);
@@ -810,9 +802,8 @@ int upload_run(mcst_engine* e, uint32_t r) {
   d.n_maps = (uint32_t)h.maps.size();
   d.steps_per_map = h.steps_per_map;
   d.map_base = h.map_base;
-  d.stage_cap = h.stage_cap;
+  d.div_cap = h.div_cap;
   d.comp_off = 0;
-  d.tile_cap_off = (uint32_t)(h.p_off / TILE);
   d.probe_cap = h.probe_cap;
   d.step_target = h.step_target;
   RunState st;
@@ -885,17 +876,22 @@ int admit(mcst_engine* e, uint32_t* n_admitted) {
   return rc;
 }
 
+// Entries of the request block counts of a pool of T tiles (blocks of 2^6,
+// 2^12 and 2^18 sub-chunks; the resolve kernel clears entry n_sc >> 18 too).
+struct RequestLevels {
+  uint64_t n1, n2, n3;
+};
+RequestLevels request_levels(uint64_t T, uint32_t nsub) {
+  const uint64_t n_sc = T * nsub;
+  return RequestLevels{(n_sc + 63) >> 6, (n_sc + 4095) >> 12, (n_sc >> 18) + 1};
+}
+
 int commit(mcst_engine* e) {
   const ModelEntry& M = *e->model;
   const uint32_t R0 = (uint32_t)e->runs.size();
   if (R0 == 0 && e->reserve_runs == 0) return fail(MCST_ERR_STATE, "no run defined");
   uint64_t tiles = 0;
-  uint32_t stride = e->reserve_runs ? 64u : 1u;
-  for (uint32_t r = 0; r < R0; ++r) {
-    const HostRun& h = e->runs[r];
-    tiles += (h.cap + TILE - 1) / TILE;
-    stride = std::max(stride, h.stage_cap);
-  }
+  for (uint32_t r = 0; r < R0; ++r) tiles += (e->runs[r].cap + TILE - 1) / TILE;
   tiles = std::max<uint64_t>(tiles, (e->reserve_slots + TILE - 1) / TILE);
   if (tiles == 0) tiles = 1;
   const uint64_t P = tiles * TILE;
@@ -906,8 +902,6 @@ int commit(mcst_engine* e) {
   b = BatchDev{};
   b.P = P;
   b.T_max = (uint32_t)T_max;
-  b.stage_stride = stride;
-  b.ST = T_max * stride;
   b.R = 0;
   b.n_comp_total = 0;
   b.nsub = TILE / (64u * (uint32_t)M.nl);
@@ -921,10 +915,7 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.state, P * nv * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.age_hyd, P * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.age_div, P * sizeof(double)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.st_state, b.ST * nv * sizeof(double)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.st_pos, b.ST * sizeof(uint32_t)))) return rc;
-  uint32_t** tile_arrays[] = {&b.tile_births, &b.tile_birth_off, &b.tile_gap, &b.tile_fill, &b.tile_gap_off,
-                              &b.tile_fill_off};
+  uint32_t** tile_arrays[] = {&b.tile_gap, &b.tile_fill, &b.tile_gap_off, &b.tile_fill_off};
   for (uint32_t** a : tile_arrays)
     if ((rc = dev_alloc(e, (void**)a, T_max * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
@@ -938,7 +929,17 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&e->d_runs, (size_t)RC * sizeof(RunDev)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_rs, (size_t)RC * sizeof(RunState)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_comp, (size_t)(RC + 1) * sizeof(uint32_t)))) return rc;
-  HIP_TRY(hipMemset(b.tile_births, 0, T_max * sizeof(uint32_t)));
+  if ((rc = dev_alloc(e, (void**)&b.rm_tile_off, (RC + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.ticket, sizeof(uint32_t)))) return rc;
+  const RequestLevels lv = request_levels(T_max, b.nsub);
+  if ((rc = dev_alloc(e, (void**)&b.div_l1, lv.n1 * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_l2, lv.n2 * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_l3, lv.n3 * sizeof(uint32_t)))) return rc;
+  HIP_TRY(hipMemset(b.div_l1, 0, lv.n1 * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.div_l2, 0, lv.n2 * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.div_l3, 0, lv.n3 * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.ticket, 0, sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.rm_tile_off, 0, (RC + 1) * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.tile_off, 0, (RC + 1) * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.sub_moves, 0, T_max * b.nsub * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.sub_divs, 0, T_max * b.nsub * sizeof(uint32_t)));
@@ -985,7 +986,7 @@ int retire(mcst_engine* e, uint32_t r) {
 
 // The kernels of one step of the batch.  With `ev`, records ev[0] before
 // the transport and ev[k] after the k-th stage (transport, particle pass,
-// resolve, bookkeeping, removal, merge) for mcst_engine_profile.
+// resolve with the bookkeeping, removal) for mcst_engine_profile.
 void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr) {
   if (e->bd.R == 0) return;  // no run admitted yet
   BatchDev b = e->bd;
@@ -1002,12 +1003,8 @@ void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr
   mark(2);
   ops.resolve(b, e->grid_move, e->stream);
   mark(3);
-  launch_bookkeep(b, e->stream);
-  mark(4);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
-  mark(5);
-  launch_merge(b, nv, e->grid_merge, e->stream);
-  mark(6);
+  mark(4);
 }
 
 int read_state(mcst_engine* e, int32_t run, RunState& s) {
@@ -1076,7 +1073,8 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
   const uint64_t T0 = b.T_max, P0 = b.P;
   const uint64_t T1 = std::max<uint64_t>(min_tiles, 2 * T0), P1 = T1 * TILE;
   if (P1 >= 0xFFFFFFFFull) return fail(MCST_ERR_MEMORY, "more than 2^32-1 particle slots in one engine");
-  const uint64_t nv = (uint64_t)e->model->nv, ST1 = T1 * b.stage_stride;
+  const uint64_t nv = (uint64_t)e->model->nv;
+  const RequestLevels lv = request_levels(T1, b.nsub);
   struct Move {
     void** slot;     // the BatchDev field
     size_t bytes;    // new size
@@ -1090,10 +1088,9 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
       {(void**)&b.age_hyd, P1 * 8, P0 * 8, 0, nullptr},
       {(void**)&b.age_div, P1 * 8, P0 * 8, 0, nullptr},
       {(void**)&b.state, P1 * nv * 8, P0 * 8, nv, nullptr},
-      {(void**)&b.st_state, ST1 * nv * 8, 0, 0, nullptr},
-      {(void**)&b.st_pos, ST1 * 4, 0, 0, nullptr},
-      {(void**)&b.tile_births, T1 * 4, 0, 0, nullptr},
-      {(void**)&b.tile_birth_off, T1 * 4, 0, 0, nullptr},
+      {(void**)&b.div_l1, lv.n1 * 4, 0, 0, nullptr},
+      {(void**)&b.div_l2, lv.n2 * 4, 0, 0, nullptr},
+      {(void**)&b.div_l3, lv.n3 * 4, 0, 0, nullptr},
       {(void**)&b.tile_gap, T1 * 4, 0, 0, nullptr},
       {(void**)&b.tile_fill, T1 * 4, 0, 0, nullptr},
       {(void**)&b.tile_gap_off, T1 * 4, 0, 0, nullptr},
@@ -1134,7 +1131,6 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
   }
   b.P = P1;
   b.T_max = (uint32_t)T1;
-  b.ST = ST1;
   give_tiles(e, (uint32_t)T0, (uint32_t)(T1 - T0));
   return MCST_OK;
 }
@@ -1185,7 +1181,6 @@ int regrow_run(mcst_engine* e, uint32_t r, uint64_t n, uint64_t new_cap) {
   RunDev& d = e->h_runs[r];
   d.p_off = h.p_off;
   d.cap = (uint32_t)new_cap;
-  d.tile_cap_off = start1;
   HIP_TRY(hipMemcpy(e->d_runs + r, &d, sizeof(RunDev), hipMemcpyHostToDevice));
   return MCST_OK;
 }
@@ -1504,8 +1499,8 @@ int mcst_run_set_growth(mcst_engine* e, int32_t run, uint32_t every, double fact
 
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap) {
   CHECK_DEFINE(e, run, h);
-  if (cap == 0 || cap > TILE) return fail(MCST_ERR_ARG, "division cap must be in 1..1024");
-  h->stage_cap = cap;
+  if (cap == 0) return fail(MCST_ERR_ARG, "division cap must be at least 1");
+  h->div_cap = cap;
   return MCST_OK;
 }
 
@@ -1811,7 +1806,7 @@ int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, doub
   CHECK_ENGINE(e);
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
   HIP_TRY(hipSetDevice(e->device));
-  constexpr int K = 6;
+  constexpr int K = 4;
   // every event created here is destroyed on every return path
   struct Events {
     std::vector<hipEvent_t> v;

@@ -2,18 +2,21 @@
 //
 // One step of every active run is, in stream order:
 //   k_transport      liquid transport of all runs      (implScalar.cpp:250-264);
-//                    its last block lays out the step's tiles: tile -> run
-//                    table and division budgets (k_layout alone at commit)
-//   k_particle<M>    fused per-particle pass: model update + division staging
-//                    + contribution scatter + move + leave
+//                    its last block lays out the step's tiles: the tile -> run
+//                    table (k_layout alone at commit)
+//   k_particle<M>    fused per-particle pass: model update + division requests
+//                    + contribution scatter + move test + leave
 //                    (model_kernel.hpp:163-268, contribution_kernel.hpp:144-186,
 //                     move_kernel.hpp:209-273,347-437,586-648)
 //   k_resolve<M>     neighbour pick + exit test of the particles that left their
-//                    compartment; division acceptance, mother halving, staging
-//   k_bookkeep       per-run counters, removal decision  (particles_container.hpp:539-557)
+//                    compartment; division acceptance in slot order over the
+//                    run, mother halving, daughters written to [n, n + births)
+//                    (particles_container.hpp:559-599); its last block keeps the
+//                    books: counters, removal decision (particles_container.hpp:539-557)
 //   k_gap_count / k_gap_scan / k_gap_list / k_gap_fill
-//                    removal of Exit/Dead slots by gap filling (particles_container.hpp:292-385,735-796)
-//   k_merge          append staged newborns                (particles_container.hpp:403-443,575-599)
+//                    removal of Exit/Dead slots by gap filling over [0, n + births)
+//                    (particles_container.hpp:292-385,735-796), when a particle
+//                    can become inactive
 // Every kernel reads particle counts from device memory, so the sequence can
 // be replayed without host round trips.  The model-dependent kernels
 // (k_transport, k_particle, k_resolve, k_properties) are templates in
@@ -107,86 +110,30 @@ __device__ uint32_t block_scan(const uint32_t* in, uint32_t* out, uint32_t n, bo
 
 }  // namespace
 
-// ------------------------------------------------------------------ bookkeeping
-// One block per run: births of the pass (scan of tile counts), counters, and
-// the removal decision of update_and_remove_inactive.
-__global__ void __launch_bounds__(SCAN_THREADS) k_bookkeep(BatchDev b) {
-  const uint32_t r = blockIdx.x;
-  RunState* rs = b.rs + r;
-  if (!rs->active) return;
-  const RunDev& rd = b.runs[r];
-  const uint32_t t0 = b.tile_off[r];
-  const uint32_t nt = b.tile_off[r + 1] - t0;
-  // the run's counts, read before the scan (which changes none of them), so
-  // their loads overlap it instead of following its barriers
-  const uint32_t n = rs->n, inactive0 = rs->inactive, step0 = rs->step;
-  const unsigned long long exits = rs->step_exits, deaths = rs->step_deaths, moves = rs->step_moves,
-                           overflow = rs->step_overflow, processed = rs->step_processed;
-  unsigned long long cnt[C_COUNT];
-#pragma unroll
-  for (int k = 0; k < C_COUNT; ++k) cnt[k] = rs->counters[k];
-  const double ratio = rd.ratio;
-  const uint64_t min_removal = rd.min_removal;
-  const uint32_t births = block_scan(b.tile_births + t0, b.tile_birth_off + t0, nt, false);
-  if (threadIdx.x != 0) return;
-  cnt[C_BIRTHS] += births;
-  cnt[C_EXITS] += exits;
-  cnt[C_DEATHS] += deaths;
-  cnt[C_MOVES] += moves;
-  cnt[C_OVERFLOW] += overflow;
-  cnt[C_PROCESSED] += processed;
-  rs->step_exits = rs->step_deaths = rs->step_moves = rs->step_overflow = rs->step_processed = 0;
-  uint64_t inactive = (uint64_t)inactive0 + exits + deaths;
-  const uint64_t thr_ratio = (uint64_t)((double)n * ratio);
-  const uint64_t thr = min_removal > thr_ratio ? min_removal : thr_ratio;
-  uint32_t n_mid = n, compact = 0;
-  if ((b.stages & STAGE_REMOVE) && inactive > thr) {
-    cnt[C_COMPACTIONS] += 1;
-    if (inactive >= n) {
-      n_mid = 0;
-      compact = 2;  // everything removed, nothing to move
-    } else {
-      n_mid = n - (uint32_t)inactive;
-      compact = 1;
-    }
-    inactive = 0;
-  }
-  rs->n_prev = n;
-  rs->n_mid = n_mid;
-  rs->compact = compact;
-  rs->births = births;
-  rs->inactive = (uint32_t)inactive;
-  rs->n = n_mid + births;
-  rs->step = step0 + 1u;
-  cnt[C_STEPS] = step0 + 1u;
-  cnt[C_N] = n_mid + births;
-  cnt[C_INACTIVE] = (uint32_t)inactive;
-#pragma unroll
-  for (int k = 0; k < C_COUNT; ++k) rs->counters[k] = cnt[k];
-}
-
 // ------------------------------------------------------------------ removal
-// Gaps are the non-Idle slots below n_mid (ascending); fillers the Idle slots
-// in [n_mid, n_prev) (descending).  The i-th gap receives the i-th filler.
+// Over the tiles of [0, n_end) of every compacting run (the removal layout
+// rm_tile_off, written by the bookkeeping): gaps are the non-Idle slots below
+// n_keep (ascending); fillers the Idle slots in [n_keep, n_end) (descending),
+// the step's newborns among them.  The i-th gap receives the i-th filler.
 __global__ void __launch_bounds__(BLOCK) k_gap_count(BatchDev b) {
   __shared__ uint32_t s_g[WAVES], s_f[WAVES];
-  const uint32_t T = b.tile_off[b.R];
+  const uint32_t* off = b.rm_tile_off;
+  const uint32_t T = off[b.R];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
-    const uint32_t r = find_run(b.tile_off, b.R, t);
+    const uint32_t r = find_run(off, b.R, t);
     const RunState* rs = b.rs + r;
-    if (rs->compact != 1u) continue;
     const RunDev& rd = b.runs[r];
-    const uint32_t slot0 = (t - b.tile_off[r]) * TILE;
-    const uint32_t n_prev = rs->n_prev, n_mid = rs->n_mid;
+    const uint32_t slot0 = (t - off[r]) * TILE;
+    const uint32_t n_end = rs->n_end, n_keep = rs->n_keep;
     uint32_t ng = 0, nf = 0;
 #pragma unroll
     for (uint32_t k = 0; k < PPT; ++k) {
       const uint32_t s = slot0 + k * BLOCK + tid;
-      if (s < n_prev) {
+      if (s < n_end) {
         const bool live = b.status[rd.p_off + s] == IDLE;
-        ng += (s < n_mid && !live) ? 1u : 0u;
-        nf += (s >= n_mid && live) ? 1u : 0u;
+        ng += (s < n_keep && !live) ? 1u : 0u;
+        nf += (s >= n_keep && live) ? 1u : 0u;
       }
     }
     ng = wave_sum(ng);
@@ -213,8 +160,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_gap_scan(BatchDev b) {
   const uint32_t r = blockIdx.x;
   RunState* rs = b.rs + r;
   if (rs->compact != 1u) return;
-  const uint32_t t0 = b.tile_off[r];
-  const uint32_t nt = b.tile_off[r + 1] - t0;
+  const uint32_t t0 = b.rm_tile_off[r];
+  const uint32_t nt = b.rm_tile_off[r + 1] - t0;
   const uint32_t g = block_scan(b.tile_gap + t0, b.tile_gap_off + t0, nt, false);
   const uint32_t f = block_scan(b.tile_fill + t0, b.tile_fill_off + t0, nt, true);
   // gaps and fillers are equinumerous when the inactive count is right; if
@@ -250,57 +197,59 @@ __device__ __forceinline__ uint32_t tile_ranks(const bool (&f)[PPT], uint32_t (&
 
 __global__ void __launch_bounds__(BLOCK) k_gap_list(BatchDev b) {
   __shared__ uint32_t s_cnt[PPT][WAVES];
-  const uint32_t T = b.tile_off[b.R];
+  const uint32_t* off = b.rm_tile_off;
+  const uint32_t T = off[b.R];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
-    const uint32_t r = find_run(b.tile_off, b.R, t);
+    if (b.tile_fill[t] == 0) continue;
+    const uint32_t r = find_run(off, b.R, t);
     const RunState* rs = b.rs + r;
-    if (rs->compact != 1u || b.tile_fill[t] == 0) continue;
     const RunDev& rd = b.runs[r];
-    const uint32_t slot0 = (t - b.tile_off[r]) * TILE;
-    const uint32_t n_prev = rs->n_prev, n_mid = rs->n_mid;
+    const uint32_t slot0 = (t - off[r]) * TILE;
+    const uint32_t n_end = rs->n_end, n_keep = rs->n_keep;
     bool f[PPT];
     uint32_t rank[PPT];
 #pragma unroll
     for (uint32_t k = 0; k < PPT; ++k) {
       const uint32_t s = slot0 + k * BLOCK + tid;
-      f[k] = s < n_prev && s >= n_mid && b.status[rd.p_off + s] == IDLE;
+      f[k] = s < n_end && s >= n_keep && b.status[rd.p_off + s] == IDLE;
     }
     const uint32_t tot = tile_ranks(f, rank, s_cnt, lane, wave);
-    const uint32_t off = b.tile_fill_off[t];
+    const uint32_t fo = b.tile_fill_off[t];
 #pragma unroll
     for (uint32_t k = 0; k < PPT; ++k)
-      if (f[k]) b.fill_src[rd.p_off + off + (tot - 1u - rank[k])] = slot0 + k * BLOCK + tid;
+      if (f[k]) b.fill_src[rd.p_off + fo + (tot - 1u - rank[k])] = slot0 + k * BLOCK + tid;
   }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_gap_fill(BatchDev b, uint32_t n_var) {
   __shared__ uint32_t s_cnt[PPT][WAVES];
-  const uint32_t T = b.tile_off[b.R];
+  const uint32_t* off = b.rm_tile_off;
+  const uint32_t T = off[b.R];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
-    const uint32_t r = find_run(b.tile_off, b.R, t);
+    if (b.tile_gap[t] == 0) continue;
+    const uint32_t r = find_run(off, b.R, t);
     const RunState* rs = b.rs + r;
-    if (rs->compact != 1u || b.tile_gap[t] == 0) continue;
     const RunDev& rd = b.runs[r];
-    const uint32_t slot0 = (t - b.tile_off[r]) * TILE;
-    const uint32_t n_mid = rs->n_mid;
+    const uint32_t slot0 = (t - off[r]) * TILE;
+    const uint32_t n_keep = rs->n_keep;
     bool f[PPT];
     uint32_t rank[PPT];
 #pragma unroll
     for (uint32_t k = 0; k < PPT; ++k) {
       const uint32_t s = slot0 + k * BLOCK + tid;
-      f[k] = s < n_mid && b.status[rd.p_off + s] != IDLE;
+      f[k] = s < n_keep && b.status[rd.p_off + s] != IDLE;
     }
     tile_ranks(f, rank, s_cnt, lane, wave);
-    const uint32_t off = b.tile_gap_off[t];
+    const uint32_t go = b.tile_gap_off[t];
     const uint32_t n_fill = rs->gap_total;
-    const uint32_t n_prev = rs->n_prev;
+    const uint32_t n_end = rs->n_end;
 #pragma unroll
     for (uint32_t k = 0; k < PPT; ++k) {
-      if (!f[k] || off + rank[k] >= n_fill) continue;
-      const uint32_t src_slot = b.fill_src[rd.p_off + off + rank[k]];
-      if (src_slot >= n_prev) continue;
+      if (!f[k] || go + rank[k] >= n_fill) continue;
+      const uint32_t src_slot = b.fill_src[rd.p_off + go + rank[k]];
+      if (src_slot >= n_end) continue;
       const uint64_t dst = rd.p_off + slot0 + k * BLOCK + tid;
       const uint64_t src = rd.p_off + src_slot;
       b.pos[dst] = b.pos[src];
@@ -313,51 +262,6 @@ __global__ void __launch_bounds__(BLOCK) k_gap_fill(BatchDev b, uint32_t n_var) 
 }
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_layout(BatchDev b) { layout_pass<SCAN_THREADS>(b); }
-
-// ------------------------------------------------------------------ merge
-// Appends staged newborns in tile order.  MERGE_LANES lanes per tile: lane k
-// copies the tile's newborns k, k + MERGE_LANES, ... with all of a newborn's
-// fields loaded before any is stored.  A tile with the full budget of
-// newborns (up to 64) then costs a lane a few round trips instead of one
-// per newborn, which set the kernel's length when one lane took a whole tile.
-constexpr uint32_t MERGE_LANES = 8;
-
-template <uint32_t NV>
-__global__ void __launch_bounds__(BLOCK) k_merge(BatchDev b) {
-  const uint32_t T = b.tile_off[b.R];
-  const uint32_t* __restrict__ st_pos = b.st_pos;
-  const double* __restrict__ st_state = b.st_state;
-  uint32_t* __restrict__ pos = b.pos;
-  uint8_t* __restrict__ status = b.status;
-  double* __restrict__ state = b.state;
-  double* __restrict__ age_hyd = b.age_hyd;
-  double* __restrict__ age_div = b.age_div;
-  const uint64_t n_lanes = (uint64_t)T * MERGE_LANES;
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n_lanes;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t t = (uint32_t)(q / MERGE_LANES), k = (uint32_t)(q % MERGE_LANES);
-    const uint32_t nb = b.tile_births[t];
-    if (k >= nb) continue;
-    const uint32_t r = find_run(b.tile_off, b.R, t);
-    const RunDev& rd = b.runs[r];
-    const RunState* rs = b.rs + r;
-    const uint64_t dst0 = rd.p_off + rs->n_mid + b.tile_birth_off[t];
-    const uint64_t src0 = (uint64_t)t * b.stage_stride;
-    for (uint32_t j = k; j < nb; j += MERGE_LANES) {
-      const uint64_t sj = src0 + j, d = dst0 + j;
-      double x[NV];
-#pragma unroll
-      for (uint32_t v = 0; v < NV; ++v) x[v] = st_state[(uint64_t)v * b.ST + sj];
-      const uint32_t p = st_pos[sj];
-#pragma unroll
-      for (uint32_t v = 0; v < NV; ++v) state[(uint64_t)v * b.P + d] = x[v];
-      pos[d] = p;
-      status[d] = IDLE;
-      age_hyd[d] = 0.0;
-      age_div[d] = 0.0;
-    }
-  }
-}
 
 // ------------------------------------------------------------------ histogram
 // Idle particles per compartment of run r (getRepartition, unit.cpp:190-230).
@@ -381,31 +285,11 @@ const ModelOps* builtin_model_ops(int model_id) {
   return &ops[model_id];
 }
 
-void launch_bookkeep(const BatchDev& b, hipStream_t s) {
-  hipLaunchKernelGGL(k_bookkeep, dim3(b.R), dim3(SCAN_THREADS), 0, s, b);
-}
-
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_gap_count, dim3(grid), dim3(BLOCK), 0, s, b);
   hipLaunchKernelGGL(k_gap_scan, dim3(b.R), dim3(SCAN_THREADS), 0, s, b);
   hipLaunchKernelGGL(k_gap_list, dim3(grid), dim3(BLOCK), 0, s, b);
   hipLaunchKernelGGL(k_gap_fill, dim3(grid), dim3(BLOCK), 0, s, b, n_var);
-}
-
-// k_merge for the model's state size (its register arrays hold two newborns)
-template <uint32_t NV>
-void launch_merge_t(const BatchDev& b, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_merge<NV>, dim3(grid), dim3(BLOCK), 0, s, b);
-}
-void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) {
-  static_assert(MAX_VAR == 16, "one instance per state size");
-  using L = void (*)(const BatchDev&, int, hipStream_t);
-  static const L table[MAX_VAR] = {
-      launch_merge_t<1>,  launch_merge_t<2>,  launch_merge_t<3>,  launch_merge_t<4>,
-      launch_merge_t<5>,  launch_merge_t<6>,  launch_merge_t<7>,  launch_merge_t<8>,
-      launch_merge_t<9>,  launch_merge_t<10>, launch_merge_t<11>, launch_merge_t<12>,
-      launch_merge_t<13>, launch_merge_t<14>, launch_merge_t<15>, launch_merge_t<16>};
-  if (n_var >= 1 && n_var <= (uint32_t)MAX_VAR) table[n_var - 1](b, grid, s);
 }
 
 void launch_layout(const BatchDev& b, hipStream_t s) {

@@ -36,9 +36,7 @@ struct ModelOps {
 };
 const ModelOps* builtin_model_ops(int model_id);
 
-void launch_bookkeep(const BatchDev& b, hipStream_t s);
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
-void launch_merge(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_layout(const BatchDev& b, hipStream_t s);
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s);
 

@@ -26,6 +26,7 @@ constexpr int MAX_SPECIES = 8;
 
 // Particle status byte (apps/libs/mc/public/mc/alias.hpp:124-130).
 enum Status : uint8_t { IDLE = 0, DIVISION = 1, EXIT = 2, DEAD = 3 };
+constexpr uint16_t MOVER_INACTIVE = 0x8000u;  // mover-list flag: the slot moves but does not leave
 
 // Stage mask of one step (values shared with the C ABI).
 enum Stage : uint32_t {
@@ -112,9 +113,8 @@ struct RunDev {
   uint32_t n_maps;
   uint32_t steps_per_map;
   uint32_t map_base;        // index of the run's first MapDev
-  uint32_t stage_cap;       // staging slots per tile (division budget cap)
+  uint32_t div_cap;         // newborns per step at most (the reference's division buffer)
   uint32_t comp_off;        // first compartment of the run in the batch transport space
-  uint32_t tile_cap_off;    // first staging tile id reserved for the run
   uint32_t probe_cap;
   uint32_t step_target;     // the run stops stepping at this step count
 };
@@ -122,18 +122,18 @@ struct RunDev {
 // Mutable per-run state, updated by the device every step.
 struct RunState {
   uint32_t n;               // used slots (Idle + inactive)
-  uint32_t n_prev;          // slots during the last particle pass
-  uint32_t n_mid;           // slots after removal, before the merge
+  uint32_t n_end;           // slots after the last step's newborns joined: the removal's range
+  uint32_t n_keep;          // slots the removal keeps (n_end - inactive)
   uint32_t inactive;        // Exit/Dead slots below n
   uint32_t step;
-  uint32_t budget;          // divisions allowed per tile in the next pass
   uint32_t compact;         // removal requested by the last bookkeeping
-  uint32_t births;          // newborns staged in the last pass
+  uint32_t births;          // newborns of the last step
   uint32_t tile_begin;      // first global tile of the run in the current layout
   uint32_t n_tiles;         // tiles of the run in the current layout
   uint32_t gap_total;
   uint32_t active;          // 1 while step < step_target
-  unsigned long long step_exits, step_deaths, step_moves, step_overflow, step_processed;
+  uint32_t pad;
+  unsigned long long step_exits, step_deaths, step_moves, step_requests, step_processed;
   unsigned long long counters[C_COUNT];
   unsigned long long probe_exit_n, probe_div_n;
 };
@@ -147,14 +147,16 @@ struct BatchDev {
   double* age_hyd;
   double* age_div;
   uint64_t P;               // total slots
-  // staging for newborns: tile t owns [t*stage_stride, (t+1)*stage_stride)
-  double* st_state;         // [n_var][T_max*stage_stride]
-  uint32_t* st_pos;
-  uint64_t ST;              // T_max * stage_stride
-  uint32_t stage_stride;
-  // per-tile bookkeeping of the current layout
-  uint32_t* tile_births;
-  uint32_t* tile_birth_off;
+  // division requests of the pass per block of 2^6, 2^12 and 2^18
+  // sub-chunks (global sub-chunk index of the layout); with sub_divs they
+  // give the requests before any sub-chunk in 64 terms per level (128 at the
+  // top level for pools of more than 2^31 slots)
+  uint32_t* div_l1;
+  uint32_t* div_l2;
+  uint32_t* div_l3;
+  uint32_t* ticket;         // blocks of the resolve kernel that are done (its last block keeps the books)
+  uint32_t* rm_tile_off;    // [R+1] removal layout: first global tile of each run's [0, n_end)
+  // per-tile bookkeeping of the removal
   uint32_t* tile_gap;
   uint32_t* tile_fill;
   uint32_t* tile_gap_off;
@@ -162,7 +164,8 @@ struct BatchDev {
   uint32_t* fill_src;       // [P] scratch list of filler slots during removal
   uint32_t* tile_off;       // [R+1] first global tile of each run in the current layout
   // lists of the particle pass, one segment of TILE/nsub entries per sub-chunk
-  uint16_t* move_slot;      // [T_max*TILE] tile-local slots of the particles leaving their compartment
+  uint16_t* move_slot;      // [T_max*TILE] tile-local slots of the particles leaving their compartment,
+                            // | MOVER_INACTIVE for a slot that is not Idle (no exit test)
   uint32_t* move_pos;       // [T_max*TILE] their compartment before the move
   uint16_t* div_slot;       // [T_max*TILE] tile-local slots of the division requests
   uint32_t* div_pos;        // [T_max*TILE] their compartment (the newborn's)

@@ -197,14 +197,14 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
 // Counters of the tiles a block processed for one run, flushed with one
 // atomic each when the block moves on to another run (or finishes).
 struct TileCounts {
-  uint32_t exits = 0, deaths = 0, moves = 0, overflow = 0, processed = 0;
+  uint32_t exits = 0, deaths = 0, moves = 0, requests = 0, processed = 0;
 };
 
 __device__ __forceinline__ void flush_counts(RunState* rs, TileCounts& c) {
   if (c.exits) atomicAdd(&rs->step_exits, (unsigned long long)c.exits);
   if (c.deaths) atomicAdd(&rs->step_deaths, (unsigned long long)c.deaths);
   if (c.moves) atomicAdd(&rs->step_moves, (unsigned long long)c.moves);
-  if (c.overflow) atomicAdd(&rs->step_overflow, (unsigned long long)c.overflow);
+  if (c.requests) atomicAdd(&rs->step_requests, (unsigned long long)c.requests);
   if (c.processed) atomicAdd(&rs->step_processed, (unsigned long long)c.processed);
   c = TileCounts{};
 }
@@ -329,8 +329,8 @@ __device__ __forceinline__ void st_status(GLOBAL_AS uint8_t* p, uint32_t v) {
 // the tile's lists, so sub-chunks are independent: the block's waves take
 // the sub-chunks of one group (one per wave) and groups are dealt to blocks G
 // at a time (b.tile_group; 0: one contiguous range per block).  Which
-// requests a tile accepts (the first `budget` in slot order) is decided
-// after the pass (k_resolve).  With PRIV the block accumulates the
+// requests the run accepts (the first min(cap - n, div_cap) in slot order)
+// is decided after the pass (k_resolve).  With PRIV the block accumulates the
 // contributions of one run in LDS (ds_add_f64), switching runs only between
 // groups; sub-chunks of another run add to S directly.
 //
@@ -575,24 +575,24 @@ __global__ void __launch_bounds__(BLK)
       ah[j] = A.ah[j];
 #pragma unroll
       for (int q = 0; q < NS; ++q) contrib[j][q] = 0.0;
-      if (!idle[j]) continue;
-      int code = IDLE;
-      if (do_model) {
-        ad[j] += dt;
-        code = M::update(p, dt, x[j], rr[j], contrib[j]);
-      }
-      if constexpr (M::CAN_DIE) {
-        if (code == DEAD) {  // a dead cell stops contributing, moving and leaving
-          dead[j] = true;
-          continue;
+      if (!valid[j]) continue;
+      if (idle[j]) {
+        int code = IDLE;
+        if (do_model) {
+          ad[j] += dt;
+          code = M::update(p, dt, x[j], rr[j], contrib[j]);
         }
+        if constexpr (M::CAN_DIE) dead[j] = code == DEAD;  // stops contributing and leaving
+        want[j] = code == DIVISION;
       }
-      want[j] = code == DIVISION;
+      // every slot below the count takes the move test, whatever its status
+      // (move_kernel.hpp:209-273: Exit and Dead slots waiting for removal
+      // keep moving); a particle that leaves its compartment (dt*Q/V > u,
+      // probability_leaving.hpp:33-46) is listed, and k_resolve finds its
+      // neighbour and, for an Idle one, runs its exit test there
       const U4 u = draw4(seed, cur.slot0 + off(j), step, TAG_TRANSPORT, 0u);
-      // a particle that leaves its compartment (dt*Q/V > u, probability_leaving.hpp:33-46)
-      // is listed; k_resolve finds its neighbour and runs its exit test
       mover[j] = do_move && rr[j][NR] > u01(u.x);
-      if (do_leave) {
+      if (do_leave && idle[j] && !dead[j]) {
         ah[j] += dt;
         const double qx = rr[j][NR + 1];
         if (!mover[j] && qx < 1.0 && u01(u.z) > qx) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
@@ -713,7 +713,8 @@ __global__ void __launch_bounds__(BLK)
 #pragma unroll
       for (int j = 0; j < NL; ++j) {
         if (mover[j]) {
-          gp(b.move_slot)[seg + rm[j]] = (uint16_t)(slot_t + off(j));
+          const bool leaves = idle[j] && !dead[j];
+          gp(b.move_slot)[seg + rm[j]] = (uint16_t)((slot_t + off(j)) | (leaves ? 0u : (uint32_t)MOVER_INACTIVE));
           gp(b.move_pos)[seg + rm[j]] = A.pos[j];
         }
         if (n_div && want[j]) {  // rare
@@ -725,6 +726,11 @@ __global__ void __launch_bounds__(BLK)
     if (lane == 0 && sc < n_sc) {
       gp(b.sub_moves)[sc] = n_mov;
       gp(b.sub_divs)[sc] = n_div;
+      if (n_div) {  // rare: the block counts the resolve kernel's prefixes start from
+        atomicAdd(b.div_l1 + (sc >> 6), n_div);
+        atomicAdd(b.div_l2 + (sc >> 12), n_div);
+        atomicAdd(b.div_l3 + (sc >> 18), n_div);
+      }
     }
     // ---- counters of the sub-chunk (wave-uniform)
 #pragma unroll
@@ -734,6 +740,7 @@ __global__ void __launch_bounds__(BLK)
       cnt_acc.deaths += (uint32_t)__popcll(__ballot(dead[j]));
     }
     cnt_acc.moves += n_mov;
+    cnt_acc.requests += n_div;
 
     __builtin_amdgcn_sched_barrier(0);
     cur = nxt;
@@ -752,16 +759,124 @@ __global__ void __launch_bounds__(BLK)
   if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
 }
 
+// The books of every run after its step, kept by the last block of
+// k_resolve: counters, births (the accepted requests), the removal decision
+// of update_and_remove_inactive (particles_container.hpp:539-557, threshold
+// from the pass's count) over [0, n_end) where the newborns already sit, and
+// the removal's tile layout; then the request block counts are cleared for
+// the next pass (n_sc: the sub-chunks of this step's layout).
+__device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs, uint32_t n_sc) {
+  constexpr uint32_t NT = 256, NWV = NT / 64;
+  __shared__ uint32_t s_w[NWV];
+  __shared__ uint32_t s_carry;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t r0 = 0; r0 < b.R; r0 += NT) {
+    const uint32_t r = r0 + tid;
+    uint32_t tiles = 0;  // tiles of [0, n_end) when the run compacts
+    if (r < b.R) {
+      RunState* rs = b.rs + r;
+      if (rs->active) {
+        const RunDev& rd = runs[r];
+        const uint32_t n = rs->n;
+        const unsigned long long req = rs->step_requests, exits = rs->step_exits, deaths = rs->step_deaths;
+        const uint32_t room = rd.cap - n;
+        const uint32_t budget = room < rd.div_cap ? room : rd.div_cap;
+        const uint32_t births = req < budget ? (uint32_t)req : budget;
+        unsigned long long* cnt = rs->counters;
+        cnt[C_BIRTHS] += births;
+        cnt[C_EXITS] += exits;
+        cnt[C_DEATHS] += deaths;
+        cnt[C_MOVES] += rs->step_moves;
+        cnt[C_OVERFLOW] += req - births;
+        cnt[C_PROCESSED] += rs->step_processed;
+        rs->step_exits = rs->step_deaths = rs->step_moves = rs->step_requests = rs->step_processed = 0;
+        uint64_t inactive = (uint64_t)rs->inactive + exits + deaths;
+        const uint64_t thr_ratio = (uint64_t)((double)n * rd.ratio);
+        const uint64_t thr = rd.min_removal > thr_ratio ? rd.min_removal : thr_ratio;
+        const uint32_t n_end = n + births;
+        uint32_t n_keep = n_end, compact = 0;
+        if ((b.stages & STAGE_REMOVE) && inactive > thr) {
+          cnt[C_COMPACTIONS] += 1;
+          if (inactive >= n_end) {
+            n_keep = 0;
+            compact = 2;  // everything removed, nothing to move
+          } else {
+            n_keep = n_end - (uint32_t)inactive;
+            compact = 1;
+            tiles = (n_end + TILE - 1u) / TILE;
+          }
+          inactive = 0;
+        }
+        const uint32_t step = rs->step + 1u;
+        rs->n_end = n_end;
+        rs->n_keep = n_keep;
+        rs->compact = compact;
+        rs->births = births;
+        rs->inactive = (uint32_t)inactive;
+        rs->n = n_keep;
+        rs->step = step;
+        cnt[C_STEPS] = step;
+        cnt[C_N] = n_keep;
+        cnt[C_INACTIVE] = (uint32_t)inactive;
+      } else {
+        rs->compact = 0;
+      }
+    }
+    const uint32_t inc = wave_incl_scan(tiles, lane);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (uint32_t w = 0; w < NWV; ++w) {
+      if (w < wave) wpre += s_w[w];
+      tot += s_w[w];
+    }
+    const uint32_t carry = s_carry;
+    if (r < b.R) b.rm_tile_off[r] = carry + wpre + inc - tiles;
+    __syncthreads();
+    if (tid == 0) s_carry = carry + tot;
+    __syncthreads();
+  }
+  if (tid == 0) b.rm_tile_off[b.R] = s_carry;
+  for (uint32_t i = tid; i < (n_sc + 63u) >> 6; i += NT) b.div_l1[i] = 0u;
+  for (uint32_t i = tid; i < (n_sc + 4095u) >> 12; i += NT) b.div_l2[i] = 0u;
+  for (uint32_t i = tid; i <= (n_sc >> 18); i += NT) b.div_l3[i] = 0u;
+  if (tid == 0) *b.ticket = 0u;
+}
+
 // After the particle pass, per group of 32 or 64 sub-chunks (one per lane of
 // the first wave): (1) the movers the pass listed take their neighbour, the
-// first CDF column at or above u, and run the exit test in the new
-// compartment (move_kernel.hpp:392-437, 586-648); (2) each tile accepts the
-// first `budget` division requests in slot order (the reference's division
-// buffer, model_kernel.hpp:163-268), halves the mother and stages the
-// newborn at its rank.  Every mover and every request is independent, so
-// the processing order is free.
+// first CDF column at or above u, and an Idle one runs the exit test in its
+// new compartment (move_kernel.hpp:392-437, 586-648); (2) the division
+// requests are accepted in slot order over the whole run: the k-th request
+// of the step (k counted from the run's first slot) succeeds iff
+// k < min(cap - n, div_cap), the reference's division buffer
+// (particles_container.hpp:559-573) made deterministic; the mother is
+// halved (M::division) and the daughter written straight to slot n + k,
+// where the reference's merge_buffer would append her (:575-599).  k is the
+// requests before the group (the per-block counts of the pass, summed over
+// at most 64 entries per level) plus the requests before it in the group.
+// Every mover and every request is independent, so the processing order is
+// free.  (3) The last block to finish keeps the books of every run: counters,
+// births, the removal decision (update_and_remove_inactive,
+// particles_container.hpp:539-557) and the removal's tile layout.
 constexpr uint32_t RES_SC = RESOLVE_GROUP_MAX;  // shared arrays; a group is GS <= RES_SC sub-chunks
 static_assert(RES_SC <= 64 && 32 % NSUB_MAX == 0, "a group is at most one wave of whole tiles");
+
+// Requests of the pass in sub-chunks [0, sc), summed by one wave from the
+// block counts (div_l3, div_l2, div_l1: blocks of 2^18, 2^12 and 2^6
+// sub-chunks) and the sub-chunk counts of sc's own block of 64.
+__device__ __forceinline__ uint32_t request_prefix(const BatchDev& b, uint32_t sc, uint32_t lane) {
+  const uint32_t a3 = sc >> 18, a2 = sc >> 12, a1 = sc >> 6;
+  uint32_t v = 0;
+  for (uint32_t i = lane; i < a3; i += 64) v += b.div_l3[i];
+  const uint32_t i2 = (a3 << 6) + lane, i1 = (a2 << 6) + lane, i0 = (a1 << 6) + lane;
+  const uint32_t x2 = i2 < a2 ? b.div_l2[i2] : 0u;
+  const uint32_t x1 = i1 < a1 ? b.div_l1[i1] : 0u;
+  const uint32_t x0 = i0 < sc ? b.sub_divs[i0] : 0u;
+  return wave_sum(v + x2 + x1 + x0);
+}
 
 template <class M>
 __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
@@ -769,14 +884,12 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
                                                  const uint32_t* __restrict__ tile_off) {
   constexpr int NV = M::NV;
   constexpr int NP = M::NP > 0 ? M::NP : 1;
-  __shared__ uint32_t s_pre[RES_SC + 1];  // movers: prefix over the group's sub-chunks
+  __shared__ uint32_t s_pre[RES_SC + 1];   // movers: prefix over the group's sub-chunks
   __shared__ uint32_t s_run[RES_SC];
-  __shared__ uint32_t s_dcnt[RES_SC];     // division requests per sub-chunk
-  __shared__ uint32_t s_dpre[RES_SC];     // ... their prefix within the tile
-  __shared__ uint32_t s_births[RES_SC];   // accepted requests per tile of the group
-  __shared__ uint32_t s_bpre[RES_SC + 1]; // ... their prefix over the group's tiles
-  __shared__ uint32_t s_any;
-  const uint32_t tid = threadIdx.x;
+  __shared__ uint32_t s_dpre[RES_SC + 1];  // division requests: prefix over the group's sub-chunks
+  __shared__ uint32_t s_p0, s_pr0;         // requests before the group, before its first run
+  __shared__ uint32_t s_last;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t R = b.R;
   const uint32_t nsub = b.nsub;          // sub-chunks per tile (divides the group)
   const uint32_t sub = TILE / nsub;      // slots per sub-chunk
@@ -786,14 +899,12 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   // shorter chains), else 64 (fewer, fuller groups per block); lanes
   // GS..63 of the first wave stay idle
   const uint32_t GS = (n_sc + 31) / 32 <= gridDim.x ? 32u : RES_SC;
-  const uint32_t tpg = GS / nsub;        // tiles per group
   const uint32_t n_groups = (n_sc + GS - 1) / GS;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
   for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
     const uint32_t sc0 = grp * GS;
-    const uint32_t t0 = sc0 / nsub;
     if (tid < 64) {  // the group's sub-chunks, one lane each
-      const uint32_t lane = tid, sc = sc0 + lane;
+      const uint32_t sc = sc0 + lane;
       uint32_t cm = 0, cd = 0, r = 0;
       if (lane < GS && sc < n_sc) {
         cm = b.sub_moves[sc];
@@ -801,33 +912,16 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         r = find_run(tile_off, R, sc / nsub);
       }
       const uint32_t inc = wave_incl_scan(cm, lane);
+      const uint32_t incd = wave_incl_scan(cd, lane);
       if (lane < GS) {
         s_pre[lane] = inc - cm;
+        s_dpre[lane] = incd - cd;
         s_run[lane] = r;
-        s_dcnt[lane] = cd;
       }
-      if (lane == GS - 1) s_pre[GS] = inc;
-      // requests: prefix within each tile (segments of nsub lanes)
-      const uint32_t incd = wave_incl_scan(cd, lane);
-      const uint32_t first = lane - lane % nsub;
-      const uint32_t before = __shfl(incd, first ? first - 1 : 0, 64);
-      const uint32_t seg_base = first ? before : 0u;
-      if (lane < GS) s_dpre[lane] = incd - cd - seg_base;
-      if (lane < GS && lane % nsub == nsub - 1) {  // the tile's births: the first `budget` requests
-        const uint32_t t = t0 + lane / nsub;
-        const uint32_t tot = incd - seg_base;
-        uint32_t births = 0;
-        if (t < T) {
-          RunState* rs = b.rs + r;
-          const uint32_t budget = rs->budget;
-          births = tot < budget ? tot : budget;
-          b.tile_births[t] = births;
-          if (tot > births) atomicAdd(&rs->step_overflow, (unsigned long long)(tot - births));
-        }
-        s_births[lane / nsub] = births;
+      if (lane == GS - 1) {
+        s_pre[GS] = inc;
+        s_dpre[GS] = incd;
       }
-      const uint64_t any = __ballot(cd != 0);
-      if (lane == 0) s_any = any != 0 ? 1u : 0u;
     }
     __syncthreads();
     // ---- movers
@@ -845,7 +939,8 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const RunDev& rd = runs[r];
       RunState* rs = b.rs + r;
       const uint64_t q = (uint64_t)sc * sub + (e - s_pre[j]);
-      const uint32_t slot = (t - tile_off[r]) * TILE + b.move_slot[q];
+      const uint32_t ms = b.move_slot[q];
+      const uint32_t slot = (t - tile_off[r]) * TILE + (ms & (TILE - 1u));
       uint32_t pos = b.move_pos[q];
       const uint64_t g = rd.p_off + slot;
       const uint32_t step = rs->step;
@@ -855,7 +950,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
       pos = gp(mp.nbr)[(size_t)pos * m + col];
       b.pos[g] = pos;
-      if (rd.enable_leave && leave_stage) {
+      if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {
         const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + record_terms<M>() + 1];
         if (qx < 1.0 && u01(u.z) > qx) {
           const double age = b.age_hyd[g];
@@ -869,36 +964,43 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         }
       }
     }
-    // ---- divisions: every accepted request of the group's tiles at once
-    // (one thread each, found through the prefix of the tiles' births)
-    if (s_any) {
-      if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t k = 0; k < tpg; ++k) {
-          s_bpre[k] = acc;
-          acc += s_births[k];
-        }
-        s_bpre[tpg] = acc;
+    // ---- divisions: every request of the group at once, one thread each
+    const uint32_t n_req = s_dpre[GS];
+    if (n_req) {
+      // requests before the group and before the start of its first run
+      if (wave == 0) {
+        const uint32_t p0 = request_prefix(b, sc0, lane);
+        if (lane == 0) s_p0 = p0;
+      } else if (wave == 1) {
+        const uint32_t s_r = tile_off[s_run[0]] * nsub;
+        const uint32_t pr = request_prefix(b, s_r, lane);
+        if (lane == 0) s_pr0 = pr;
       }
       __syncthreads();
-      const uint32_t n_births = s_bpre[tpg];
-      for (uint32_t f = tid; f < n_births; f += blockDim.x) {
-        uint32_t k = 0;  // the tile of birth f: the last k with s_bpre[k] <= f
-        while (k + 1 < tpg && s_bpre[k + 1] <= f) ++k;
-        const uint32_t e = f - s_bpre[k];  // request e of tile t, in slot order
-        const uint32_t t = t0 + k;
-        const uint32_t r = s_run[k * nsub];
+      for (uint32_t f = tid; f < n_req; f += blockDim.x) {
+        uint32_t lo = 0, hi = GS;  // the sub-chunk j with s_dpre[j] <= f < s_dpre[j+1]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_dpre[mid] <= f) lo = mid; else hi = mid;
+        }
+        const uint32_t j = lo;
+        const uint32_t sc = sc0 + j;
+        const uint32_t t = sc / nsub;
+        const uint32_t r = s_run[j];
         const RunDev& rd = runs[r];
         RunState* rs = b.rs + r;
-        const uint32_t step = rs->step;
+        // rank of the request among the run's requests of the step, in slot order
+        const uint32_t first = tile_off[r] * nsub;  // the run's first sub-chunk
+        const uint32_t k = first <= sc0 ? s_p0 - s_pr0 + f : f - s_dpre[first - sc0];
+        const uint32_t n = rs->n;
+        const uint32_t room = rd.cap - n;
+        if (k >= (room < rd.div_cap ? room : rd.div_cap)) continue;  // refused: Overflow
+        const uint64_t q = (uint64_t)sc * sub + (f - s_dpre[j]);
+        const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
+        const uint64_t g = rd.p_off + slot;
         double p[NP];
 #pragma unroll
         for (int i = 0; i < NP; ++i) p[i] = rd.params[i];
-        uint32_t j = k * nsub;  // the tile's sub-chunk holding request e
-        while (j + 1 < (k + 1) * nsub && s_dpre[j + 1] <= e) ++j;
-        const uint64_t q = (uint64_t)(sc0 + j) * sub + (e - s_dpre[j]);
-        const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
-        const uint64_t g = rd.p_off + slot;
         double x[NV], xd[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
@@ -906,24 +1008,35 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
           const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
           if (i < rd.probe_cap) rd.probe_div[i] = b.age_div[g];
         }
-        M::division(p, x, xd, RngKey{rd.seed, slot, step});
-        const uint64_t sidx = (uint64_t)t * b.stage_stride + e;
+        M::division(p, x, xd, RngKey{rd.seed, slot, rs->step});
+        const uint64_t d = rd.p_off + n + k;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           b.state[(uint64_t)v * b.P + g] = x[v];
-          b.st_state[(uint64_t)v * b.ST + sidx] = xd[v];
+          b.state[(uint64_t)v * b.P + d] = xd[v];
         }
         b.age_div[g] = 0.0;
-        b.st_pos[sidx] = b.div_pos[q];
+        b.pos[d] = b.div_pos[q];
+        b.status[d] = IDLE;
+        b.age_hyd[d] = 0.0;
+        b.age_div[d] = 0.0;
       }
     }
     __syncthreads();
   }
+  // ---- the last block to finish keeps the books
+  if (tid == 0) {
+    __threadfence();
+    s_last = atomicAdd(b.ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  bookkeep_pass(b, runs, n_sc);
 }
 
 // ------------------------------------------------------------------ layout
-// Tile -> run table and per-tile division budget for the next pass:
-// budget = min(stage_cap, (cap - n) / tiles), so staged newborns always fit.
+// Tile -> run table of the next pass (the tiles of [0, n) of every active run).
 template <uint32_t NT>
 __device__ void layout_pass(const BatchDev& b) {
   constexpr uint32_t NWV = NT / 64;
@@ -941,12 +1054,6 @@ __device__ void layout_pass(const BatchDev& b) {
       const uint32_t active = rs->step < rd.step_target ? 1u : 0u;
       rs->active = active;
       tiles = active ? (rs->n + TILE - 1u) / TILE : 0u;
-      uint32_t budget = 0;
-      if (tiles) {
-        const uint32_t q = (rd.cap - rs->n) / tiles;
-        budget = q < rd.stage_cap ? q : rd.stage_cap;
-      }
-      rs->budget = budget;
       rs->n_tiles = tiles;
     }
     const uint32_t inc = wave_incl_scan(tiles, lane);

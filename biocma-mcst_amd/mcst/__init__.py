@@ -238,9 +238,9 @@ class Engine:
         return {"block": blk.value, "group": grp.value, "particle_ms": ms.value}
 
     def profile(self, n_steps):
-        """(total ms, per-kernel mean ms: transport+layout, particle, resolve, bookkeep, removal, merge)."""
+        """(total ms, per-kernel mean ms: transport+layout, particle, resolve+bookkeeping, removal)."""
         tot = C.c_double()
-        per = np.zeros(6)
+        per = np.zeros(4)
         _ck(lib().mcst_engine_profile(self.h, n_steps, C.byref(tot), per), "profile")
         return tot.value, per
 
@@ -281,7 +281,8 @@ class Run:
     def set_compaction(self, ratio, min_removal):
         _ck(lib().mcst_run_set_removal(self._h, self.id, ratio, min_removal), "set_removal")
 
-    def set_stage_cap(self, cap):
+    def set_division_cap(self, cap):
+        """At most `cap` newborns per step (the reference's division buffer)."""
         _ck(lib().mcst_run_set_division_cap(self._h, self.id, cap), "set_division_cap")
 
     def set_sort_interval(self, steps):

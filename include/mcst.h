@@ -163,7 +163,12 @@ int mcst_run_sort_particles(mcst_engine* e, int32_t run);
  * every = 0 (the default) keeps the capacity fixed.  An engine with a growing
  * run cannot capture a graph. */
 int mcst_run_set_growth(mcst_engine* e, int32_t run, uint32_t every, double factor);
-/* At most `cap` (1..1024) divisions per 1024-slot tile and step. */
+/* The division buffer (particles_container.hpp:559-573, whose size is
+ * buffer_ratio x the allocated slots): at most `cap` (>= 1) newborns per
+ * step.  Division requests are taken in slot order over the whole run and
+ * the first min(capacity - n, cap) succeed; the others are refused
+ * (Overflow) and ask again next step.  Default: no cap beyond the free
+ * slots. */
 int mcst_run_set_division_cap(mcst_engine* e, int32_t run, uint32_t cap);
 /* Keep up to `cap` exit ages and division ages (the reference's probes). */
 int mcst_run_set_probes(mcst_engine* e, int32_t run, uint32_t cap);
@@ -201,8 +206,9 @@ int mcst_engine_admit(mcst_engine* e, uint32_t* n_admitted);
 int mcst_engine_retire(mcst_engine* e, int32_t run);
 
 /* ---- stepping ---- */
-/* n steps of every active run: feed, transport, particle cycle, removal,
- * merge (the body of main_loop, apps/core/src/host_specific.cpp:280-291). */
+/* n steps of every active run: feed, transport, particle cycle, newborns
+ * appended, removal (the body of main_loop,
+ * apps/core/src/host_specific.cpp:280-291). */
 int mcst_engine_step(mcst_engine* e, uint32_t n_steps);
 /* One step restricted to the MCST_STAGE_* bits in `stages`. */
 int mcst_engine_step_stages(mcst_engine* e, uint32_t stages);
@@ -214,11 +220,10 @@ int mcst_engine_capture(mcst_engine* e, uint32_t n_steps);
 int mcst_engine_replay(mcst_engine* e, uint32_t times);
 int mcst_engine_synchronize(mcst_engine* e);
 /* Take n steps, as mcst_engine_step, timed with HIP events on the engine
- * stream: total ms and the mean
- * duration of each step kernel, in the order transport (with the tile
- * layout), particle pass, mover and division resolution, bookkeeping, removal (4 kernels
- * together, skipped when no particle can become inactive), merge
- * -> per_kernel_ms[6]. */
+ * stream: total ms and the mean duration of each step kernel, in the order
+ * transport (with the tile layout), particle pass, mover and division
+ * resolution (with the bookkeeping), removal (4 kernels together, skipped
+ * when no particle can become inactive) -> per_kernel_ms[4]. */
 int mcst_engine_profile(mcst_engine* e, uint32_t n_steps, double* total_ms, double* per_kernel_ms);
 /* Autotuner of the particle pass (no counterpart in the reference; the
  * launch shapes of its Kokkos kernels are fixed).  Runs `steps` real steps

@@ -811,7 +811,7 @@ struct orc_run {
   double weight;
   uint64_t inactive, min_removal;
   double ratio;
-  uint32_t stage_cap;
+  uint64_t div_cap;       /* newborns per step at most (the reference's buffer size) */
   uint32_t sort_interval; /* reorder by compartment every k steps (0: never) */
   uint32_t grow_every;    /* capacity check every k steps (0: fixed capacity) */
   double grow_factor;     /* new capacity = ceil(n * grow_factor) */
@@ -842,7 +842,7 @@ orc_run* orc_create(int model, uint32_t n_c, uint32_t n_s, double dt, uint64_t s
   r->q_exit = (double*)calloc(n_c, sizeof(double));
   r->ratio = 0.01;
   r->min_removal = 0;
-  r->stage_cap = 64;
+  r->div_cap = UINT64_MAX;
   return r;
 }
 
@@ -931,9 +931,9 @@ int orc_set_gas_transfer(orc_run* r, const double* kla, const double* cstar) {
   return 0;
 }
 
-int orc_set_stage_cap(orc_run* r, uint32_t cap) {
-  if (cap == 0 || cap > TILE) return -1;
-  r->stage_cap = cap;
+int orc_set_division_cap(orc_run* r, uint32_t cap) {
+  if (cap == 0) return -1;
+  r->div_cap = cap;
   return 0;
 }
 
@@ -1079,60 +1079,62 @@ double orc_feed_flow(int type, double f0, double param, double t) {
 static double feed_flow(const feed_t* f, double t) { return orc_feed_flow(f->type, f->f0, f->param, t); }
 
 typedef struct {
-  uint32_t births, exits, deaths, moves, overflow, processed;
-  uint32_t stage_n;
-  double* stage_state; /* [stage_cap][n_var] */
-  uint32_t* stage_pos;
-  dvec exit_ages, div_ages; /* probes of this tile (threaded path), appended in tile order */
+  uint32_t exits, deaths, moves, processed;
+  uint32_t n_req;         /* division requests of the tile, in slot order */
+  uint32_t* req_slot;
+  uint32_t* req_pos;      /* the mother's compartment when she asked (before her move) */
+  dvec exit_ages;         /* probes of this tile (threaded path), appended in tile order */
 } tile_out;
 
-static void particle_tile(orc_run* r, const flowmap* f, uint64_t step, uint32_t tile, uint32_t budget,
-                          uint32_t stages, int enable_move, int enable_leave, double* S, tile_out* o,
-                          dvec* exit_ages, dvec* div_ages) {
+/* One 1024-slot tile of the particle cycle, in the reference's stage order:
+ * the model pass over the Idle slots (model_kernel.hpp:163-268), their
+ * contributions (contribution_kernel.hpp:144-186), the move pass over EVERY
+ * slot below n whatever its status (move_kernel.hpp:209-273,392-437: Exit
+ * and Dead slots waiting for removal keep moving and count Move events),
+ * then the leave pass over the Idle slots (move_kernel.hpp:347-359,586-648).
+ * Division requests are only listed here: which are accepted depends on the
+ * whole run's requests (see accept_divisions). */
+static void particle_tile(orc_run* r, const flowmap* f, uint64_t step, uint32_t tile, uint32_t stages,
+                          int enable_move, int enable_leave, double* S, tile_out* o, dvec* exit_ages) {
   const uint64_t p0 = (uint64_t)tile * TILE;
   const uint64_t p1 = (p0 + TILE < r->n) ? p0 + TILE : r->n;
   const uint32_t n_c = r->n_c;
   const int nv = r->n_var, ns = r->n_s;
   const double dt = r->dt;
-  double x[MAX_VAR], xd[MAX_VAR], c[MAX_SPECIES], contrib[MAX_SPECIES];
+  double x[MAX_VAR], c[MAX_SPECIES], contrib[MAX_SPECIES];
   for (uint64_t p = p0; p < p1; ++p) {
-    if (r->status[p] != ORC_IDLE) continue;
-    o->processed++;
     uint32_t pos = r->pos[p];
-    /* model cycle: model_kernel.hpp:189-194, 230-268 */
-    for (int v = 0; v < nv; ++v) x[v] = r->state[(size_t)v * r->cap + p];
-    for (int s = 0; s < ns; ++s) contrib[s] = 0.0;
-    int st = ORC_IDLE;
-    if (stages & ORC_STAGE_MODEL) {
-      r->age_div[p] += dt;
-      for (int s = 0; s < ns; ++s) c[s] = r->C[(size_t)s * n_c + pos];
-      st = model_update(r->model, r->params, dt, x, c, contrib);
-    }
-    if (st == ORC_DEAD) {
+    int idle = r->status[p] == ORC_IDLE;
+    if (idle) {
+      o->processed++;
+      for (int v = 0; v < nv; ++v) x[v] = r->state[(size_t)v * r->cap + p];
+      for (int s = 0; s < ns; ++s) contrib[s] = 0.0;
+      int st = ORC_IDLE;
+      if (stages & ORC_STAGE_MODEL) {
+        r->age_div[p] += dt;
+        for (int s = 0; s < ns; ++s) c[s] = r->C[(size_t)s * n_c + pos];
+        st = model_update(r->model, r->params, dt, x, c, contrib);
+      }
       for (int v = 0; v < nv; ++v) r->state[(size_t)v * r->cap + p] = x[v];
-      r->status[p] = ORC_DEAD;
-      o->deaths++;
-      continue;
-    }
-    if (st == ORC_DIVISION) {
-      if (o->stage_n < budget) {
-        if (div_ages) dvec_push(div_ages, r->age_div[p]);
-        model_division(r->model, r->params, x, xd, r->seed, (uint32_t)p, (uint32_t)step);
-        for (int v = 0; v < nv; ++v) o->stage_state[(size_t)o->stage_n * nv + v] = xd[v];
-        o->stage_pos[o->stage_n] = pos;
-        o->stage_n++;
-        r->age_div[p] = 0.0;
+      if (st == ORC_DEAD) { /* a dead cell stops contributing and leaving */
+        r->status[p] = ORC_DEAD;
+        o->deaths++;
+        idle = 0;
       } else {
-        o->overflow++;
+        if (st == ORC_DIVISION) {
+          o->req_slot[o->n_req] = (uint32_t)p;
+          o->req_pos[o->n_req] = pos;
+          o->n_req++;
+        }
+        /* contribution: S(j,pos) += w * contrib(j) */
+        if (stages & ORC_STAGE_CONTRIB)
+          for (int s = 0; s < ns; ++s) S[(size_t)s * n_c + pos] += r->weight * contrib[s];
       }
     }
-    for (int v = 0; v < nv; ++v) r->state[(size_t)v * r->cap + p] = x[v];
-    /* contribution: S(j,pos) += w * contrib(j)   (contribution_kernel.hpp:144-186) */
-    if (stages & ORC_STAGE_CONTRIB)
-      for (int s = 0; s < ns; ++s) S[(size_t)s * n_c + pos] += r->weight * contrib[s];
     uint32_t w[4];
     draw4(r->seed, (uint32_t)p, (uint32_t)step, TAG_TRANSPORT, 0u, w);
-    /* move: move_kernel.hpp:392-437 with the linear test of probability_leaving.hpp:33-46 */
+    /* move, every slot: the linear test of probability_leaving.hpp:33-46,
+     * then the first CDF column >= u (move_kernel.hpp:61-103) */
     if (enable_move && (stages & ORC_STAGE_MOVE)) {
       const double u1 = orc_u01(w[0]);
       if (f->p_move[pos] > u1) {
@@ -1151,8 +1153,8 @@ static void particle_tile(orc_run* r, const flowmap* f, uint64_t step, uint32_t 
         o->moves++;
       }
     }
-    /* leave: move_kernel.hpp:347-359, 586-648 */
-    if (enable_leave && (stages & ORC_STAGE_LEAVE)) {
+    /* leave, Idle slots: move_kernel.hpp:347-359, 586-648 */
+    if (idle && enable_leave && (stages & ORC_STAGE_LEAVE)) {
       r->age_hyd[p] += dt;
       const double q = r->q_exit[pos];
       if (q < 1.0 && orc_u01(w[2]) > q) {
@@ -1163,6 +1165,43 @@ static void particle_tile(orc_run* r, const flowmap* f, uint64_t step, uint32_t 
       }
     }
   }
+}
+
+/* The division buffer (particles_container.hpp:559-573, merge_buffer
+ * :575-599), made deterministic: the step's requests are taken in slot
+ * order and the first min(cap - n, div_cap) are accepted (the reference
+ * refuses a division only when its buffer is full); the k-th accepted
+ * mother divides (Model::division) and her daughter takes slot n + k, at
+ * the mother's compartment when she asked, with both ages 0.  The others
+ * are refused (Overflow) and stay as they are, to ask again next step.
+ * Returns the births. */
+static uint64_t accept_divisions(orc_run* r, uint64_t step, const tile_out* outs, uint32_t n_tiles) {
+  const uint64_t n = r->n;
+  uint64_t budget = r->cap - n;
+  if (budget > r->div_cap) budget = r->div_cap;
+  const int nv = r->n_var;
+  double x[MAX_VAR], xd[MAX_VAR];
+  uint64_t k = 0, requests = 0;
+  for (uint32_t tl = 0; tl < n_tiles; ++tl) {
+    requests += outs[tl].n_req;
+    for (uint32_t q = 0; q < outs[tl].n_req && k < budget; ++q, ++k) {
+      const uint64_t p = outs[tl].req_slot[q], d = n + k;
+      dvec_push(&r->div_ages, r->age_div[p]);
+      for (int v = 0; v < nv; ++v) x[v] = r->state[(size_t)v * r->cap + p];
+      model_division(r->model, r->params, x, xd, r->seed, (uint32_t)p, (uint32_t)step);
+      for (int v = 0; v < nv; ++v) {
+        r->state[(size_t)v * r->cap + p] = x[v];
+        r->state[(size_t)v * r->cap + d] = xd[v];
+      }
+      r->age_div[p] = 0.0;
+      r->pos[d] = outs[tl].req_pos[q];
+      r->status[d] = ORC_IDLE;
+      r->age_hyd[d] = 0.0;
+      r->age_div[d] = 0.0;
+    }
+  }
+  r->counters[ORC_C_OVERFLOW] += requests - k;
+  return k;
 }
 
 static void copy_particle(orc_run* r, uint64_t dst, uint64_t src) {
@@ -1204,7 +1243,7 @@ int orc_step(orc_run* r, uint32_t n_steps, int n_threads) {
 int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages) {
   if (r->n_maps == 0) return -1;
   const uint32_t n_c = r->n_c;
-  const int ns = r->n_s, nv = r->n_var;
+  const int ns = r->n_s;
   const size_t nsc = (size_t)ns * n_c;
   int n_leaving = 0;
   for (uint32_t i = 0; i < r->n_feeds; ++i) n_leaving += r->feeds[i].out >= 0;
@@ -1258,21 +1297,14 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
     /* particle cycle */
     const uint64_t N = r->n;
     const uint32_t n_tiles = (uint32_t)((N + TILE - 1) / TILE);
-    uint32_t budget = 0;
-    if (n_tiles) {
-      const uint64_t free_slots = r->cap - N;
-      const uint64_t b = free_slots / n_tiles;
-      budget = b < r->stage_cap ? (uint32_t)b : r->stage_cap;
-    }
     tile_out* outs = (tile_out*)calloc(n_tiles ? n_tiles : 1, sizeof(tile_out));
     for (uint32_t tl = 0; tl < n_tiles; ++tl) {
-      outs[tl].stage_state = (double*)malloc(sizeof(double) * (budget ? budget : 1) * nv);
-      outs[tl].stage_pos = (uint32_t*)malloc(sizeof(uint32_t) * (budget ? budget : 1));
+      outs[tl].req_slot = (uint32_t*)malloc(sizeof(uint32_t) * TILE);
+      outs[tl].req_pos = (uint32_t*)malloc(sizeof(uint32_t) * TILE);
     }
     if (n_threads == 1) {
       for (uint32_t tl = 0; tl < n_tiles; ++tl)
-        particle_tile(r, f, step, tl, budget, stages, enable_move, enable_leave, r->S, &outs[tl], &r->exit_ages,
-                      &r->div_ages);
+        particle_tile(r, f, step, tl, stages, enable_move, enable_leave, r->S, &outs[tl], &r->exit_ages);
     } else {
       double* Sp = (double*)calloc(nsc * (size_t)n_threads, sizeof(double));
 #pragma omp parallel num_threads(n_threads)
@@ -1284,32 +1316,36 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
         double* S_loc = Sp + nsc * (size_t)tid;
 #pragma omp for schedule(static)
         for (int64_t tl = 0; tl < (int64_t)n_tiles; ++tl)
-          particle_tile(r, f, step, (uint32_t)tl, budget, stages, enable_move, enable_leave, S_loc, &outs[tl],
-                        &outs[tl].exit_ages, &outs[tl].div_ages);
+          particle_tile(r, f, step, (uint32_t)tl, stages, enable_move, enable_leave, S_loc, &outs[tl],
+                        &outs[tl].exit_ages);
       }
       for (int th = 0; th < n_threads; ++th)
         for (size_t o = 0; o < nsc; ++o) r->S[o] += Sp[nsc * (size_t)th + o];
       free(Sp);
       for (uint32_t tl = 0; tl < n_tiles; ++tl) {
         for (uint64_t k = 0; k < outs[tl].exit_ages.n; ++k) dvec_push(&r->exit_ages, outs[tl].exit_ages.v[k]);
-        for (uint64_t k = 0; k < outs[tl].div_ages.n; ++k) dvec_push(&r->div_ages, outs[tl].div_ages.v[k]);
         free(outs[tl].exit_ages.v);
-        free(outs[tl].div_ages.v);
       }
     }
-    uint64_t births = 0, exits = 0, deaths = 0;
+    uint64_t exits = 0, deaths = 0;
     for (uint32_t tl = 0; tl < n_tiles; ++tl) {
-      births += outs[tl].stage_n;
       exits += outs[tl].exits;
       deaths += outs[tl].deaths;
       r->counters[ORC_C_MOVES] += outs[tl].moves;
-      r->counters[ORC_C_OVERFLOW] += outs[tl].overflow;
       r->counters[ORC_C_PROCESSED] += outs[tl].processed;
     }
+    const uint64_t births = accept_divisions(r, step, outs, n_tiles);
+    for (uint32_t tl = 0; tl < n_tiles; ++tl) {
+      free(outs[tl].req_slot);
+      free(outs[tl].req_pos);
+    }
+    free(outs);
     r->counters[ORC_C_BIRTHS] += births;
     r->counters[ORC_C_EXITS] += exits;
     r->counters[ORC_C_DEATHS] += deaths;
-    /* post_cycle: update_and_remove_inactive then merge_buffer */
+    /* post_cycle: the newborns join the container, then
+     * update_and_remove_inactive with the threshold of the pass's count */
+    r->n = N + births;
     r->inactive += exits + deaths;
     const uint64_t thr_ratio = (uint64_t)((double)N * r->ratio);
     const uint64_t thr = r->min_removal > thr_ratio ? r->min_removal : thr_ratio;
@@ -1317,20 +1353,6 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
       remove_inactive(r, r->inactive);
       r->counters[ORC_C_COMPACTIONS]++;
     }
-    uint64_t dst = r->n;
-    for (uint32_t tl = 0; tl < n_tiles; ++tl) {
-      for (uint32_t k = 0; k < outs[tl].stage_n; ++k, ++dst) {
-        r->pos[dst] = outs[tl].stage_pos[k];
-        r->status[dst] = ORC_IDLE;
-        r->age_hyd[dst] = 0.0;
-        r->age_div[dst] = 0.0;
-        for (int v = 0; v < nv; ++v) r->state[(size_t)v * r->cap + dst] = outs[tl].stage_state[(size_t)k * nv + v];
-      }
-      free(outs[tl].stage_state);
-      free(outs[tl].stage_pos);
-    }
-    free(outs);
-    r->n = dst;
     r->counters[ORC_C_STEPS]++;
     if (r->sort_interval && r->counters[ORC_C_STEPS] % r->sort_interval == 0 && sort_by_compartment(r)) return -2;
   }

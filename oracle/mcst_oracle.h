@@ -101,7 +101,7 @@ int orc_add_feed(orc_run* r, int type, double f0, double param, uint32_t input, 
                  uint32_t n_values, const uint32_t* species, const double* conc);
 int orc_set_concentrations(orc_run* r, const double* C);
 int orc_set_compaction(orc_run* r, double ratio, uint64_t min_removal);
-int orc_set_stage_cap(orc_run* r, uint32_t cap);
+int orc_set_division_cap(orc_run* r, uint32_t cap);
 int orc_set_gas_transfer(orc_run* r, const double* kla, const double* cstar);
 int orc_set_sort_interval(orc_run* r, uint32_t steps);
 int orc_sort_particles(orc_run* r);

@@ -78,7 +78,7 @@ def lib():
                                    C.c_uint32, _u32p, _f64p]
         L.orc_set_concentrations.argtypes = [C.c_void_p, _f64p]
         L.orc_set_compaction.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
-        L.orc_set_stage_cap.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_set_division_cap.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_init_particles.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double]
         L.orc_set_particles.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, _u32p, _u8p, _f64p, _f64p, _f64p,
                                         C.c_double]
@@ -208,8 +208,8 @@ class OracleRun:
     def set_compaction(self, ratio, min_removal):
         _check(lib().orc_set_compaction(self.h, ratio, min_removal), "set_compaction")
 
-    def set_stage_cap(self, cap):
-        _check(lib().orc_set_stage_cap(self.h, cap), "set_stage_cap")
+    def set_division_cap(self, cap):
+        _check(lib().orc_set_division_cap(self.h, cap), "set_division_cap")
 
     def set_sort_interval(self, steps):
         _check(lib().orc_set_sort_interval(self.h, steps), "set_sort_interval")

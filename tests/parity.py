@@ -5,7 +5,7 @@ import numpy as np
 RTOL = 1e-12
 
 
-def make_pair(case, stage_cap=64, ratio=0.01, min_removal=0, probes=1 << 20, capacity=None):
+def make_pair(case, division_cap=None, ratio=0.01, min_removal=0, probes=1 << 20, capacity=None):
     import mcst
     from oracle.oracle import OracleRun
 
@@ -16,7 +16,8 @@ def make_pair(case, stage_cap=64, ratio=0.01, min_removal=0, probes=1 << 20, cap
         case.capacity = capacity
     for r in (g, o):
         case.apply(r)
-        r.set_stage_cap(stage_cap)
+        if division_cap:
+            r.set_division_cap(division_cap)
         r.set_compaction(ratio, min_removal)
     g.set_probes(probes)
     eng.commit()

@@ -9,10 +9,9 @@ from parity import RTOL, assert_parity, close, make_pair
 pytestmark = pytest.mark.gpu
 
 
-def _define(eng, case, stage_cap=64, probes=1 << 20):
+def _define(eng, case, probes=1 << 20):
     r = eng.add_run(case.n_c, case.dt, case.seed)
     case.apply(r, particles=False)
-    r.set_stage_cap(stage_cap)
     r.set_compaction(0.01, 0)
     r.set_probes(probes)
     return r

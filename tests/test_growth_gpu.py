@@ -37,7 +37,7 @@ def test_all_dividing_population_doubles_every_step():
 
     c = cases.small_case("monod_mass", n_c=1, n_particles=1000, seed=2, feeds=False)
     c.params = np.array([0.5, 0.01, 0.5, 0.0, 1e-15, 0.1])
-    eng, g, o = make_pair(c, stage_cap=1024, capacity=1000)
+    eng, g, o = make_pair(c, capacity=1000)
     for r in (g, o):
         r.set_growth(1, 2.5)
     for k in range(1, 8):

@@ -88,27 +88,34 @@ def test_quad_monod_uptake_lag_and_aerobic_growth(oracle_lib):
     np.testing.assert_allclose(S[1] / S[3], -QUAD[7] / QUAD[11], rtol=1e-13)
 
 
-def test_division_conserves_mass_and_respects_budget(oracle_lib):
+@pytest.mark.parametrize("free,cap", [(50, None), (5000, 70), (5000, None)])
+def test_division_conserves_mass_and_respects_budget(oracle_lib, free, cap):
+    """Every cell of three tiles asks to divide: the first min(free slots,
+    per-step cap) requests of the run in slot order are accepted (the
+    reference refuses only when its division buffer is full,
+    particles_container.hpp:559-573), the k-th daughter takes slot n + k, and
+    mass is conserved."""
     r = one_box(oracle_lib)
     r.set_params([0.5, 0.01, 0.5, 2.0, 1e-15, 0.1])
     r.set_concentrations(np.full((1, 1), 1.0))
     n = 3000  # three tiles
     m = np.full(n, 2.5e-15)
-    r.set_particles(n + 50, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([m, np.full(n, 1e-15)]),
+    r.set_particles(n + free, np.zeros(n, np.uint32), np.zeros(n, np.uint8), np.stack([m, np.full(n, 1e-15)]),
                     np.zeros(n), np.full(n, 3.0), 1.0)
-    r.set_stage_cap(64)
+    if cap:
+        r.set_division_cap(cap)
     r.step_stages(MODEL)
     c = r.counters()
-    # budget per tile = min(64, (cap - n) / tiles) = min(64, 50 // 3) = 16
-    assert c["births"] == 3 * 16 and c["overflow"] == n - 48 and r.n == n + 48
+    born = min(n, free, cap or n)
+    assert c["births"] == born and c["overflow"] == n - born and r.n == n + born
     p = r.particles()
     grown = 2.5e-15 + 0.01 * (0.5 * 1.0 / 1.01) * 2.5e-15
     np.testing.assert_allclose(p["state"][0].sum(), n * grown, rtol=1e-13)
     div_slots = np.nonzero(p["state"][0][:n] < 2e-15)[0]
-    # the first 16 requests of each tile, in slot order
-    assert list(div_slots) == [t * 1024 + k for t in range(3) for k in range(16)]
+    assert list(div_slots) == list(range(born))  # the first requests in slot order
     assert np.all(p["age_div"][div_slots] == 0.0) and np.all(p["age_div"][n:] == 0.0)
     np.testing.assert_array_equal(p["state"][0][n:], p["state"][0][div_slots])
+    assert np.all(p["age_div"][born:n] == 3.01)  # refused: the division age runs on
 
 
 def test_closed_system_mass_balance(oracle_lib):
@@ -391,8 +398,7 @@ def test_udf_starvation_death_times_and_contributions(oracle_lib):
 
 
 def test_container_growth_doubles_an_all_dividing_population(oracle_lib):
-    """Every cell divides every step (division ratio 0, a 1024-division cap
-    per tile): with a capacity check before every step (grow to 2.5 n when
+    """Every cell divides every step (division ratio 0): with a capacity check before every step (grow to 2.5 n when
     fewer slots than cells are free) nothing is refused and n doubles each
     step; with the capacity fixed at the initial count nothing is born."""
     for every in (1, 0):
@@ -400,7 +406,6 @@ def test_container_growth_doubles_an_all_dividing_population(oracle_lib):
         r.set_params([0.5, 0.01, 0.5, 0.0, 1e-15, 0.1])
         r.set_concentrations(np.full((1, 1), 1.0))
         r.init_particles(1000, 1000, 1.0)
-        r.set_stage_cap(1024)
         if every:
             r.set_growth(every, 2.5)
         for k in range(1, 7):

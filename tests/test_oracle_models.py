@@ -25,7 +25,7 @@ def lin_density(d):
     return 1000.0 * math.pi * d * d / 4.0
 
 
-def box(oracle_lib, model, C, state, w=1.0, dt=0.01, params=None, cap=None, stage_cap=None):
+def box(oracle_lib, model, C, state, w=1.0, dt=0.01, params=None, cap=None):
     """One compartment, concentrations held fixed (no transport stage)."""
     nv, ns, _ = oracle_lib.model_info(model)
     r = oracle_lib.OracleRun(model, 1, ns, dt, 5)
@@ -37,8 +37,6 @@ def box(oracle_lib, model, C, state, w=1.0, dt=0.01, params=None, cap=None, stag
     n = state.shape[1]
     r.set_particles(cap or 2 * n, np.zeros(n, np.uint32), np.zeros(n, np.uint8), state, np.zeros(n),
                     np.zeros(n), w)
-    if stage_cap:
-        r.set_stage_cap(stage_cap)
     return r
 
 
@@ -250,7 +248,7 @@ def test_simple_acetate_division_draws(oracle_lib):
     n, dt = 6000, 0.01
     a_p = A_MAX / 4
     x0 = sa_state(n, a_p, length=1.95e-6, l_max=1.9e-6)
-    r = box(oracle_lib, "simple_acetate", [K_S, 0.0], x0, dt=dt, cap=3 * n, stage_cap=1024)
+    r = box(oracle_lib, "simple_acetate", [K_S, 0.0], x0, dt=dt)
     x, _ = step(r)
     assert r.n == 2 * n and r.counters()["overflow"] == 0
     mother, daughter = x[:, :n], x[:, n:]

@@ -73,14 +73,20 @@ def test_closed_network_without_leave():
 
 
 def test_division_overflow_and_tight_capacity():
+    """Requests beyond the per-step cap, then beyond the free slots, are
+    refused in slot order over the whole run (Overflow) and retried."""
     from mcst import cases
 
     c = cases.small_case("simple_acetate", n_particles=3000)
-    eng, g, o = make_pair(c, stage_cap=8, capacity=3100)
+    eng, g, o = make_pair(c, division_cap=8, capacity=3100)
     eng.step(10)
     o.step(10)
     assert_parity(g, o)
     assert g.counters()["overflow"] > 0
+    eng.step(25)
+    o.step(25)
+    assert_parity(g, o)
+    assert g.n == 3100
 
 
 def test_everything_leaves():
