@@ -435,6 +435,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "served": served,
+        # events of the timed steps: every division request is accepted while
+        # the run has free slots (overflow counts refusals)
+        "events": {k: int(c1[k] - c0[k]) for k in ("births", "overflow", "moves", "exits")},
     }
     if ensemble is not None:
         out["ensemble"] = ensemble

@@ -930,7 +930,7 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&e->d_rs, (size_t)RC * sizeof(RunState)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_comp, (size_t)(RC + 1) * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.rm_tile_off, (RC + 1) * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.ticket, sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.ticket, (1 + RESOLVE_TICKETS) * sizeof(uint32_t)))) return rc;
   const RequestLevels lv = request_levels(T_max, b.nsub);
   if ((rc = dev_alloc(e, (void**)&b.div_l1, lv.n1 * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_l2, lv.n2 * sizeof(uint32_t)))) return rc;
@@ -938,7 +938,7 @@ int commit(mcst_engine* e) {
   HIP_TRY(hipMemset(b.div_l1, 0, lv.n1 * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.div_l2, 0, lv.n2 * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.div_l3, 0, lv.n3 * sizeof(uint32_t)));
-  HIP_TRY(hipMemset(b.ticket, 0, sizeof(uint32_t)));
+  HIP_TRY(hipMemset(b.ticket, 0, (1 + RESOLVE_TICKETS) * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.rm_tile_off, 0, (RC + 1) * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.tile_off, 0, (RC + 1) * sizeof(uint32_t)));
   HIP_TRY(hipMemset(b.sub_moves, 0, T_max * b.nsub * sizeof(uint32_t)));

@@ -20,6 +20,7 @@ constexpr bool LANE_INTERLEAVE = true;     // particle pass: a lane's slots lie 
 constexpr uint32_t NSUB_MAX = TILE / 128;  // sub-chunks (the pass's unit of work) per tile: TILE / (64 * M::NL)
 constexpr uint32_t WAVES = BLOCK / 64;
 constexpr uint32_t RESOLVE_GROUP_MAX = 64;  // sub-chunks per group of the resolve kernel: 32 or 64
+constexpr uint32_t RESOLVE_TICKETS = 64;    // sub-counters of the resolve kernel's done-count
 constexpr int MAX_PARAMS = 16;
 constexpr int MAX_VAR = 16;
 constexpr int MAX_SPECIES = 8;
@@ -154,7 +155,7 @@ struct BatchDev {
   uint32_t* div_l1;
   uint32_t* div_l2;
   uint32_t* div_l3;
-  uint32_t* ticket;         // blocks of the resolve kernel that are done (its last block keeps the books)
+  uint32_t* ticket;         // [1 + RESOLVE_TICKETS] resolve blocks that are done (the last keeps the books)
   uint32_t* rm_tile_off;    // [R+1] removal layout: first global tile of each run's [0, n_end)
   // per-tile bookkeeping of the removal
   uint32_t* tile_gap;

@@ -780,7 +780,13 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
       if (rs->active) {
         const RunDev& rd = runs[r];
         const uint32_t n = rs->n;
-        const unsigned long long req = rs->step_requests, exits = rs->step_exits, deaths = rs->step_deaths;
+        // read and reset with atomics: the resolve kernel's blocks added to
+        // the exit counter with atomics up to a moment ago
+        const unsigned long long req = atomicExch(&rs->step_requests, 0ull);
+        const unsigned long long exits = atomicExch(&rs->step_exits, 0ull);
+        const unsigned long long deaths = atomicExch(&rs->step_deaths, 0ull);
+        const unsigned long long moves = atomicExch(&rs->step_moves, 0ull);
+        const unsigned long long processed = atomicExch(&rs->step_processed, 0ull);
         const uint32_t room = rd.cap - n;
         const uint32_t budget = room < rd.div_cap ? room : rd.div_cap;
         const uint32_t births = req < budget ? (uint32_t)req : budget;
@@ -788,10 +794,9 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
         cnt[C_BIRTHS] += births;
         cnt[C_EXITS] += exits;
         cnt[C_DEATHS] += deaths;
-        cnt[C_MOVES] += rs->step_moves;
+        cnt[C_MOVES] += moves;
         cnt[C_OVERFLOW] += req - births;
-        cnt[C_PROCESSED] += rs->step_processed;
-        rs->step_exits = rs->step_deaths = rs->step_moves = rs->step_requests = rs->step_processed = 0;
+        cnt[C_PROCESSED] += processed;
         uint64_t inactive = (uint64_t)rs->inactive + exits + deaths;
         const uint64_t thr_ratio = (uint64_t)((double)n * rd.ratio);
         const uint64_t thr = rd.min_removal > thr_ratio ? rd.min_removal : thr_ratio;
@@ -842,7 +847,7 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
   for (uint32_t i = tid; i < (n_sc + 63u) >> 6; i += NT) b.div_l1[i] = 0u;
   for (uint32_t i = tid; i < (n_sc + 4095u) >> 12; i += NT) b.div_l2[i] = 0u;
   for (uint32_t i = tid; i <= (n_sc >> 18); i += NT) b.div_l3[i] = 0u;
-  if (tid == 0) *b.ticket = 0u;
+  for (uint32_t i = tid; i <= RESOLVE_TICKETS; i += NT) b.ticket[i] = 0u;
 }
 
 // After the particle pass, per group of 32 or 64 sub-chunks (one per lane of
@@ -1024,14 +1029,33 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
     }
     __syncthreads();
   }
-  // ---- the last block to finish keeps the books
+  // ---- the last block to finish keeps the books.  The books read, of this
+  // kernel's work, only the exit counters, which are updated by atomics:
+  // every thread waits for its memory operations to complete before its
+  // block counts itself done, and the last block reads the counters with
+  // atomics.  No cache writeback is needed (an agent-scope fence per block
+  // writes back this XCD's L2 each time: 60 us over a 2048-block grid).
+  // Blocks count themselves done on RESOLVE_TICKETS sub-counters (block b on
+  // b % RESOLVE_TICKETS), and the last of each on the main counter, so no counter
+  // takes more than a few dozen atomics; blocks without a group take no part.
+  const uint32_t nb = n_groups < gridDim.x ? n_groups : gridDim.x;  // blocks with work
+  if (nb == 0) {
+    if (blockIdx.x == 0) bookkeep_pass(b, runs, n_sc);
+    return;
+  }
+  if (blockIdx.x >= nb) return;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
   if (tid == 0) {
-    __threadfence();
-    s_last = atomicAdd(b.ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+    const uint32_t j = blockIdx.x % RESOLVE_TICKETS;
+    const uint32_t in_j = (nb - j + RESOLVE_TICKETS - 1u) / RESOLVE_TICKETS;  // blocks counting on sub-counter j
+    const uint32_t n_sub = nb < RESOLVE_TICKETS ? nb : RESOLVE_TICKETS;        // sub-counters in use
+    uint32_t last = 0;
+    if (atomicAdd(b.ticket + 1u + j, 1u) == in_j - 1u) last = atomicAdd(b.ticket, 1u) == n_sub - 1u ? 1u : 0u;
+    s_last = last;
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
   bookkeep_pass(b, runs, n_sc);
 }
 
