@@ -297,6 +297,74 @@ int build_map(uint32_t n, const double* V, uint64_t nnz, const uint32_t* rows, c
   return MCST_OK;
 }
 
+// Transition data of one flowmap from the tables the reference's
+// hydrodynamic state hands over (ReactorDomain::update, domain.cpp:44-74;
+// ScalarSimulation::set_transition, implScalar.cpp:316-320): neighbour and
+// cumulative-probability rows [n][n_cols] used as given (rcmtool's
+// neighbour order and padding), out-flows (p_move = dt*Q/V), and the
+// transition matrix as COO triplets, taken by destination column, ascending
+// source, duplicates summed in input order (as sparse_from_coo sums them).
+int build_map_tables(uint32_t n, const double* V, uint32_t n_cols, const uint64_t* nbr, const double* cdf,
+                     const double* out_flow, uint64_t nnz, const uint64_t* rows, const uint64_t* cols,
+                     const double* vals, double dt, HostMap& f) {
+  if (n_cols == 0) return fail(MCST_ERR_ARG, "neighbour rows need at least one column");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!(V[i] > 0.0) || !std::isfinite(V[i])) return fail(MCST_ERR_ARG, "compartment volume must be positive");
+    if (!(out_flow[i] >= 0.0) || !std::isfinite(out_flow[i]))
+      return fail(MCST_ERR_ARG, "out-flows must be finite and non-negative");
+    for (uint32_t k = 0; k < n_cols; ++k) {
+      const size_t o = (size_t)i * n_cols + k;
+      if (nbr[o] >= n) return fail(MCST_ERR_ARG, "neighbour id out of range");
+      if (!std::isfinite(cdf[o])) return fail(MCST_ERR_ARG, "cumulative probability is not finite");
+      if (k && cdf[o] < cdf[o - 1]) return fail(MCST_ERR_ARG, "cumulative probability rows must not decrease");
+    }
+  }
+  std::vector<Edge> e;
+  e.reserve(nnz);
+  for (uint64_t t = 0; t < nnz; ++t) {
+    if (rows[t] >= n || cols[t] >= n) return fail(MCST_ERR_ARG, "transition entry out of range");
+    if (!std::isfinite(vals[t])) return fail(MCST_ERR_ARG, "transition entry is not finite");
+    e.push_back(Edge{(uint32_t)rows[t], (uint32_t)cols[t], t, vals[t]});
+  }
+  if (e.size() >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "flowmap too large");
+  std::sort(e.begin(), e.end(), [](const Edge& a, const Edge& b) {
+    if (a.j != b.j) return a.j < b.j;
+    if (a.i != b.i) return a.i < b.i;
+    return a.ord < b.ord;
+  });
+  f.m = n_cols;
+  f.V.assign(V, V + n);
+  f.inv_v.resize(n);
+  f.out_flow.assign(out_flow, out_flow + n);
+  f.p_move.resize(n);
+  f.q_exit.assign(n, 2.0);
+  for (uint32_t i = 0; i < n; ++i) {
+    f.inv_v[i] = 1.0 / V[i];
+    f.p_move[i] = (dt * out_flow[i]) / V[i];
+  }
+  f.cdf.assign(cdf, cdf + (size_t)n * n_cols);
+  f.nbr.resize((size_t)n * n_cols);
+  for (size_t o = 0; o < f.nbr.size(); ++o) f.nbr[o] = (uint32_t)nbr[o];
+  f.col_ptr.assign(n + 1, 0);
+  f.src.clear();
+  f.val.clear();
+  for (size_t t = 0; t < e.size();) {
+    Edge acc = e[t];
+    size_t k = t + 1;
+    for (; k < e.size() && e[k].i == acc.i && e[k].j == acc.j; ++k) acc.q += e[k].q;
+    f.col_ptr[acc.j + 1]++;
+    f.src.push_back(acc.i);
+    f.val.push_back(acc.q);
+    t = k;
+  }
+  for (uint32_t j = 0; j < n; ++j) f.col_ptr[j + 1] += f.col_ptr[j];
+  if (f.src.empty()) {  // keep every device pointer valid
+    f.src.push_back(0);
+    f.val.push_back(0.0);
+  }
+  return MCST_OK;
+}
+
 // ------------------------------------------------------------ runs
 struct HostFeed {
   int kind;
@@ -1411,6 +1479,20 @@ int mcst_run_add_flowmap(mcst_engine* e, int32_t run, const double* volumes, uin
   if (!volumes || (nnz && (!rows || !cols || !flows))) return fail(MCST_ERR_ARG, "null flowmap array");
   HostMap mp;
   const int rc = build_map(h->n_c, volumes, nnz, rows, cols, flows, h->dt, mp);
+  if (rc) return rc;
+  h->maps.push_back(std::move(mp));
+  return MCST_OK;
+}
+
+int mcst_run_add_flowmap_tables(mcst_engine* e, int32_t run, const double* volumes, uint32_t n_cols,
+                                const uint64_t* neighbors, const double* cumulative, const double* out_flows,
+                                uint64_t nnz, const uint64_t* rows, const uint64_t* cols, const double* values) {
+  CHECK_DEFINE(e, run, h);
+  if (!volumes || !neighbors || !cumulative || !out_flows || (nnz && (!rows || !cols || !values)))
+    return fail(MCST_ERR_ARG, "null flowmap array");
+  HostMap mp;
+  const int rc = build_map_tables(h->n_c, volumes, n_cols, neighbors, cumulative, out_flows, nnz, rows, cols, values,
+                                  h->dt, mp);
   if (rc) return rc;
   h->maps.push_back(std::move(mp));
   return MCST_OK;

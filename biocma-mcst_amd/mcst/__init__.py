@@ -35,7 +35,7 @@ EXPORTS = [
     "mcst_engine_reserve", "mcst_engine_admit", "mcst_engine_retire",
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
     "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
-    "mcst_engine_get_steps", "mcst_model_load", "mcst_run_set_growth",
+    "mcst_engine_get_steps", "mcst_model_load", "mcst_run_set_growth", "mcst_run_add_flowmap_tables",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -71,6 +71,7 @@ def lib():
             "mcst_run_create": [_vp, u32, u32, dbl, u64, C.POINTER(i32)],
             "mcst_run_set_model_params": [_vp, i32, _f64p, C.c_int],
             "mcst_run_add_flowmap": [_vp, i32, _f64p, u64, _u32p, _u32p, _f64p],
+            "mcst_run_add_flowmap_tables": [_vp, i32, _f64p, u32, _u64p, _f64p, _f64p, u64, _u64p, _u64p, _f64p],
             "mcst_run_set_steps_per_flowmap": [_vp, i32, u32],
             "mcst_run_add_feed": [_vp, i32, C.c_int, dbl, dbl, u32, i64, u32, _u32p, _f64p],
             "mcst_run_set_concentrations": [_vp, i32, _f64p],
@@ -267,6 +268,18 @@ class Run:
 
     def set_steps_per_map(self, k):
         _ck(lib().mcst_run_set_steps_per_flowmap(self._h, self.id, k), "steps_per_map")
+
+    def add_flowmap_tables(self, V, nbr, cdf, out_flow, rows, cols, vals):
+        """A flowmap from neighbour/cumulative-probability rows [n][m] as given,
+        out-flows [n] and the transition matrix as COO triplets
+        (mcst_run_add_flowmap_tables)."""
+        nbr = np.ascontiguousarray(nbr, np.uint64)
+        _ck(lib().mcst_run_add_flowmap_tables(self._h, self.id, np.ascontiguousarray(V, np.float64), nbr.shape[1],
+                                              nbr, np.ascontiguousarray(cdf, np.float64),
+                                              np.ascontiguousarray(out_flow, np.float64), len(vals),
+                                              np.ascontiguousarray(rows, np.uint64),
+                                              np.ascontiguousarray(cols, np.uint64),
+                                              np.ascontiguousarray(vals, np.float64)), "add_flowmap_tables")
 
     def add_feed(self, kind, f0, param, inp, out, species, conc):
         sp = np.ascontiguousarray(species, np.uint32)

@@ -117,6 +117,28 @@ int mcst_run_set_model_params(mcst_engine* e, int32_t run, const double* params,
  * ReactorDomain::update provide upstream, domain.cpp:43-74). */
 int mcst_run_add_flowmap(mcst_engine* e, int32_t run, const double* volumes, uint64_t nnz, const uint32_t* rows,
                          const uint32_t* cols, const double* flows);
+/* One flowmap from the tables the reference's hydrodynamic state hands to
+ * the step (rcmtool's output): ReactorDomain::update(volumes, neighbors_flat,
+ * out_flows, proba_flat) (apps/libs/mc/src/domain.cpp:44-74) plus the
+ * liquid transition matrix of ScalarSimulation::set_transition
+ * (apps/libs/simulation/src/implScalar.cpp:316-320).
+ *   volumes[n_c]                    liquid volumes V (m3)
+ *   neighbors[n_c][n_cols]          neighbour ids, row-major, in the caller's
+ *                                   order and padding (size_t in the reference)
+ *   cumulative[n_c][n_cols]         cumulative leaving probabilities of those
+ *                                   columns, non-decreasing along a row; a
+ *                                   leaving particle takes the first column with
+ *                                   cumulative >= u (move_kernel.hpp:61-103)
+ *   out_flows[n_c]                  diag_transition Q_out: leaves if dt*Q/V > u
+ *   rows/cols/values[nnz]           transition matrix M as COO (off-diagonal
+ *                                   F(i,j), diagonal -Q_out(i)); transport is
+ *                                   m += dt*(C*M - C*diag(sink) + S); duplicates
+ *                                   are summed
+ * Errors (MCST_ERR_ARG): an id out of range, a non-finite value, a
+ * decreasing cumulative row, a non-positive volume or negative out-flow. */
+int mcst_run_add_flowmap_tables(mcst_engine* e, int32_t run, const double* volumes, uint32_t n_cols,
+                                const uint64_t* neighbors, const double* cumulative, const double* out_flows,
+                                uint64_t nnz, const uint64_t* rows, const uint64_t* cols, const double* values);
 /* The flowmap of step k is (k / steps) mod n_flowmaps. */
 int mcst_run_set_steps_per_flowmap(mcst_engine* e, int32_t run, uint32_t steps);
 /* A feed (FeedDescriptor, apps/libs/simulation/public/simulation/feed_descriptor.hpp:11-161):

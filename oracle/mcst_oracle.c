@@ -581,6 +581,15 @@ static int cmp_col_row(const void* a, const void* b) {
   return 0;
 }
 
+static int cmp_col_row_ord(const void* a, const void* b) {
+  const coo_t* x = (const coo_t*)a;
+  const coo_t* y = (const coo_t*)b;
+  if (x->j != y->j) return x->j < y->j ? -1 : 1;
+  if (x->i != y->i) return x->i < y->i ? -1 : 1;
+  if (x->ord != y->ord) return x->ord < y->ord ? -1 : 1;
+  return 0;
+}
+
 /* Canonical flow list: off-diagonal, positive, duplicates summed in input
  * order, sorted by (row, col). */
 static coo_t* canonical_flows(uint32_t n, uint64_t nnz, const uint32_t* rows, const uint32_t* cols,
@@ -700,6 +709,76 @@ static int flowmap_build(flowmap* f, uint32_t n, const double* V, uint64_t nnz, 
   free(e);
   free(deg);
   free(rs);
+  return 0;
+}
+
+/* A flowmap from the tables the reference's hydrodynamic state hands over
+ * (ReactorDomain::update, domain.cpp:44-74, and set_transition,
+ * implScalar.cpp:316-320): the neighbour rows and cumulative probability rows
+ * as given (n x n_cols, any neighbour order, any padding), the out-flows
+ * (diag_transition: p_move = dt*Q/V), and the transition matrix as COO
+ * triplets (off-diagonal F(i,j), diagonal -Q_out(i)).  The triplets are taken
+ * by destination column, ascending source, duplicates summed in input order
+ * (as sparse_from_coo sums them). */
+static int flowmap_from_tables(flowmap* f, uint32_t n, const double* V, uint32_t n_cols, const uint64_t* nbr,
+                               const double* cdf, const double* out_flow, uint64_t nnz, const uint64_t* rows,
+                               const uint64_t* cols, const double* vals, double dt) {
+  memset(f, 0, sizeof(*f));
+  if (n_cols == 0) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!(V[i] > 0.0) || !isfinite(V[i]) || !(out_flow[i] >= 0.0) || !isfinite(out_flow[i])) return -3;
+    for (uint32_t k = 0; k < n_cols; ++k) {
+      const size_t o = (size_t)i * n_cols + k;
+      if (nbr[o] >= n || !isfinite(cdf[o])) return -3;
+      if (k && cdf[o] < cdf[o - 1]) return -3; /* rows must be cumulative */
+    }
+  }
+  coo_t* T = (coo_t*)malloc(sizeof(coo_t) * (nnz ? nnz : 1));
+  for (uint64_t t = 0; t < nnz; ++t) {
+    if (rows[t] >= n || cols[t] >= n || !isfinite(vals[t])) {
+      free(T);
+      return -3;
+    }
+    T[t].i = (uint32_t)rows[t];
+    T[t].j = (uint32_t)cols[t];
+    T[t].ord = t;
+    T[t].q = vals[t];
+  }
+  qsort(T, nnz, sizeof(coo_t), cmp_col_row_ord);
+  f->n = n;
+  f->m = n_cols;
+  f->V = (double*)malloc(sizeof(double) * n);
+  f->invV = (double*)malloc(sizeof(double) * n);
+  f->out_flow = (double*)malloc(sizeof(double) * n);
+  f->p_move = (double*)malloc(sizeof(double) * n);
+  f->cdf = (double*)malloc(sizeof(double) * (size_t)n * n_cols);
+  f->nbr = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * n_cols);
+  for (uint32_t i = 0; i < n; ++i) {
+    f->V[i] = V[i];
+    f->invV[i] = 1.0 / V[i];
+    f->out_flow[i] = out_flow[i];
+    f->p_move[i] = (dt * out_flow[i]) / V[i];
+  }
+  for (size_t o = 0; o < (size_t)n * n_cols; ++o) {
+    f->cdf[o] = cdf[o];
+    f->nbr[o] = (uint32_t)nbr[o];
+  }
+  f->col_ptr = (uint64_t*)calloc((size_t)n + 1, sizeof(uint64_t));
+  f->src = (uint32_t*)malloc(sizeof(uint32_t) * (nnz ? nnz : 1));
+  f->val = (double*)malloc(sizeof(double) * (nnz ? nnz : 1));
+  uint64_t w = 0;
+  for (uint64_t t = 0; t < nnz;) {
+    coo_t acc = T[t];
+    uint64_t u = t + 1;
+    for (; u < nnz && T[u].i == acc.i && T[u].j == acc.j; ++u) acc.q += T[u].q;
+    f->col_ptr[acc.j + 1]++;
+    f->src[w] = acc.i;
+    f->val[w] = acc.q;
+    ++w;
+    t = u;
+  }
+  for (uint32_t j = 0; j < n; ++j) f->col_ptr[j + 1] += f->col_ptr[j];
+  free(T);
   return 0;
 }
 
@@ -871,6 +950,20 @@ int orc_add_flowmap(orc_run* r, const double* V, uint64_t nnz, const uint32_t* r
   const int rc = flowmap_build(&r->maps[r->n_maps], r->n_c, V, nnz, rows, cols, flows, r->dt);
   if (rc) return rc;
   r->n_maps++;
+  return 0;
+}
+
+int orc_add_flowmap_tables(orc_run* r, const double* V, uint32_t n_cols, const uint64_t* nbr, const double* cdf,
+                           const double* out_flow, uint64_t nnz, const uint64_t* rows, const uint64_t* cols,
+                           const double* vals) {
+  flowmap f;
+  const int rc = flowmap_from_tables(&f, r->n_c, V, n_cols, nbr, cdf, out_flow, nnz, rows, cols, vals, r->dt);
+  if (rc) {
+    flowmap_free(&f);
+    return rc;
+  }
+  r->maps = (flowmap*)realloc(r->maps, sizeof(flowmap) * (r->n_maps + 1));
+  r->maps[r->n_maps++] = f;
   return 0;
 }
 

@@ -96,6 +96,12 @@ void orc_destroy(orc_run* r);
 int orc_set_model_params(orc_run* r, const double* p, int n);
 int orc_add_flowmap(orc_run* r, const double* V, uint64_t nnz, const uint32_t* rows,
                     const uint32_t* cols, const double* flows);
+/* A flowmap from ready-made tables (the reference's ReactorDomain::update and
+ * set_transition): neighbour and cumulative-probability rows [n][n_cols] as
+ * given, out-flows [n], and the transition matrix as COO triplets. */
+int orc_add_flowmap_tables(orc_run* r, const double* V, uint32_t n_cols, const uint64_t* nbr, const double* cdf,
+                           const double* out_flow, uint64_t nnz, const uint64_t* rows, const uint64_t* cols,
+                           const double* vals);
 int orc_set_steps_per_map(orc_run* r, uint32_t k);
 int orc_add_feed(orc_run* r, int type, double f0, double param, uint32_t input, int64_t output,
                  uint32_t n_values, const uint32_t* species, const double* conc);

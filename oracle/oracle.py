@@ -74,6 +74,8 @@ def lib():
         L.orc_set_model_params.argtypes = [C.c_void_p, _f64p, C.c_int]
         L.orc_add_flowmap.argtypes = [C.c_void_p, _f64p, C.c_uint64, _u32p, _u32p, _f64p]
         L.orc_set_steps_per_map.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_add_flowmap_tables.argtypes = [C.c_void_p, _f64p, C.c_uint32, _u64p, _f64p, _f64p, C.c_uint64,
+                                             _u64p, _u64p, _f64p]
         L.orc_add_feed.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_int64,
                                    C.c_uint32, _u32p, _f64p]
         L.orc_set_concentrations.argtypes = [C.c_void_p, _f64p]
@@ -195,6 +197,17 @@ class OracleRun:
 
     def set_steps_per_map(self, k):
         _check(lib().orc_set_steps_per_map(self.h, k), "steps_per_map")
+
+    def add_flowmap_tables(self, V, nbr, cdf, out_flow, rows, cols, vals):
+        """A flowmap from neighbour/cumulative-probability rows [n][m] as given,
+        out-flows [n] and the transition matrix as COO triplets."""
+        nbr = np.ascontiguousarray(nbr, np.uint64)
+        _check(lib().orc_add_flowmap_tables(self.h, np.ascontiguousarray(V, np.float64), nbr.shape[1], nbr,
+                                            np.ascontiguousarray(cdf, np.float64),
+                                            np.ascontiguousarray(out_flow, np.float64), len(vals),
+                                            np.ascontiguousarray(rows, np.uint64),
+                                            np.ascontiguousarray(cols, np.uint64),
+                                            np.ascontiguousarray(vals, np.float64)), "add_flowmap_tables")
 
     def add_feed(self, kind, f0, param, inp, out, species, conc):
         sp = np.ascontiguousarray(species, np.uint32)

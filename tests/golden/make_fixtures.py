@@ -5,8 +5,8 @@
 Writes data only (inputs and expected outputs):
   reference_kats.json   known-answer data held by the reference's own tests:
       - transition matrices of apps/libs/cma_utils/tests/test_transition_matrix.cpp:7-26
-      - the 16-compartment flow matrix of apps/libs/cma_utils/tests/test_transport.cpp:96-373
-        (numbers read from the file's text)
+      - the 16-compartment flow matrix and neighbour table of
+        apps/libs/cma_utils/tests/test_transport.cpp:96-373 (numbers read from the file's text)
       - the 0-D case of apps/api/tests/data/0d (vofL.raw: u32 count + f64 volumes;
         flowL.raw: u32 rows, u32 cols + f64 values)
       - feed flows of apps/libs/simulation/tests/test_feed.cpp:44-123 (f0=1,
@@ -39,6 +39,20 @@ def read_transport16(ref):
     nums = [float(x) for x in re.findall(r"[-+]?\d+\.\d*(?:[eE][-+]?\d+)?", body)]
     assert len(nums) == 256, len(nums)
     return np.array(nums).reshape(16, 16)
+
+
+def read_neighbors16(ref):
+    """The neighbour table of the same test (test_transport.cpp:96-114): rows
+    in rcmtool's order, padded with repeated ids (the compartment's own id
+    where it has fewer neighbours)."""
+    path = os.path.join(ref, "apps/libs/cma_utils/tests/test_transport.cpp")
+    text = open(path).read()
+    body = text[text.index("get_neighbors() //"):]
+    body = body[body.index("return {{"):]
+    body = body[: body.index("}};") + 3]
+    rows = [[int(x) for x in re.findall(r"\d+", r)] for r in re.findall(r"\{([\d,\s]+)\}", body)]
+    assert len(rows) == 16 and all(len(r) == 3 for r in rows), rows
+    return rows
 
 
 def read_0d(ref):
@@ -110,6 +124,8 @@ def main():
                            "source": "apps/libs/cma_utils/tests/test_transition_matrix.cpp:21-26"},
         "transport16_flows": {"flows": read_transport16(ref).tolist(),
                               "source": "apps/libs/cma_utils/tests/test_transport.cpp:96-373"},
+        "transport16_neighbors": {"neighbors": read_neighbors16(ref),
+                                  "source": "apps/libs/cma_utils/tests/test_transport.cpp:96-114"},
         "case_0d": dict(read_0d(ref), source="apps/api/tests/data/0d"),
         "feed": dict(feed_expectations(), source="apps/libs/simulation/tests/test_feed.cpp:44-123"),
     }
