@@ -2030,6 +2030,62 @@ int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist) {
   return MCST_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free repartition
+// The device scratch of mcst_run_get_repartition_noalloc: one counter per
+// compartment, in a file-scope device array that each GPU the library runs
+// on holds until the process ends (8 MiB).  Host code uses the address
+// hipGetSymbolAddress gives for the current device, never the array's name
+// (which on the host is only a stand-in for the symbol).
+namespace {
+constexpr uint32_t NOALLOC_MAX_COMPARTMENTS = 1u << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_MAX_COMPARTMENTS];
+
+namespace {
+struct ScratchTable {
+  std::mutex mu;
+  unsigned long long* ptr[NOALLOC_MAX_DEVICES] = {};
+};
+
+// The device address of `scratch` on `device`, looked up once per device
+// (on the first call on it) with that device current.
+int scratch_on_device(int device, unsigned long long** out) {
+  static ScratchTable t;
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(MCST_ERR_DEVICE, "device id above the scratch table");
+  std::lock_guard<std::mutex> lock(t.mu);
+  if (!t.ptr[device]) {
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p)
+      return fail(MCST_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+    t.ptr[device] = (unsigned long long*)p;
+  }
+  *out = t.ptr[device];
+  return MCST_OK;
+}
+}  // namespace
+
+int mcst_run_get_repartition_noalloc(mcst_engine* e, int32_t run, uint64_t* hist) {
+  CHECK_READY(e, run);
+  if (!hist) return fail(MCST_ERR_ARG, "null output");
+  const uint32_t n_c = e->h_runs[run].n_c;
+  if (n_c > NOALLOC_MAX_COMPARTMENTS)
+    return fail(MCST_ERR_ARG, "run has " + std::to_string(n_c) + " compartments; the allocation-free repartition holds " +
+                                  std::to_string(NOALLOC_MAX_COMPARTMENTS));
+  HIP_TRY(hipSetDevice(e->device));
+  unsigned long long* d = nullptr;
+  const int rc = scratch_on_device(e->device, &d);
+  if (rc) return rc;
+  // the array keeps the last call's counts: clear the run's n_c counters
+  launch_clear_u64(d, n_c, e->stream);
+  launch_repartition(e->bd, (uint32_t)run, d, e->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(hist, d, n_c * sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return MCST_OK;
+}
+
 int mcst_run_get_properties(mcst_engine* e, int32_t run, double* values, double* spatial) {
   CHECK_READY(e, run);
   if (!values && !spatial) return fail(MCST_ERR_ARG, "null outputs");

@@ -24,6 +24,8 @@
 // holds the others.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "step_kernels.hpp"
 
 namespace mcst {
@@ -274,6 +276,13 @@ __global__ void __launch_bounds__(256) k_repartition(BatchDev b, uint32_t r, uns
   }
 }
 
+// Clears n counters (a kernel rather than a memset node, so that a caller
+// may capture it in a graph).
+__global__ void __launch_bounds__(256) k_clear_u64(unsigned long long* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
 // ------------------------------------------------------------------ launchers
 const ModelOps* builtin_model_ops(int model_id) {
   static const ModelOps ops[] = {model_ops<MonodMass>(), model_ops<DualMonod>(), model_ops<Monod>(),
@@ -294,6 +303,11 @@ void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s) 
 
 void launch_layout(const BatchDev& b, hipStream_t s) {
   hipLaunchKernelGGL(k_layout, dim3(1), dim3(SCAN_THREADS), 0, s, b);
+}
+
+void launch_clear_u64(unsigned long long* p, uint64_t n, hipStream_t s) {
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_clear_u64, dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(256), 0, s, p, n);
 }
 
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s) {

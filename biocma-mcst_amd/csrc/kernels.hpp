@@ -39,6 +39,7 @@ const ModelOps* builtin_model_ops(int model_id);
 void launch_removal(const BatchDev& b, uint32_t n_var, int grid, hipStream_t s);
 void launch_layout(const BatchDev& b, hipStream_t s);
 void launch_repartition(const BatchDev& b, uint32_t run, unsigned long long* hist, hipStream_t s);
+void launch_clear_u64(unsigned long long* p, uint64_t n, hipStream_t s);
 
 // Stable sort of one run's particles by position (sort.hip).  pos, status
 // and the n_f64 FP64 fields (state variables and both ages) are the run's

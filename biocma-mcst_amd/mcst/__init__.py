@@ -36,6 +36,7 @@ EXPORTS = [
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
     "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
     "mcst_engine_get_steps", "mcst_model_load", "mcst_run_set_growth", "mcst_run_add_flowmap_tables",
+    "mcst_run_get_repartition_noalloc",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -99,6 +100,7 @@ def lib():
             "mcst_run_get_mass": [_vp, i32, _f64p],
             "mcst_run_get_contributions": [_vp, i32, _f64p],
             "mcst_run_get_repartition": [_vp, i32, _u64p],
+            "mcst_run_get_repartition_noalloc": [_vp, i32, _u64p],
             "mcst_run_get_exit_ages": [_vp, i32, _vp, u64, C.POINTER(u64)],
             "mcst_run_get_division_ages": [_vp, i32, _vp, u64, C.POINTER(u64)],
             "mcst_run_checkpoint": [_vp, i32, _vp, u64, C.POINTER(u64)],
@@ -381,6 +383,13 @@ class Run:
     def repartition(self):
         h = np.zeros(self.n_c, np.uint64)
         _ck(lib().mcst_run_get_repartition(self._h, self.id, h), "get_repartition")
+        return h
+
+    def repartition_noalloc(self):
+        """repartition() through the entry point that allocates no device
+        memory (mcst_run_get_repartition_noalloc)."""
+        h = np.zeros(self.n_c, np.uint64)
+        _ck(lib().mcst_run_get_repartition_noalloc(self._h, self.id, h), "get_repartition_noalloc")
         return h
 
     def _probe(self, fn):

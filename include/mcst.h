@@ -272,6 +272,14 @@ int mcst_run_get_mass(mcst_engine* e, int32_t run, double* m);
 /* Particle sources of the last step, [n_species][n_c] (kg/s). */
 int mcst_run_get_contributions(mcst_engine* e, int32_t run, double* S);
 int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c] */);
+/* mcst_run_get_repartition without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its counters live in one library-owned device array per GPU (8 MiB, held
+ * until the process ends), so a run may have at most 2^20 compartments
+ * (MCST_ERR_ARG beyond).  Calls on one GPU share that array: calls that
+ * overlap on one GPU (from several threads or engines at once) race. */
+int mcst_run_get_repartition_noalloc(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c] */);
 /* Exit / division ages recorded so far (up to the probe capacity). */
 /* Export values (PostProcessing::get_properties, apps/core/public/core/post_process.hpp:54-98,173-190):
  * values[n_var+1][n]: each Idle slot's model state then Model::mass (zero
