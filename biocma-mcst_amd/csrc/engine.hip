@@ -368,6 +368,7 @@ int build_map_tables(uint32_t n, const double* V, uint32_t n_cols, const uint64_
 // ------------------------------------------------------------ runs
 struct HostFeed {
   int kind;
+  bool gas;  // a feed of the gas phase: its output is a gas outlet only
   double f0, param;
   uint32_t in;
   int64_t out;
@@ -385,6 +386,17 @@ struct HostRun {
   std::vector<HostFeed> feeds;
   std::vector<double> C0;
   std::vector<double> kla, cstar;  // gas-liquid transfer, [n_s][n_c] (empty: none)
+  // two-phase runs: Henry constants, initial gas concentrations, the gas side
+  // of each flowmap with its energy dissipation (empty: none), the kLa rule
+  bool two_phase = false;
+  double henry[MAX_SPECIES] = {};
+  std::vector<double> Cg0;
+  std::vector<HostMap> gmaps;
+  std::vector<std::vector<double>> g_eps;
+  int kla_mode = MCST_KLA_FIXED;
+  double kla_fixed[MAX_SPECIES] = {};
+  uint32_t kla_species = 0;
+  double bubble_d = 5e-3;
   std::vector<uint32_t> vf;        // time-dependent feeds (indices into feeds), in feed order
   std::vector<uint32_t> vx_out;    // outlets whose first leaving feed is time-dependent ...
   std::vector<uint32_t> vx_feed;   // ... and that feed (index into vf)
@@ -409,7 +421,7 @@ struct HostRun {
   // state restored from a checkpoint (mcst_run_restore): liquid mass and
   // pending contributions, step, counters and probes
   bool restored = false;
-  std::vector<double> r_m, r_S, r_exit, r_div;
+  std::vector<double> r_m, r_S, r_exit, r_div, r_cg, r_mg;
   uint32_t r_step = 0;
   unsigned long long r_counters[C_COUNT] = {};
   // device-side placement
@@ -563,10 +575,11 @@ HostRun* get_run(mcst_engine* e, int32_t run) {
 // Feed sources F[s][c], sink[c] (constant feeds) and the per-map leave
 // thresholds exp(-dt*Q/V) of the first leaving flow of each outlet
 // (update_feed, simulation.model.cpp:72-122; find_flow, move_kernel.hpp:105-127).
-void feed_tables(const HostRun& r, std::vector<double>& F, std::vector<double>& sink) {
+void feed_tables(const HostRun& r, std::vector<double>& F, std::vector<double>& sink, bool gas = false) {
   F.assign((size_t)r.n_s * r.n_c, 0.0);
   sink.assign(r.n_c, 0.0);
   for (const auto& fd : r.feeds) {
+    if (fd.gas != gas) continue;
     if (fd.kind != MCST_FEED_CONSTANT) continue;  // added by the transport kernel each step
     const double q = fd.f0;
     for (size_t v = 0; v < fd.sp.size(); ++v) F[(size_t)fd.sp[v] * r.n_c + fd.in] += q * fd.conc[v];
@@ -590,7 +603,7 @@ void exit_thresholds(const HostRun& r, HostMap& mp) {
   mp.q_exit.assign(r.n_c, 2.0);
   std::vector<char> seen(r.n_c, 0);
   for (const auto& fd : r.feeds) {
-    if (fd.out < 0) continue;
+    if (fd.out < 0 || fd.gas) continue;
     const uint32_t o = (uint32_t)fd.out;
     if (seen[o]) continue;
     seen[o] = 1;
@@ -599,9 +612,47 @@ void exit_thresholds(const HostRun& r, HostMap& mp) {
   }
 }
 
+// Kinematic viscosity of water at 20 C as the reference's correlation gives
+// it (impl_mtr.cpp:58-84 at temperature 20, :98-99), and the oxygen
+// diffusion constant (impl_mtr.cpp:97).
+constexpr double WATER_NU_20C = 1.0023e-6;
+constexpr double O2_DIFFUSION = 1e-9;
+
+// kLa [n_s][n_c] of flowmap k of a two-phase run: FixedKla rows, or for
+// one species kL = 0.3 (eps nu)^(1/4) Sc^(-1/2) times the interfacial area
+// a = 6 alpha / (d_b (1 - alpha)), alpha = V_gas / (V_liq + V_gas)
+// (impl_mtr.cpp:21-50,90-134).
+std::vector<double> gas_kla(const HostRun& h, size_t k) {
+  const uint32_t n = h.n_c;
+  std::vector<double> kla((size_t)h.n_s * n, 0.0);
+  if (h.kla_mode == MCST_KLA_FIXED) {
+    for (uint32_t s = 0; s < h.n_s; ++s)
+      for (uint32_t j = 0; j < n; ++j) kla[(size_t)s * n + j] = h.kla_fixed[s];
+    return kla;
+  }
+  const double nu = WATER_NU_20C;
+  const double sc = nu / O2_DIFFUSION;
+  const HostMap &f = h.maps[k], &g = h.gmaps[k];
+  for (uint32_t j = 0; j < n; ++j) {
+    const double kl = (0.3 * std::pow(h.g_eps[k][j] * nu, 0.25)) * std::pow(sc, -0.5);
+    const double alpha = g.V[j] / (f.V[j] + g.V[j]);
+    const double a = (6.0 * alpha) / (h.bubble_d * (1.0 - alpha));
+    kla[(size_t)h.kla_species * n + j] = kl * a;
+  }
+  return kla;
+}
+
 int check_definable(const mcst_engine* e, const HostRun& h, uint32_t r) {
   if (h.maps.empty()) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no flowmap");
   if (!h.has_particles) return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has no particles");
+  if (h.two_phase && h.gmaps.size() != h.maps.size())
+    return fail(MCST_ERR_STATE, "two-phase run " + std::to_string(r) + " needs one gas flowmap per flowmap");
+  if (h.two_phase && !h.kla.empty())
+    return fail(MCST_ERR_STATE, "run " + std::to_string(r) + " has both a gas phase and a fixed gas transfer");
+  if (h.two_phase && h.kla_mode == MCST_KLA_TURBULENCE)
+    for (size_t k = 0; k < h.gmaps.size(); ++k)
+      if (h.g_eps[k].empty())
+        return fail(MCST_ERR_STATE, "turbulence kLa needs the energy dissipation of every gas flowmap");
   return MCST_OK;
 }
 
@@ -763,6 +814,16 @@ int upload_run(mcst_engine* e, uint32_t r) {
     if ((rc = run_upload(h, (double**)&md.val, mp.val.data(), mp.val.size()))) return rc;
     if ((rc = run_upload(h, (double**)&md.inv_v, mp.inv_v.data(), mp.inv_v.size()))) return rc;
     if ((rc = run_upload(h, (double**)&md.vol, mp.V.data(), mp.V.size()))) return rc;
+    if (h.two_phase) {
+      const size_t k = e->h_maps.size() - h.map_base;
+      const HostMap& g = h.gmaps[k];
+      const std::vector<double> kla = gas_kla(h, k);
+      if ((rc = run_upload(h, (uint32_t**)&md.g_col_ptr, g.col_ptr.data(), g.col_ptr.size()))) return rc;
+      if ((rc = run_upload(h, (uint32_t**)&md.g_src, g.src.data(), g.src.size()))) return rc;
+      if ((rc = run_upload(h, (double**)&md.g_val, g.val.data(), g.val.size()))) return rc;
+      if ((rc = run_upload(h, (double**)&md.g_inv_v, g.inv_v.data(), g.inv_v.size()))) return rc;
+      if ((rc = run_upload(h, (double**)&md.g_kla, kla.data(), kla.size()))) return rc;
+    }
     e->h_maps.push_back(md);
   }
   const size_t nsc = (size_t)h.n_s * h.n_c;
@@ -781,6 +842,19 @@ int upload_run(mcst_engine* e, uint32_t r) {
   if ((rc = run_upload(h, &d.S, h.restored ? h.r_S.data() : zeros.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.F, F.data(), nsc))) return rc;
   if ((rc = run_upload(h, (double**)&d.sink, sink.data(), sink.size()))) return rc;
+  if (h.two_phase) {
+    std::vector<double> Fg, sink_g, mg(nsc);
+    feed_tables(h, Fg, sink_g, true);
+    const std::vector<double>& cg = h.restored ? h.r_cg : h.Cg0;
+    for (uint32_t sp = 0; sp < h.n_s; ++sp)
+      for (uint32_t j = 0; j < h.n_c; ++j) mg[(size_t)sp * h.n_c + j] = cg[(size_t)sp * h.n_c + j] * h.gmaps[0].V[j];
+    if ((rc = run_upload(h, &d.Cg[0], cg.data(), nsc))) return rc;
+    if ((rc = run_upload(h, &d.Cg[1], cg.data(), nsc))) return rc;
+    if ((rc = run_upload(h, &d.mg, h.restored ? h.r_mg.data() : mg.data(), nsc))) return rc;
+    if ((rc = run_upload(h, (double**)&d.Fg, Fg.data(), nsc))) return rc;
+    if ((rc = run_upload(h, (double**)&d.sink_g, sink_g.data(), sink_g.size()))) return rc;
+    for (uint32_t sp = 0; sp < h.n_s; ++sp) d.henry[sp] = h.henry[sp];
+  }
   h.vf.clear();
   h.vx_out.clear();
   h.vx_feed.clear();
@@ -804,18 +878,19 @@ int upload_run(mcst_engine* e, uint32_t r) {
     std::vector<uint32_t> fill_s(sp_ptr.begin(), sp_ptr.end() - 1), fill_k(sk_ptr.begin(), sk_ptr.end() - 1);
     for (uint32_t q = 0; q < h.vf.size(); ++q) {
       const HostFeed& fd = h.feeds[h.vf[q]];
+      const uint32_t phase = fd.gas ? VF_GAS : 0u;
       for (size_t v = 0; v < fd.sp.size(); ++v) {
         const uint32_t at = fill_s[fd.in]++;
-        src_k[at] = q, src_s[at] = fd.sp[v], src_c[at] = fd.conc[v];
+        src_k[at] = q, src_s[at] = fd.sp[v] | phase, src_c[at] = fd.conc[v];
       }
-      if (fd.out >= 0) sink_k[fill_k[(size_t)fd.out]++] = q;
+      if (fd.out >= 0) sink_k[fill_k[(size_t)fd.out]++] = q | phase;
     }
     // outlets whose first leaving feed is time-dependent take this step's threshold
     std::vector<int32_t> vx_slot(h.n_c, -1);
     std::vector<char> seen(h.n_c, 0);
     for (uint32_t k = 0; k < h.feeds.size(); ++k) {
       const HostFeed& fd = h.feeds[k];
-      if (fd.out < 0 || seen[(size_t)fd.out]) continue;
+      if (fd.out < 0 || fd.gas || seen[(size_t)fd.out]) continue;
       seen[(size_t)fd.out] = 1;
       if (vf_of[k] >= 0) {
         vx_slot[(size_t)fd.out] = (int32_t)h.vx_out.size();
@@ -866,7 +941,7 @@ int upload_run(mcst_engine* e, uint32_t r) {
   d.enable_move = h.n_c > 1 ? 1u : 0u;
   d.enable_leave = 0;
   for (const auto& fd : h.feeds)
-    if (fd.out >= 0) d.enable_leave = 1;
+    if (fd.out >= 0 && !fd.gas) d.enable_leave = 1;
   d.n_maps = (uint32_t)h.maps.size();
   d.steps_per_map = h.steps_per_map;
   d.map_base = h.map_base;
@@ -901,6 +976,8 @@ int upload_run(mcst_engine* e, uint32_t r) {
   std::vector<double>().swap(h.r_S);
   std::vector<double>().swap(h.r_exit);
   std::vector<double>().swap(h.r_div);
+  std::vector<double>().swap(h.r_cg);
+  std::vector<double>().swap(h.r_mg);
   h.phase = 1;
   return MCST_OK;
 }
@@ -1505,14 +1582,14 @@ int mcst_run_set_steps_per_flowmap(mcst_engine* e, int32_t run, uint32_t steps) 
   return MCST_OK;
 }
 
-int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
-                      int64_t output, uint32_t n_values, const uint32_t* species, const double* conc) {
+static int add_feed(mcst_engine* e, int32_t run, bool gas, int kind, double f0, double param, uint32_t input,
+                    int64_t output, uint32_t n_values, const uint32_t* species, const double* conc) {
   CHECK_DEFINE(e, run, h);
   if (kind < MCST_FEED_CONSTANT || kind > MCST_FEED_EXPONENTIAL) return fail(MCST_ERR_ARG, "bad feed kind");
   if (input >= h->n_c || output >= (int64_t)h->n_c) return fail(MCST_ERR_ARG, "feed position out of range");
   if (f0 < 0.0) return fail(MCST_ERR_ARG, "negative feed flow");  // feed_descriptor.cpp:53-57
   if (n_values > MAX_SPECIES || (n_values && (!species || !conc))) return fail(MCST_ERR_ARG, "bad feed values");
-  HostFeed fd{kind, f0, param, input, output < 0 ? -1 : output, {}, {}};
+  HostFeed fd{kind, gas, f0, param, input, output < 0 ? -1 : output, {}, {}};
   for (uint32_t v = 0; v < n_values; ++v) {
     if (species[v] >= h->n_s) return fail(MCST_ERR_ARG, "feed species out of range");
     if (!(conc[v] > 0.0)) return fail(MCST_ERR_ARG, "feed concentration must be positive");
@@ -1520,6 +1597,71 @@ int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double p
     fd.conc.push_back(conc[v]);
   }
   h->feeds.push_back(std::move(fd));
+  return MCST_OK;
+}
+
+int mcst_run_add_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
+                      int64_t output, uint32_t n_values, const uint32_t* species, const double* conc) {
+  return add_feed(e, run, false, kind, f0, param, input, output, n_values, species, conc);
+}
+
+int mcst_run_add_gas_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
+                          int64_t output, uint32_t n_values, const uint32_t* species, const double* conc) {
+  return add_feed(e, run, true, kind, f0, param, input, output, n_values, species, conc);
+}
+
+int mcst_run_set_gas_phase(mcst_engine* e, int32_t run, const double* henry, const double* c_gas) {
+  CHECK_DEFINE(e, run, h);
+  if (!henry || !c_gas) return fail(MCST_ERR_ARG, "null gas phase array");
+  const size_t nsc = (size_t)h->n_s * h->n_c;
+  for (uint32_t sp = 0; sp < h->n_s; ++sp)
+    if (!(henry[sp] >= 0.0) || !std::isfinite(henry[sp])) return fail(MCST_ERR_ARG, "Henry constants must be finite, >= 0");
+  for (size_t i = 0; i < nsc; ++i)
+    if (!(c_gas[i] >= 0.0) || !std::isfinite(c_gas[i])) return fail(MCST_ERR_ARG, "negative initial gas concentration");
+  h->two_phase = true;
+  for (uint32_t sp = 0; sp < h->n_s; ++sp) h->henry[sp] = henry[sp];
+  h->Cg0.assign(c_gas, c_gas + nsc);
+  return MCST_OK;
+}
+
+int mcst_run_set_kla(mcst_engine* e, int32_t run, int mode, const double* kla, uint32_t species,
+                     double bubble_diameter) {
+  CHECK_DEFINE(e, run, h);
+  if (mode == MCST_KLA_FIXED) {
+    if (!kla) return fail(MCST_ERR_ARG, "null kLa values");
+    for (uint32_t sp = 0; sp < h->n_s; ++sp)
+      if (!(kla[sp] >= 0.0) || !std::isfinite(kla[sp])) return fail(MCST_ERR_ARG, "kLa must be finite, >= 0");
+    for (uint32_t sp = 0; sp < h->n_s; ++sp) h->kla_fixed[sp] = kla[sp];
+  } else if (mode == MCST_KLA_TURBULENCE) {
+    if (species >= h->n_s) return fail(MCST_ERR_ARG, "kLa species out of range");
+    if (!(bubble_diameter > 0.0) || !std::isfinite(bubble_diameter))
+      return fail(MCST_ERR_ARG, "bubble diameter must be positive");
+    h->kla_species = species;
+    h->bubble_d = bubble_diameter;
+  } else {
+    return fail(MCST_ERR_ARG, "unknown kLa rule");
+  }
+  h->kla_mode = mode;
+  return MCST_OK;
+}
+
+int mcst_run_add_gas_flowmap(mcst_engine* e, int32_t run, const double* gas_volumes, uint64_t nnz,
+                             const uint32_t* rows, const uint32_t* cols, const double* flows,
+                             const double* energy_dissipation) {
+  CHECK_DEFINE(e, run, h);
+  if (!gas_volumes || (nnz && (!rows || !cols || !flows))) return fail(MCST_ERR_ARG, "null gas flowmap array");
+  HostMap g;
+  const int rc = build_map(h->n_c, gas_volumes, nnz, rows, cols, flows, h->dt, g);
+  if (rc) return rc;
+  std::vector<double> eps;
+  if (energy_dissipation) {
+    for (uint32_t j = 0; j < h->n_c; ++j)
+      if (!(energy_dissipation[j] >= 0.0) || !std::isfinite(energy_dissipation[j]))
+        return fail(MCST_ERR_ARG, "energy dissipation must be finite, >= 0");
+    eps.assign(energy_dissipation, energy_dissipation + h->n_c);
+  }
+  h->gmaps.push_back(std::move(g));
+  h->g_eps.push_back(std::move(eps));
   return MCST_OK;
 }
 
@@ -2003,7 +2145,9 @@ static int get_liquid(mcst_engine* e, int32_t run, double* out, int which) {
   if (rc) return rc;
   const RunDev& d = e->h_runs[run];
   const size_t nsc = (size_t)d.n_s * d.n_c;
-  const double* src = which == 0 ? d.C[s.step & 1u] : (which == 1 ? d.m : d.S);
+  const double* src = which == 0 ? d.C[s.step & 1u] : (which == 1 ? d.m : which == 2 ? d.S :
+                      which == 3 ? d.Cg[s.step & 1u] : d.mg);
+  if (!src) return fail(MCST_ERR_STATE, "run has no gas phase");
   HIP_TRY(hipMemcpy(out, src, nsc * sizeof(double), hipMemcpyDeviceToHost));
   return MCST_OK;
 }
@@ -2011,6 +2155,8 @@ static int get_liquid(mcst_engine* e, int32_t run, double* out, int which) {
 int mcst_run_get_concentrations(mcst_engine* e, int32_t run, double* C) { return get_liquid(e, run, C, 0); }
 int mcst_run_get_mass(mcst_engine* e, int32_t run, double* m) { return get_liquid(e, run, m, 1); }
 int mcst_run_get_contributions(mcst_engine* e, int32_t run, double* S) { return get_liquid(e, run, S, 2); }
+int mcst_run_get_gas_concentrations(mcst_engine* e, int32_t run, double* C) { return get_liquid(e, run, C, 3); }
+int mcst_run_get_gas_mass(mcst_engine* e, int32_t run, double* m) { return get_liquid(e, run, m, 4); }
 
 int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist) {
   CHECK_READY(e, run);
@@ -2135,24 +2281,27 @@ int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_
 // ---------------------------------------------------------------- checkpoints
 // A checkpoint is one flat little-endian buffer: the header below, then
 // pos u32[n], status u8[n] padded to 8 bytes, state f64[n_var][n],
-// age_hyd f64[n], age_div f64[n], C, m, S f64[n_s][n_c] each, and the
-// recorded exit and division ages.  It holds everything a step carries to
+// age_hyd f64[n], age_div f64[n], C, m, S f64[n_s][n_c] each, the
+// recorded exit and division ages, and for a two-phase run (flags bit 0)
+// the gas concentrations and gas mass f64[n_s][n_c] each.  It holds everything a step carries to
 // the next (random draws are keyed by step, so none is stored); the network,
 // feeds and model parameters are the run's definition and are not stored.
 // The header's `model` is the model name's tag (model_tag).
 namespace {
 struct CkptHeader {
   char magic[8];
-  uint32_t version, model, n_c, n_s, nv, step;
+  uint32_t version, model, n_c, n_s, nv, step, flags, pad;
   uint64_t n, cap, seed, n_exit, n_div;
   double dt, weight;
   unsigned long long counters[C_COUNT];
 };
 constexpr char CKPT_MAGIC[8] = {'M', 'C', 'S', 'T', 'C', 'K', 'P', 'T'};
-constexpr uint32_t CKPT_VERSION = 2;
+constexpr uint32_t CKPT_VERSION = 3;
+constexpr uint32_t CKPT_TWO_PHASE = 1u;
 
-uint64_t ckpt_bytes(uint64_t n, uint32_t nv, uint64_t nsc, uint64_t n_exit, uint64_t n_div) {
-  return sizeof(CkptHeader) + 4 * n + ((n + 7) / 8) * 8 + 8 * ((uint64_t)nv * n + 2 * n + 3 * nsc + n_exit + n_div);
+uint64_t ckpt_bytes(uint64_t n, uint32_t nv, uint64_t nsc, uint64_t n_exit, uint64_t n_div, bool gas) {
+  return sizeof(CkptHeader) + 4 * n + ((n + 7) / 8) * 8 +
+         8 * ((uint64_t)nv * n + 2 * n + (gas ? 5 : 3) * nsc + n_exit + n_div);
 }
 }  // namespace
 
@@ -2166,7 +2315,8 @@ int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, u
   const uint64_t n = s.n, nsc = (uint64_t)d.n_s * d.n_c;
   const uint64_t n_exit = std::min<uint64_t>(s.probe_exit_n, d.probe_cap);
   const uint64_t n_div = std::min<uint64_t>(s.probe_div_n, d.probe_cap);
-  const uint64_t bytes = ckpt_bytes(n, nv, nsc, n_exit, n_div);
+  const bool gas = d.Cg[0] != nullptr;
+  const uint64_t bytes = ckpt_bytes(n, nv, nsc, n_exit, n_div, gas);
   if (needed) *needed = bytes;
   if (!buf) return MCST_OK;
   if (size < bytes) return fail(MCST_ERR_ARG, "checkpoint buffer too small (" + std::to_string(bytes) + " bytes needed)");
@@ -2178,6 +2328,7 @@ int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, u
   hd.n_s = d.n_s;
   hd.nv = nv;
   hd.step = s.step;
+  hd.flags = gas ? CKPT_TWO_PHASE : 0u;
   hd.n = n;
   hd.cap = d.cap;
   hd.seed = d.seed;
@@ -2208,6 +2359,10 @@ int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, u
   HIP_TRY(down(d.S, 8 * nsc));
   HIP_TRY(down(d.probe_exit, 8 * n_exit));
   HIP_TRY(down(d.probe_div, 8 * n_div));
+  if (gas) {
+    HIP_TRY(down(d.Cg[s.step & 1u], 8 * nsc));
+    HIP_TRY(down(d.mg, 8 * nsc));
+  }
   return MCST_OK;
 }
 
@@ -2224,7 +2379,11 @@ int mcst_run_restore(mcst_engine* e, int32_t run, const void* buf, uint64_t size
     return fail(MCST_ERR_ARG, "checkpoint of a run with another compartment count, species count, dt or seed");
   const uint64_t n = hd.n, nsc = (uint64_t)hd.n_s * hd.n_c;
   if (hd.cap < n || hd.cap >= 0xFFFFFFFFull) return fail(MCST_ERR_ARG, "bad particle counts in the checkpoint");
-  if (size < ckpt_bytes(n, nv, nsc, hd.n_exit, hd.n_div)) return fail(MCST_ERR_ARG, "truncated checkpoint");
+  const bool gas = (hd.flags & CKPT_TWO_PHASE) != 0;
+  if (gas != h->two_phase)
+    return fail(MCST_ERR_ARG, gas ? "checkpoint of a two-phase run: define the gas phase first"
+                                  : "checkpoint of a run without a gas phase");
+  if (size < ckpt_bytes(n, nv, nsc, hd.n_exit, hd.n_div, gas)) return fail(MCST_ERR_ARG, "truncated checkpoint");
   const char* p = (const char*)buf + sizeof(hd);
   auto take = [&](auto& vec, uint64_t count) {
     using T = typename std::decay_t<decltype(vec)>::value_type;
@@ -2251,6 +2410,10 @@ int mcst_run_restore(mcst_engine* e, int32_t run, const void* buf, uint64_t size
   take(h->r_S, nsc);
   take(h->r_exit, hd.n_exit);
   take(h->r_div, hd.n_div);
+  if (gas) {
+    take(h->r_cg, nsc);
+    take(h->r_mg, nsc);
+  }
   h->n = n;
   h->cap = hd.cap;
   h->weight = hd.weight;

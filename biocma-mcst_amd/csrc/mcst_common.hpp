@@ -69,9 +69,18 @@ struct MapDev {
   const double* val;        // [nnz]   F(i,j) off-diagonal, -Q_out(j) on the diagonal
   const double* inv_v;      // [n_c]
   const double* vol;        // [n_c]   liquid volumes V
+  // gas side of the flowmap in a two-phase run (null otherwise)
+  const uint32_t* g_col_ptr;  // [n_c+1] gas transport matrix by destination column
+  const uint32_t* g_src;
+  const double* g_val;
+  const double* g_inv_v;    // [n_c]   1 / gas volume
+  const double* g_kla;      // [n_s][n_c] kLa of the flowmap
   uint32_t m;
   uint32_t pad;
 };
+
+// vf_src_s / vf_sink_k entries that belong to the gas phase
+constexpr uint32_t VF_GAS = 0x80000000u;
 
 // Immutable description of one run.
 struct RunDev {
@@ -89,6 +98,14 @@ struct RunDev {
   const double* sink;       // [n_c]      outlet flow per compartment
   const double* kla;        // [n_s][n_c] gas-liquid transfer coefficient (null: no transfer)
   const double* cstar;      // [n_s][n_c] saturation concentration set by the gas phase
+  // gas phase of a two-phase run (Cg[0] null otherwise): concentrations
+  // (ping-pong like C), mass, constant gas feed sources, gas outlet flows,
+  // Henry constants (C* = H C_gas)
+  double* Cg[2];
+  double* mg;
+  const double* Fg;         // [n_s][n_c]
+  const double* sink_g;     // [n_c]
+  double henry[MAX_SPECIES];
   // time-dependent (linear, exponential) feeds: their flows this step, then
   // the exit thresholds of the outlets they lead (written by the host before
   // each step); sources and sinks by compartment (CSR); vx_slot[j] >= 0
@@ -96,10 +113,10 @@ struct RunDev {
   const double* vf_q;
   const uint32_t* vf_src_ptr;  // [n_c+1]
   const uint32_t* vf_src_k;    // feed (index into vf_q) of each source entry
-  const uint32_t* vf_src_s;    // its species
+  const uint32_t* vf_src_s;    // its species (| VF_GAS for a gas feed)
   const double* vf_src_c;      // its concentration
   const uint32_t* vf_sink_ptr; // [n_c+1]
-  const uint32_t* vf_sink_k;
+  const uint32_t* vf_sink_k;   // (| VF_GAS for a gas outlet)
   const int32_t* vx_slot;      // [n_c]
   uint32_t n_vf;
   double* rec;              // [n_c][rec] per-compartment record read by the particle pass

@@ -92,6 +92,39 @@ constexpr uint32_t SCAN_WAVES = SCAN_THREADS / 64;
 template <uint32_t NT>
 __device__ void layout_pass(const BatchDev& b);
 
+// sum_i C[s][i] * M(i, j) over column j of a transport matrix stored by
+// destination column, for every species: entries in groups of 8 whose loads
+// are all in flight together (a column holds the degree plus the diagonal,
+// rarely more than 8); the sums still run in entry order.
+template <int NS>
+__device__ __forceinline__ void column_inflow(const uint32_t* __restrict__ col_ptr, const uint32_t* __restrict__ src,
+                                              const double* __restrict__ val, const double* c, uint32_t n_c,
+                                              uint32_t j, double (&acc)[NS]) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  const uint32_t e0 = col_ptr[j], e1 = col_ptr[j + 1];
+  constexpr uint32_t EG = 8;
+  for (uint32_t e = e0; e < e1; e += EG) {
+    uint32_t i[EG];
+    double v[EG], cv[EG][NS];
+#pragma unroll
+    for (uint32_t k = 0; k < EG; ++k) {
+      const uint32_t ek = e + k < e1 ? e + k : e1 - 1u;
+      i[k] = src[ek];
+      v[k] = val[ek];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < EG; ++k)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) cv[k][s] = c[(size_t)s * n_c + i[k]];
+#pragma unroll
+    for (uint32_t k = 0; k < EG; ++k)
+      if (e + k < e1)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s] += cv[k][s] * v[k];
+  }
+}
+
 template <class M>
 __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
   if (blockIdx.x == gridDim.x - 1) {
@@ -124,60 +157,72 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) cn[s] = c[(size_t)s * n_c + j];
     if (b.stages & STAGE_TRANSPORT) {
-      const uint32_t e0 = mp.col_ptr[j], e1 = mp.col_ptr[j + 1];
-      const uint32_t* __restrict__ src = mp.src;
-      const double* __restrict__ val = mp.val;
       double acc[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) acc[s] = 0.0;
-      // entries in groups of 8 whose loads are all in flight together (a
-      // column holds the degree plus the diagonal, rarely more than 8);
-      // the sums still run in entry order
-      constexpr uint32_t EG = 8;
-      for (uint32_t e = e0; e < e1; e += EG) {
-        uint32_t i[EG];
-        double v[EG], cv[EG][NS];
-#pragma unroll
-        for (uint32_t k = 0; k < EG; ++k) {
-          const uint32_t ek = e + k < e1 ? e + k : e1 - 1u;
-          i[k] = src[ek];
-          v[k] = val[ek];
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < EG; ++k)
-#pragma unroll
-          for (int s = 0; s < NS; ++s) cv[k][s] = c[(size_t)s * n_c + i[k]];
-#pragma unroll
-        for (uint32_t k = 0; k < EG; ++k)
-          if (e + k < e1)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) acc[s] += cv[k][s] * v[k];
-      }
+      column_inflow<NS>(mp.col_ptr, mp.src, mp.val, c, n_c, j, acc);
       double sink = rd.sink[j];
       const double inv_v = mp.inv_v[j];
-      double fv[NS];  // time-dependent feeds of this compartment, in feed order
+      const bool gas = rd.Cg[0] != nullptr;
+      double fv[NS], fvg[NS];  // time-dependent feeds of this compartment, in feed order
+      double sink_g = gas ? rd.sink_g[j] : 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) fv[s] = 0.0;
+      for (int s = 0; s < NS; ++s) fv[s] = fvg[s] = 0.0;
       if (rd.n_vf) {
-        for (uint32_t q = rd.vf_sink_ptr[j]; q < rd.vf_sink_ptr[j + 1]; ++q) sink = sink + rd.vf_q[rd.vf_sink_k[q]];
+        for (uint32_t q = rd.vf_sink_ptr[j]; q < rd.vf_sink_ptr[j + 1]; ++q) {
+          const uint32_t k = rd.vf_sink_k[q];
+          if (k & VF_GAS)
+            sink_g = sink_g + rd.vf_q[k & ~VF_GAS];
+          else
+            sink = sink + rd.vf_q[k];
+        }
         for (uint32_t q = rd.vf_src_ptr[j]; q < rd.vf_src_ptr[j + 1]; ++q) {
           const double add = rd.vf_q[rd.vf_src_k[q]] * rd.vf_src_c[q];
           const uint32_t sp = rd.vf_src_s[q];
 #pragma unroll
-          for (int s = 0; s < NS; ++s)
+          for (int s = 0; s < NS; ++s) {
             if ((uint32_t)s == sp) fv[s] += add;
+            if (((uint32_t)s | VF_GAS) == sp) fvg[s] += add;
+          }
         }
       }
+      if (gas) {
+        // two-phase step (SimulationUnit::ode_step, simulation.model.cpp:131-150):
+        // mtr = kla*(H*C_gas - C)*V from the old concentrations, the gas
+        // step with -mtr and the liquid step with +mtr (performStepGL,
+        // implScalar.cpp:228-248), then clearNegs on the liquid
+        // (implScalar.cpp:266-297): a value in (-1e-4*5e-3, 0) becomes 0
+        const double* cg = rd.Cg[step & 1u];
+        double* Cgn = rd.Cg[(step + 1u) & 1u];
+        double accg[NS];
+        column_inflow<NS>(mp.g_col_ptr, mp.g_src, mp.g_val, cg, n_c, j, accg);
+        const double inv_vg = mp.g_inv_v[j];
+        const double tol = 1e-4 * 5e-3;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const size_t o = (size_t)s * n_c + j;
-        double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]));
-        // gas-liquid transfer from a fixed gas phase (performStepGL,
-        // implScalar.cpp:228-248): kla*(C* - C)*V
-        if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];
-        const double mm = rd.m[o] + rd.dt * dmdt;
-        rd.m[o] = mm;
-        cn[s] = mm * inv_v;
+        for (int s = 0; s < NS; ++s) {
+          const size_t o = (size_t)s * n_c + j;
+          const double cgs = cg[o];
+          const double mtr = (mp.g_kla[o] * ((cgs * rd.henry[s]) - cn[s])) * mp.vol[j];
+          const double dg = ((accg[s] - cgs * sink_g) + (rd.Fg[o] + fvg[s])) - mtr;
+          const double mgo = rd.mg[o] + rd.dt * dg;
+          rd.mg[o] = mgo;
+          Cgn[o] = mgo * inv_vg;
+          const double dl = ((acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]))) + mtr;
+          const double mm = rd.m[o] + rd.dt * dl;
+          rd.m[o] = mm;
+          double cl = mm * inv_v;
+          if (cl < 0.0 && fabs(cl) < tol) cl = 0.0;
+          cn[s] = cl;
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          const size_t o = (size_t)s * n_c + j;
+          double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]));
+          // gas-liquid transfer from a fixed gas phase: kla*(C* - C)*V
+          if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];
+          const double mm = rd.m[o] + rd.dt * dmdt;
+          rd.m[o] = mm;
+          cn[s] = mm * inv_v;
+        }
       }
     }
     double rt[NR];

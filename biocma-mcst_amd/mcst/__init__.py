@@ -15,6 +15,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "libmcst.so")
 
 FEED_KINDS = {"constant": 0, "linear": 1, "exponential": 2}
+KLA_MODES = {"fixed": 0, "turbulence": 1}
 COUNTERS = ["steps", "births", "exits", "deaths", "moves", "overflow", "compactions", "n", "inactive", "processed"]
 STAGES = {"transport": 1, "model": 2, "contrib": 4, "move": 8, "leave": 16, "remove": 32}
 STAGE_ALL = 63
@@ -36,7 +37,8 @@ EXPORTS = [
     "mcst_run_checkpoint", "mcst_run_restore", "mcst_run_get_properties", "mcst_engine_autotune",
     "mcst_run_set_gas_transfer", "mcst_run_set_sort_interval", "mcst_run_sort_particles",
     "mcst_engine_get_steps", "mcst_model_load", "mcst_run_set_growth", "mcst_run_add_flowmap_tables",
-    "mcst_run_get_repartition_noalloc",
+    "mcst_run_get_repartition_noalloc", "mcst_run_set_gas_phase", "mcst_run_set_kla", "mcst_run_add_gas_flowmap",
+    "mcst_run_add_gas_feed", "mcst_run_get_gas_concentrations", "mcst_run_get_gas_mass",
 ]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -108,6 +110,12 @@ def lib():
             "mcst_run_get_properties": [_vp, i32, _vp, _vp],
             "mcst_engine_autotune": [_vp, u32, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)],
             "mcst_run_set_gas_transfer": [_vp, i32, _f64p, _f64p],
+            "mcst_run_set_gas_phase": [_vp, i32, _f64p, _f64p],
+            "mcst_run_set_kla": [_vp, i32, C.c_int, _vp, u32, dbl],
+            "mcst_run_add_gas_flowmap": [_vp, i32, _f64p, u64, _u32p, _u32p, _f64p, _vp],
+            "mcst_run_add_gas_feed": [_vp, i32, C.c_int, dbl, dbl, u32, i64, u32, _u32p, _f64p],
+            "mcst_run_get_gas_concentrations": [_vp, i32, _f64p],
+            "mcst_run_get_gas_mass": [_vp, i32, _f64p],
             "mcst_run_set_sort_interval": [_vp, i32, u32],
             "mcst_run_sort_particles": [_vp, i32],
             "mcst_engine_get_steps": [_vp, _u32p, u32],
@@ -319,6 +327,40 @@ class Run:
         cstar = np.ascontiguousarray(np.broadcast_to(np.asarray(cstar, float).reshape(self.n_s, -1),
                                                      (self.n_s, self.n_c)), np.float64)
         _ck(lib().mcst_run_set_gas_transfer(self._h, self.id, kla, cstar), "set_gas_transfer")
+
+    def set_gas_phase(self, henry, c_gas):
+        """Two-phase run: Henry constants [n_s], initial gas concentrations
+        [n_s][n_c] (or per species)."""
+        c_gas = np.ascontiguousarray(np.broadcast_to(np.asarray(c_gas, float).reshape(self.n_s, -1),
+                                                     (self.n_s, self.n_c)), np.float64)
+        _ck(lib().mcst_run_set_gas_phase(self._h, self.id, np.ascontiguousarray(henry, np.float64), c_gas),
+            "set_gas_phase")
+
+    def set_kla(self, mode, kla=None, species=0, bubble_diameter=5e-3):
+        """mode "fixed" (kla per species) or "turbulence" (species, bubble diameter)."""
+        k = None if kla is None else np.ascontiguousarray(kla, np.float64)
+        _ck(lib().mcst_run_set_kla(self._h, self.id, KLA_MODES[mode], None if k is None else k.ctypes.data,
+                                   species, bubble_diameter), "set_kla")
+
+    def add_gas_flowmap(self, Vg, rows, cols, flows, eps=None):
+        e = None if eps is None else np.ascontiguousarray(eps, np.float64)
+        _ck(lib().mcst_run_add_gas_flowmap(self._h, self.id, np.ascontiguousarray(Vg, np.float64), len(flows),
+                                           np.ascontiguousarray(rows, np.uint32),
+                                           np.ascontiguousarray(cols, np.uint32),
+                                           np.ascontiguousarray(flows, np.float64),
+                                           None if e is None else e.ctypes.data), "add_gas_flowmap")
+
+    def add_gas_feed(self, kind, f0, param, inp, out, species, conc):
+        sp = np.ascontiguousarray(species, np.uint32)
+        cc = np.ascontiguousarray(conc, np.float64)
+        _ck(lib().mcst_run_add_gas_feed(self._h, self.id, FEED_KINDS[kind], f0, param, inp,
+                                        -1 if out is None else out, len(sp), sp, cc), "add_gas_feed")
+
+    def gas_concentrations(self):
+        return self._mat("mcst_run_get_gas_concentrations")
+
+    def gas_mass(self):
+        return self._mat("mcst_run_get_gas_mass")
 
     def set_probes(self, cap):
         _ck(lib().mcst_run_set_probes(self._h, self.id, cap), "set_probes")

@@ -34,6 +34,10 @@ class Case:
     x0: float = 1.0
     n_steps: int = 1
     gas: tuple = None               # (kla[n_s], C*[n_s]) gas-liquid transfer, or None
+    # two-phase run: {"henry": [n_s], "c_gas": [n_s][n_c], "kla": ("fixed", [n_s]) or
+    # ("turbulence", species, d_b), "gas_maps": [(Vg, rows, cols, flows, eps)] (one per
+    # flowmap), "gas_feeds": [(kind, f0, param, in, out|None, species, conc)]}
+    two_phase: dict = None
 
     def apply(self, run, particles=True):
         if self.params is not None:
@@ -46,6 +50,17 @@ class Case:
         run.set_concentrations(self.C0)
         if self.gas is not None:
             run.set_gas_transfer(*self.gas)
+        if self.two_phase is not None:
+            tp = self.two_phase
+            run.set_gas_phase(tp["henry"], tp["c_gas"])
+            if tp["kla"][0] == "fixed":
+                run.set_kla("fixed", tp["kla"][1])
+            else:
+                run.set_kla("turbulence", species=tp["kla"][1], bubble_diameter=tp["kla"][2])
+            for Vg, r, c, q, eps in tp["gas_maps"]:
+                run.add_gas_flowmap(Vg, r, c, q, eps)
+            for f in tp.get("gas_feeds", []):
+                run.add_gas_feed(*f)
         if particles:
             run.init_particles(self.n_particles, self.capacity or 2 * self.n_particles, self.x0)
         return run
@@ -273,6 +288,71 @@ def small_case(model="monod_mass", n_c=12, n_particles=5000, seed=3, n_maps=2, s
 
 
 MONOD_MASS_RING64 = np.array([0.5 / 3600.0, 0.01, 0.5, 2.0, 1e-15, 0.1])
+
+
+def bubble_column(n_particles=20000, levels=10, columns=3, seed=4, n_maps=2, steps_per_map=5, kla="turbulence",
+                  model="dual_monod", holdup=0.05, air_o2=0.28, dt=0.01):
+    """A two-phase bubble column: `columns` side-by-side stacks of `levels`
+    compartments (compartment c*levels + z).  Liquid: an up-flow loop in the
+    first stack and down-flow in the others, exchanges between neighbouring
+    compartments, a glucose feed at the bottom of the first stack and an
+    outlet at the top of the last.  Gas: air (oxygen `air_o2` kg/m3) fed at
+    the bottom of every stack and rising level by level to a gas outlet at
+    the top; gas volume `holdup` of the compartment; energy dissipation
+    highest at the sparger level.  Species: glucose (0) and oxygen (1, Henry
+    constant 3.181e-2 as the reference's mass transfer model sets,
+    mass_transfer.cpp:125).  kLa from the turbulence correlation for oxygen
+    (or fixed per species with kla="fixed")."""
+    rng = np.random.default_rng(seed)
+    n = levels * columns
+
+    def cid(c, z):
+        return c * levels + z
+
+    V = rng.uniform(0.8, 1.2, n) * L
+    Vg = V * holdup / (1.0 - holdup)
+    maps, gas_maps = [], []
+    for _ in range(n_maps):
+        pairs = {}
+
+        def add(a, b, q):
+            pairs[(a, b)] = pairs.get((a, b), 0.0) + q
+
+        loop = rng.uniform(2e-5, 5e-5)
+        for z in range(levels - 1):  # up the first stack, down the others
+            add(cid(0, z), cid(0, z + 1), loop)
+            for c in range(1, columns):
+                add(cid(c, z + 1), cid(c, z), loop / (columns - 1))
+        for c in range(1, columns):  # closing the loop at the ends
+            add(cid(0, levels - 1), cid(c, levels - 1), loop / (columns - 1))
+            add(cid(c, 0), cid(0, 0), loop / (columns - 1))
+        for c in range(columns):
+            for z in range(levels):
+                if c + 1 < columns:
+                    q = rng.uniform(1e-6, 1e-5)
+                    add(cid(c, z), cid(c + 1, z), q)
+                    add(cid(c + 1, z), cid(c, z), q)
+        keys = sorted(pairs)
+        maps.append((V, np.array([k[0] for k in keys], np.uint32), np.array([k[1] for k in keys], np.uint32),
+                     np.array([pairs[k] for k in keys])))
+        gq = rng.uniform(1e-5, 2e-5)  # gas rising through each stack
+        g_rows, g_cols, g_flows = [], [], []
+        for c in range(columns):
+            for z in range(levels - 1):
+                g_rows.append(cid(c, z))
+                g_cols.append(cid(c, z + 1))
+                g_flows.append(gq * (1.0 + 0.1 * z))
+        eps = np.array([2.0 if z == 0 else 0.2 + 0.05 * rng.random() for c in range(columns) for z in range(levels)])
+        gas_maps.append((Vg, np.array(g_rows, np.uint32), np.array(g_cols, np.uint32), np.array(g_flows), eps))
+    C0 = np.stack([np.full(n, 5.0), np.full(n, 8e-3)])
+    feeds = [("constant", 1e-6, 0.0, cid(0, 0), cid(columns - 1, levels - 1), [0], [10.0])]
+    gas_feeds = [("constant", 2e-5, 0.0, cid(c, 0), cid(c, levels - 1), [1], [air_o2]) for c in range(columns)]
+    tp = {"henry": [0.0, 3.181e-2], "c_gas": np.stack([np.zeros(n), np.full(n, air_o2)]),
+          "kla": ("turbulence", 1, 5e-3) if kla == "turbulence" else ("fixed", [0.0, 0.05]),
+          "gas_maps": gas_maps, "gas_feeds": gas_feeds}
+    p = np.array([2.8e-3, 0.01, 1e-4, 0.5, 1.0, 10.0, 1e-3, 2e-15, 1.9e-15, 0.05e-15])
+    return Case("bubble_column", model, n, 2, dt, seed, maps, steps_per_map=steps_per_map, feeds=feeds, C0=C0,
+                params=p, n_particles=n_particles, x0=5e-3, n_steps=200, two_phase=tp)
 
 
 def ensemble_run(i, seed=11, n_steps=500, n_c_range=(32, 4000), n_range=(1e4, 2e6)):

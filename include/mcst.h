@@ -53,6 +53,10 @@ extern "C" {
 #define MCST_FEED_LINEAR 1
 #define MCST_FEED_EXPONENTIAL 2
 
+/* kLa rules of two-phase runs (MassTransfer::Type, mass_transfer.hpp:18-38) */
+#define MCST_KLA_FIXED 0       /* FixedKla: one value per species */
+#define MCST_KLA_TURBULENCE 1  /* FlowmapTurbulence: from each flowmap's energy dissipation and gas fraction */
+
 /* stage mask of mcst_engine_step_stages */
 #define MCST_STAGE_TRANSPORT 1u
 #define MCST_STAGE_MODEL 2u
@@ -162,6 +166,38 @@ int mcst_run_set_removal(mcst_engine* e, int32_t run, double ratio, uint64_t min
  * liquid mass of species s in compartment j gains dt*kla[s][j]*(c_star[s][j] - C)*V_j
  * each step; c_star = Henry*C_gas is held constant (no gas transport). */
 int mcst_run_set_gas_transfer(mcst_engine* e, int32_t run, const double* kla, const double* c_star);
+
+/* ---- two-phase runs: a transported gas phase (SimulationUnit::ode_step with
+ * a gas phase, apps/libs/simulation/src/simulation.model.cpp:131-150) ----
+ * Each step: mtr = kla*(H*C_gas - C_liq)*V_liq from the old concentrations
+ * (mass_transfer.cpp:140-157); the gas phase m_gas += dt*(C_gas*M_gas -
+ * C_gas*diag(sink_gas) + F_gas - mtr) and the liquid m += dt*(C*M -
+ * C*diag(sink) + S + F + mtr) (performStepGL, implScalar.cpp:228-248); then
+ * liquid concentrations in (-5e-7, 0) are set to 0 (clearNegs,
+ * implScalar.cpp:266-297; the mass keeps its value).  A two-phase run cannot
+ * also use mcst_run_set_gas_transfer.
+ * Henry constants henry[n_species] (C* = H*C_gas) and the initial gas
+ * concentrations c_gas[n_species][n_c]. */
+int mcst_run_set_gas_phase(mcst_engine* e, int32_t run, const double* henry, const double* c_gas);
+/* kLa: MCST_KLA_FIXED takes kla[n_species] for every compartment;
+ * MCST_KLA_TURBULENCE gives species `species` kL*a in each compartment of
+ * each flowmap, kL = 0.3*(eps*nu)^(1/4)*Sc^(-1/2) (nu = 1.0023e-6 m2/s,
+ * Sc = nu/1e-9), a = 6*alpha/(d_b*(1 - alpha)), alpha = V_gas/(V_liq + V_gas),
+ * bubble diameter d_b (impl_mtr.cpp:21-50,90-134), and 0 to the others.
+ * Default: fixed, all 0. */
+int mcst_run_set_kla(mcst_engine* e, int32_t run, int mode, const double* kla, uint32_t species,
+                     double bubble_diameter);
+/* The gas side of the flowmap of the same rank: gas volumes [n_c], the gas
+ * flow matrix as COO (as mcst_run_add_flowmap), and the energy dissipation
+ * eps [n_c] (W/kg; may be NULL unless the kLa rule is MCST_KLA_TURBULENCE).
+ * A two-phase run needs one per liquid flowmap. */
+int mcst_run_add_gas_flowmap(mcst_engine* e, int32_t run, const double* gas_volumes, uint64_t nnz,
+                             const uint32_t* rows, const uint32_t* cols, const double* flows,
+                             const double* energy_dissipation);
+/* A gas feed (SimulationFeed::gas_feeds): as mcst_run_add_feed, into the gas
+ * phase; its output is a gas outlet (no particle leaves with it). */
+int mcst_run_add_gas_feed(mcst_engine* e, int32_t run, int kind, double f0, double param, uint32_t input,
+                          int64_t output, uint32_t n_values, const uint32_t* species, const double* conc);
 /* Particle order (no counterpart in the reference, whose container keeps
  * particles in creation order): mcst_run_sort_particles reorders the run's
  * particles by compartment now, stable in slot order; with an interval k > 0
@@ -271,6 +307,9 @@ int mcst_run_get_concentrations(mcst_engine* e, int32_t run, double* C);
 int mcst_run_get_mass(mcst_engine* e, int32_t run, double* m);
 /* Particle sources of the last step, [n_species][n_c] (kg/s). */
 int mcst_run_get_contributions(mcst_engine* e, int32_t run, double* S);
+/* Gas concentrations and gas mass of a two-phase run, [n_species][n_c]. */
+int mcst_run_get_gas_concentrations(mcst_engine* e, int32_t run, double* C);
+int mcst_run_get_gas_mass(mcst_engine* e, int32_t run, double* m);
 int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c] */);
 /* mcst_run_get_repartition without any device allocation.  No counterpart in
  * the reference: for callers that cannot allocate device memory (code under

@@ -848,6 +848,7 @@ int orc_transport_once(uint32_t n, uint32_t n_s, const double* V, uint64_t nnz, 
 
 typedef struct {
   int type;
+  int gas; /* a feed of the gas phase (SimulationFeed::gas_feeds): its output is a gas sink only */
   double f0, param;
   uint32_t in;
   int64_t out;
@@ -881,6 +882,20 @@ struct orc_run {
   uint32_t n_feeds;
   double *C, *C2, *m, *S, *S_total, *sink, *q_exit;
   double *kla, *cstar; /* gas-liquid transfer [n_s][n_c], NULL without */
+  /* two-phase runs: the gas phase (gas side of each flowmap, its state and
+     feeds), Henry constants and the kLa rule */
+  int two_phase;
+  double henry[MAX_SPECIES];
+  double *Cg, *Cg2, *mg, *Sg_total, *sink_g, *Cg0;
+  flowmap* gmaps;
+  double** g_eps; /* energy dissipation per gas map [n_c], NULL if not given */
+  double** g_kla; /* kLa per gas map [n_s][n_c], computed at the first step */
+  uint32_t n_gmaps;
+  int kla_mode;
+  double kla_fixed[MAX_SPECIES];
+  uint32_t kla_species;
+  double bubble_d;
+  int gas_ready;
   uint64_t n, cap;
   uint32_t* pos;
   uint8_t* status;
@@ -932,6 +947,13 @@ void orc_destroy(orc_run* r) {
   free(r->feeds);
   free(r->C); free(r->C2); free(r->m); free(r->S); free(r->S_total); free(r->sink); free(r->q_exit);
   free(r->kla); free(r->cstar);
+  for (uint32_t i = 0; i < r->n_gmaps; ++i) {
+    flowmap_free(&r->gmaps[i]);
+    free(r->g_eps[i]);
+    free(r->g_kla[i]);
+  }
+  free(r->gmaps); free(r->g_eps); free(r->g_kla);
+  free(r->Cg); free(r->Cg2); free(r->mg); free(r->Sg_total); free(r->sink_g); free(r->Cg0);
   free(r->pos); free(r->status); free(r->state); free(r->age_hyd); free(r->age_div);
   free(r->exit_ages.v);
   free(r->div_ages.v);
@@ -973,12 +995,13 @@ int orc_set_steps_per_map(orc_run* r, uint32_t k) {
   return 0;
 }
 
-int orc_add_feed(orc_run* r, int type, double f0, double param, uint32_t input, int64_t output,
-                 uint32_t n_values, const uint32_t* species, const double* conc) {
+static int add_feed_phase(orc_run* r, int gas, int type, double f0, double param, uint32_t input, int64_t output,
+                          uint32_t n_values, const uint32_t* species, const double* conc) {
   if (input >= r->n_c || output >= (int64_t)r->n_c || n_values > MAX_SPECIES || f0 < 0.0) return -1;
   r->feeds = (feed_t*)realloc(r->feeds, sizeof(feed_t) * (r->n_feeds + 1));
   feed_t* f = &r->feeds[r->n_feeds];
   memset(f, 0, sizeof(*f));
+  f->gas = gas;
   f->type = type;
   f->f0 = f0;
   f->param = param;
@@ -991,6 +1014,190 @@ int orc_add_feed(orc_run* r, int type, double f0, double param, uint32_t input, 
     f->conc[i] = conc[i];
   }
   r->n_feeds++;
+  return 0;
+}
+
+int orc_add_feed(orc_run* r, int type, double f0, double param, uint32_t input, int64_t output,
+                 uint32_t n_values, const uint32_t* species, const double* conc) {
+  return add_feed_phase(r, 0, type, f0, param, input, output, n_values, species, conc);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gas phase of two-phase runs: SimulationUnit::ode_step with a gas phase
+ * (simulation.model.cpp:131-150): mtr = kla*(H*C_gas - C_liq)*V_liq
+ * (mass_transfer.cpp:140-157), the gas step with -mtr and the liquid step
+ * with +mtr (performStepGL, implScalar.cpp:228-248), then clearNegs on the
+ * liquid concentrations (implScalar.cpp:266-297).  kLa is fixed per species
+ * (MassTransfer::Type::FixedKla, mass_transfer.cpp:26-37) or, for one
+ * species, kL(eps)*a(alpha_g) from the flowmap's energy dissipation and gas
+ * fraction (FlowmapTurbulence, impl_mtr.cpp:21-50,90-134). */
+int orc_add_gas_feed(orc_run* r, int type, double f0, double param, uint32_t input, int64_t output,
+                     uint32_t n_values, const uint32_t* species, const double* conc) {
+  return add_feed_phase(r, 1, type, f0, param, input, output, n_values, species, conc);
+}
+
+int orc_set_gas_phase(orc_run* r, const double* henry, const double* c_gas) {
+  const size_t nsc = (size_t)r->n_s * r->n_c;
+  if (!henry || !c_gas || r->kla) return -1;
+  for (int s = 0; s < r->n_s; ++s)
+    if (!(henry[s] >= 0.0) || !isfinite(henry[s])) return -1;
+  for (size_t o = 0; o < nsc; ++o)
+    if (!(c_gas[o] >= 0.0) || !isfinite(c_gas[o])) return -1;
+  r->two_phase = 1;
+  for (int s = 0; s < r->n_s; ++s) r->henry[s] = henry[s];
+  free(r->Cg0);
+  r->Cg0 = (double*)malloc(sizeof(double) * nsc);
+  memcpy(r->Cg0, c_gas, sizeof(double) * nsc);
+  r->gas_ready = 0;
+  return 0;
+}
+
+int orc_set_kla(orc_run* r, int mode, const double* kla, uint32_t species, double bubble_d) {
+  if (mode == ORC_KLA_FIXED) {
+    if (!kla) return -1;
+    for (int s = 0; s < r->n_s; ++s) {
+      if (!(kla[s] >= 0.0) || !isfinite(kla[s])) return -1;
+      r->kla_fixed[s] = kla[s];
+    }
+  } else if (mode == ORC_KLA_TURBULENCE) {
+    if (species >= (uint32_t)r->n_s || !(bubble_d > 0.0) || !isfinite(bubble_d)) return -1;
+    r->kla_species = species;
+    r->bubble_d = bubble_d;
+  } else {
+    return -1;
+  }
+  r->kla_mode = mode;
+  r->gas_ready = 0;
+  return 0;
+}
+
+int orc_add_gas_flowmap(orc_run* r, const double* Vg, uint64_t nnz, const uint32_t* rows, const uint32_t* cols,
+                        const double* flows, const double* eps) {
+  flowmap g;
+  const int rc = flowmap_build(&g, r->n_c, Vg, nnz, rows, cols, flows, r->dt);
+  if (rc) return rc;
+  if (eps)
+    for (uint32_t j = 0; j < r->n_c; ++j)
+      if (!(eps[j] >= 0.0) || !isfinite(eps[j])) {
+        flowmap_free(&g);
+        return -3;
+      }
+  r->gmaps = (flowmap*)realloc(r->gmaps, sizeof(flowmap) * (r->n_gmaps + 1));
+  r->g_eps = (double**)realloc(r->g_eps, sizeof(double*) * (r->n_gmaps + 1));
+  r->g_kla = (double**)realloc(r->g_kla, sizeof(double*) * (r->n_gmaps + 1));
+  r->gmaps[r->n_gmaps] = g;
+  r->g_eps[r->n_gmaps] = NULL;
+  if (eps) {
+    r->g_eps[r->n_gmaps] = (double*)malloc(sizeof(double) * r->n_c);
+    memcpy(r->g_eps[r->n_gmaps], eps, sizeof(double) * r->n_c);
+  }
+  r->g_kla[r->n_gmaps] = NULL;
+  r->n_gmaps++;
+  r->gas_ready = 0;
+  return 0;
+}
+
+/* Kinematic viscosity of water at 20 C as the reference's correlation gives
+   it (impl_mtr.cpp:58-84 at temperature 20, impl_mtr.cpp:98-99), and the
+   oxygen diffusion constant (impl_mtr.cpp:97). */
+#define GL_WATER_NU_20C 1.0023e-6
+#define GL_O2_DIFFUSION 1e-9
+
+/* kLa [n_s][n] of gas map g with liquid map f (FixedKla rows, or the
+   turbulence correlation kL = 0.3 (eps nu)^(1/4) Sc^(-1/2) times the
+   interfacial area a = 6 alpha/(d_b (1 - alpha)), alpha = Vg/(Vl + Vg)). */
+static void gas_kla(const orc_run* r, const flowmap* f, const flowmap* g, const double* eps, double* kla) {
+  const uint32_t n = r->n_c;
+  for (int s = 0; s < r->n_s; ++s)
+    for (uint32_t j = 0; j < n; ++j)
+      kla[(size_t)s * n + j] = r->kla_mode == ORC_KLA_FIXED ? r->kla_fixed[s] : 0.0;
+  if (r->kla_mode != ORC_KLA_TURBULENCE) return;
+  const double nu = GL_WATER_NU_20C;
+  const double sc = nu / GL_O2_DIFFUSION;
+  for (uint32_t j = 0; j < n; ++j) {
+    const double e = eps ? eps[j] : 0.0;
+    const double kl = (0.3 * pow(e * nu, 0.25)) * pow(sc, -0.5);
+    const double alpha = g->V[j] / (f->V[j] + g->V[j]);
+    const double a = (6.0 * alpha) / (r->bubble_d * (1.0 - alpha));
+    kla[(size_t)r->kla_species * n + j] = kl * a;
+  }
+}
+
+/* Checks and first-step set-up of a two-phase run: one gas map per liquid
+   map, the gas mass from the initial gas concentrations and the first gas
+   volumes, the kLa of every map. */
+static int gas_prepare(orc_run* r) {
+  if (r->gas_ready) return 0;
+  if (r->n_gmaps != r->n_maps || r->kla) return -4;
+  const uint32_t n = r->n_c;
+  const size_t nsc = (size_t)r->n_s * n;
+  if (!r->Cg) {
+    r->Cg = (double*)calloc(nsc, sizeof(double));
+    r->Cg2 = (double*)calloc(nsc, sizeof(double));
+    r->mg = (double*)calloc(nsc, sizeof(double));
+    r->Sg_total = (double*)calloc(nsc, sizeof(double));
+    r->sink_g = (double*)calloc(n, sizeof(double));
+    for (int s = 0; s < r->n_s; ++s)
+      for (uint32_t j = 0; j < n; ++j) {
+        const size_t o = (size_t)s * n + j;
+        r->Cg[o] = r->Cg0[o];
+        r->mg[o] = r->Cg0[o] * r->gmaps[0].V[j];
+      }
+  }
+  for (uint32_t k = 0; k < r->n_gmaps; ++k) {
+    free(r->g_kla[k]);
+    r->g_kla[k] = (double*)malloc(sizeof(double) * nsc);
+    gas_kla(r, &r->maps[k], &r->gmaps[k], r->g_eps[k], r->g_kla[k]);
+  }
+  r->gas_ready = 1;
+  return 0;
+}
+
+/* One two-phase transport step: mtr from the old concentrations, the gas
+   step with -mtr, the liquid step with +mtr, clearNegs on the liquid. */
+static void transport_gl(orc_run* r, const flowmap* f, const flowmap* g, const double* kla) {
+  const uint32_t n = r->n_c;
+  const double dt = r->dt;
+  const double tol = 1e-4 * 5e-3; /* clearNegs: O(dt) relative error x the species' magnitude */
+  for (int s = 0; s < r->n_s; ++s) {
+    const double* cl = r->C + (size_t)s * n;
+    const double* cg = r->Cg + (size_t)s * n;
+    for (uint32_t j = 0; j < n; ++j) {
+      const size_t o = (size_t)s * n + j;
+      const double mtr = (kla[o] * ((cg[j] * r->henry[s]) - cl[j])) * f->V[j];
+      double ag = 0.0;
+      for (uint64_t t = g->col_ptr[j]; t < g->col_ptr[j + 1]; ++t) ag += cg[g->src[t]] * g->val[t];
+      const double dg = ((ag - cg[j] * r->sink_g[j]) + r->Sg_total[o]) - mtr;
+      r->mg[o] = r->mg[o] + dt * dg;
+      r->Cg2[o] = r->mg[o] * g->invV[j];
+      double al = 0.0;
+      for (uint64_t t = f->col_ptr[j]; t < f->col_ptr[j + 1]; ++t) al += cl[f->src[t]] * f->val[t];
+      const double dl = ((al - cl[j] * r->sink[j]) + r->S_total[o]) + mtr;
+      r->m[o] = r->m[o] + dt * dl;
+      double c = r->m[o] * f->invV[j];
+      if (c < 0.0 && fabs(c) < tol) c = 0.0;
+      r->C2[o] = c;
+    }
+  }
+}
+
+int orc_get_gas_concentrations(const orc_run* r, double* C) {
+  if (!r->two_phase) return -1;
+  const size_t nsc = (size_t)r->n_s * r->n_c;
+  memcpy(C, r->Cg ? r->Cg : r->Cg0, sizeof(double) * nsc);
+  return 0;
+}
+
+int orc_get_gas_mass(const orc_run* r, double* m) {
+  if (!r->two_phase) return -1;
+  const size_t nsc = (size_t)r->n_s * r->n_c;
+  if (r->mg) {
+    memcpy(m, r->mg, sizeof(double) * nsc);
+  } else {
+    if (r->n_gmaps == 0) return -1;
+    for (int s = 0; s < r->n_s; ++s)
+      for (uint32_t j = 0; j < r->n_c; ++j) m[(size_t)s * r->n_c + j] = r->Cg0[(size_t)s * r->n_c + j] * r->gmaps[0].V[j];
+  }
   return 0;
 }
 
@@ -1359,13 +1566,24 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
     /* update_feed (simulation.model.cpp:72-122).  Feed terms of one
      * (species, compartment) are summed in feed order, then added to the
      * particle sources: S_total = S + (f1 + f2 + ...). */
+    if (r->two_phase && gas_prepare(r)) return -4;
     memset(r->S_total, 0, sizeof(double) * nsc);
     memset(r->sink, 0, sizeof(double) * n_c);
+    if (r->two_phase) {
+      memset(r->Sg_total, 0, sizeof(double) * nsc);
+      memset(r->sink_g, 0, sizeof(double) * n_c);
+    }
     for (uint32_t j = 0; j < n_c; ++j) r->q_exit[j] = 2.0;
     unsigned char* has_exit = (unsigned char*)calloc(n_c, 1);
     for (uint32_t i = 0; i < r->n_feeds; ++i) {
       const feed_t* fd = &r->feeds[i];
       const double q = feed_flow(fd, t);
+      if (fd->gas) { /* the gas scalar's feed and sink; no particle leaves with it */
+        if (!r->two_phase) continue;
+        for (uint32_t v = 0; v < fd->nv; ++v) r->Sg_total[(size_t)fd->sp[v] * n_c + fd->in] += q * fd->conc[v];
+        if (fd->out >= 0) r->sink_g[(uint32_t)fd->out] += q;
+        continue;
+      }
       for (uint32_t v = 0; v < fd->nv; ++v) r->S_total[(size_t)fd->sp[v] * n_c + fd->in] += q * fd->conc[v];
       if (fd->out >= 0) {
         const uint32_t o = (uint32_t)fd->out;
@@ -1380,7 +1598,15 @@ int orc_step_stages(orc_run* r, uint32_t n_steps, int n_threads, uint32_t stages
     for (size_t o = 0; o < nsc; ++o) r->S_total[o] = r->S[o] + r->S_total[o];
     /* ode_step then clear contributions */
     if (stages & ORC_STAGE_TRANSPORT) {
-      transport(f, (uint32_t)ns, r->sink, r->S_total, r->dt, r->C, r->m, r->C2, r->kla, r->cstar);
+      if (r->two_phase) {
+        const uint32_t k = (uint32_t)((step / r->steps_per_map) % r->n_maps);
+        transport_gl(r, f, &r->gmaps[k], r->g_kla[k]);
+        double* tg = r->Cg;
+        r->Cg = r->Cg2;
+        r->Cg2 = tg;
+      } else {
+        transport(f, (uint32_t)ns, r->sink, r->S_total, r->dt, r->C, r->m, r->C2, r->kla, r->cstar);
+      }
       double* tmp = r->C;
       r->C = r->C2;
       r->C2 = tmp;

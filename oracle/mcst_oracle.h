@@ -34,6 +34,9 @@ enum orc_status { ORC_IDLE = 0, ORC_DIVISION = 1, ORC_EXIT = 2, ORC_DEAD = 3 };
 
 enum orc_feed_type { ORC_FEED_CONSTANT = 0, ORC_FEED_LINEAR = 1, ORC_FEED_EXPONENTIAL = 2 };
 
+/* kLa rules of two-phase runs */
+enum orc_kla_mode { ORC_KLA_FIXED = 0, ORC_KLA_TURBULENCE = 1 };
+
 /* counters returned by orc_get_counters, in this order */
 enum orc_counter {
   ORC_C_STEPS = 0,
@@ -109,6 +112,19 @@ int orc_set_concentrations(orc_run* r, const double* C);
 int orc_set_compaction(orc_run* r, double ratio, uint64_t min_removal);
 int orc_set_division_cap(orc_run* r, uint32_t cap);
 int orc_set_gas_transfer(orc_run* r, const double* kla, const double* cstar);
+/* Two-phase runs: Henry constants [n_s] and initial gas concentrations
+ * [n_s][n_c]; the kLa rule (fixed per species, or from the turbulence of the
+ * flowmap for one species with bubble diameter d_b); the gas side of each
+ * flowmap (gas volumes, gas flows as COO, energy dissipation [n_c] or NULL);
+ * gas feeds.  A two-phase run needs one gas flowmap per liquid flowmap. */
+int orc_set_gas_phase(orc_run* r, const double* henry, const double* c_gas);
+int orc_set_kla(orc_run* r, int mode, const double* kla, uint32_t species, double bubble_d);
+int orc_add_gas_flowmap(orc_run* r, const double* Vg, uint64_t nnz, const uint32_t* rows, const uint32_t* cols,
+                        const double* flows, const double* eps);
+int orc_add_gas_feed(orc_run* r, int type, double f0, double param, uint32_t input, int64_t output,
+                     uint32_t n_values, const uint32_t* species, const double* conc);
+int orc_get_gas_concentrations(const orc_run* r, double* C);
+int orc_get_gas_mass(const orc_run* r, double* m);
 int orc_set_sort_interval(orc_run* r, uint32_t steps);
 int orc_sort_particles(orc_run* r);
 int orc_set_growth(orc_run* r, uint32_t every, double factor);

@@ -26,6 +26,7 @@ MODEL_IDS = {
     "udf_starvation": 7,
 }
 FEED_TYPES = {"constant": 0, "linear": 1, "exponential": 2}
+KLA_MODES = {"fixed": 0, "turbulence": 1}
 COUNTERS = ["steps", "births", "exits", "deaths", "moves", "overflow", "compactions", "n", "inactive", "processed"]
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -96,6 +97,13 @@ def lib():
         L.orc_get_repartition.argtypes = [C.c_void_p, _u64p]
         L.orc_get_properties.argtypes = [C.c_void_p, _f64p, _f64p]
         L.orc_set_gas_transfer.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_set_gas_phase.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_set_kla.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.c_double]
+        L.orc_add_gas_flowmap.argtypes = [C.c_void_p, _f64p, C.c_uint64, _u32p, _u32p, _f64p, C.c_void_p]
+        L.orc_add_gas_feed.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_int64,
+                                       C.c_uint32, _u32p, _f64p]
+        L.orc_get_gas_concentrations.argtypes = [C.c_void_p, _f64p]
+        L.orc_get_gas_mass.argtypes = [C.c_void_p, _f64p]
         L.orc_set_sort_interval.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_sort_particles.argtypes = [C.c_void_p]
         L.orc_set_growth.argtypes = [C.c_void_p, C.c_uint32, C.c_double]
@@ -239,6 +247,39 @@ class OracleRun:
         cstar = np.ascontiguousarray(np.broadcast_to(np.asarray(cstar, float).reshape(self.n_s, -1),
                                                      (self.n_s, self.n_c)), np.float64)
         _check(lib().orc_set_gas_transfer(self.h, kla, cstar), "set_gas_transfer")
+
+    def set_gas_phase(self, henry, c_gas):
+        """Two-phase run: Henry constants [n_s], initial gas concentrations
+        [n_s][n_c] (or per species)."""
+        c_gas = np.ascontiguousarray(np.broadcast_to(np.asarray(c_gas, float).reshape(self.n_s, -1),
+                                                     (self.n_s, self.n_c)), np.float64)
+        _check(lib().orc_set_gas_phase(self.h, np.ascontiguousarray(henry, np.float64), c_gas), "set_gas_phase")
+
+    def set_kla(self, mode, kla=None, species=0, bubble_diameter=5e-3):
+        """mode "fixed" (kla per species) or "turbulence" (species, bubble diameter)."""
+        k = None if kla is None else np.ascontiguousarray(kla, np.float64)
+        _check(lib().orc_set_kla(self.h, KLA_MODES[mode], None if k is None else k.ctypes.data, species,
+                                 bubble_diameter), "set_kla")
+        self._kla_keep = k
+
+    def add_gas_flowmap(self, Vg, rows, cols, flows, eps=None):
+        e = None if eps is None else np.ascontiguousarray(eps, np.float64)
+        _check(lib().orc_add_gas_flowmap(self.h, np.ascontiguousarray(Vg, np.float64), len(flows),
+                                         np.ascontiguousarray(rows, np.uint32), np.ascontiguousarray(cols, np.uint32),
+                                         np.ascontiguousarray(flows, np.float64),
+                                         None if e is None else e.ctypes.data), "add_gas_flowmap")
+
+    def add_gas_feed(self, kind, f0, param, inp, out, species, conc):
+        sp = np.ascontiguousarray(species, np.uint32)
+        cc = np.ascontiguousarray(conc, np.float64)
+        _check(lib().orc_add_gas_feed(self.h, FEED_TYPES[kind], f0, param, inp, -1 if out is None else out,
+                                      len(sp), sp, cc), "add_gas_feed")
+
+    def gas_concentrations(self):
+        return self._mat("orc_get_gas_concentrations")
+
+    def gas_mass(self):
+        return self._mat("orc_get_gas_mass")
 
     def set_probes(self, cap):
         """The oracle always records exit and division ages."""
