@@ -4,8 +4,9 @@ model plugins for it.
     python biocma-mcst_amd/build.py
     python biocma-mcst_amd/build.py --udf model.hpp --model Struct -o plugin.so
 
-The first builds the library, the C++ example driver and the example model
-plugins (examples/libudf_*.so); the second builds a plugin from a model
+The first builds the library, the C++ example drivers (examples/ring_driver,
+and examples/case_driver with the C++ case loader of host/) and the example
+model plugins (examples/libudf_*.so); the second builds a plugin from a model
 struct (the contract of csrc/models.hpp), for mcst_model_load.
 
 Flags: -ffp-contract=off keeps every FP64 expression rounded as written (the
@@ -86,14 +87,16 @@ def build(verbose: bool = True) -> str:
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stderr)
         os.replace(out, LIB)
-    # the C++ host driver example over the C ABI (host code only: g++)
-    ex_src = os.path.join(HERE, "examples", "ring_driver.cpp")
-    ex_out = os.path.join(HERE, "examples", "ring_driver")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"), ex_src,
-           "-o", ex_out, "-L", HERE, "-lmcst", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("example driver failed to build:\n" + r.stderr)
+    # the C++ host drivers over the C ABI (host code only: g++): the ring
+    # example, and the case driver with the C++ case loader (host/)
+    for name, extra in (("ring_driver", []), ("case_driver", [os.path.join(HERE, "host", "case_loader.cpp")])):
+        ex_src = os.path.join(HERE, "examples", name + ".cpp")
+        ex_out = os.path.join(HERE, "examples", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"), ex_src,
+               *extra, "-o", ex_out, "-L", HERE, "-lmcst", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example {name} failed to build:\n" + r.stderr)
     # the example model plugins
     for hdr, model in (("udf_minimal.hpp", "UdfMinimal"), ("udf_starvation.hpp", "UdfStarvation")):
         build_udf(os.path.join(HERE, "examples", hdr), model,
