@@ -2240,10 +2240,11 @@ int mcst_run_get_properties(mcst_engine* e, int32_t run, double* values, double*
   if (rc) return rc;
   const RunDev& d = e->h_runs[run];
   const uint64_t rows = (uint64_t)e->model->nv + 1, n = s.n;
-  const size_t vb = rows * n * sizeof(double), sb = rows * d.n_c * sizeof(double);
+  // the per-particle values take device memory only when they are asked for
+  const size_t vb = values ? rows * n * sizeof(double) : 0, sb = rows * d.n_c * sizeof(double);
   char* buf = nullptr;
   HIP_TRY(hipMalloc((void**)&buf, vb + sb + 8));
-  double* dv = (double*)buf;
+  double* dv = values ? (double*)buf : nullptr;
   double* ds = (double*)(buf + vb);
   hipError_t err = hipMemsetAsync(ds, 0, sb, e->stream);
   if (err == hipSuccess && n) {

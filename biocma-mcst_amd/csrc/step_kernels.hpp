@@ -1150,6 +1150,7 @@ __device__ void layout_pass(const BatchDev& b) {
 // Per-particle export values of run r (get_properties, post_process.hpp:54-98):
 // for every Idle slot the model state and M::mass, row-major [n_var+1][n]
 // (zero for inactive slots), and their sums per compartment [n_var+1][n_c].
+// values may be null: the sums only.
 template <class M>
 __global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint64_t n, double* values,
                                                     double* spatial) {
@@ -1159,7 +1160,8 @@ __global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint
   for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n; s += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t g = rd.p_off + s;
     if (b.status[g] != IDLE) {
-      for (int v = 0; v <= NV; ++v) values[(uint64_t)v * n + s] = 0.0;
+      if (values)
+        for (int v = 0; v <= NV; ++v) values[(uint64_t)v * n + s] = 0.0;
       continue;
     }
     const uint32_t c = b.pos[g];
@@ -1167,11 +1169,11 @@ __global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       x[v] = b.state[(uint64_t)v * b.P + g];
-      values[(uint64_t)v * n + s] = x[v];
+      if (values) values[(uint64_t)v * n + s] = x[v];
       unsafeAtomicAdd(spatial + (uint64_t)v * n_c + c, x[v]);
     }
     const double m = M::mass(rd.params, x);
-    values[(uint64_t)NV * n + s] = m;
+    if (values) values[(uint64_t)NV * n + s] = m;
     unsafeAtomicAdd(spatial + (uint64_t)NV * n_c + c, m);
   }
 }

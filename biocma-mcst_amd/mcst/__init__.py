@@ -447,15 +447,16 @@ class Run:
     def division_ages(self):
         return self._probe("mcst_run_get_division_ages")
 
-    def properties(self):
+    def properties(self, values=True):
         """(values [n_var+1][n], spatial [n_var+1][n_c]): each Idle particle's
-        state and mass, and their per-compartment sums (mcst_run_get_properties)."""
+        state and mass, and their per-compartment sums (mcst_run_get_properties).
+        values=False reads only the sums (values is then None)."""
         n = self.n
-        values = np.zeros((self.n_var + 1, n))
+        vals = np.zeros((self.n_var + 1, n)) if values else None
         spatial = np.zeros((self.n_var + 1, self.n_c))
-        _ck(lib().mcst_run_get_properties(self._h, self.id, values.ctypes.data if n else None,
+        _ck(lib().mcst_run_get_properties(self._h, self.id, vals.ctypes.data if values and n else None,
                                           spatial.ctypes.data), "get_properties")
-        return values, spatial
+        return vals, spatial
 
     def checkpoint(self):
         """Everything the next step needs, as bytes (mcst_run_checkpoint)."""

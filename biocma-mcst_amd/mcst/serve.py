@@ -163,19 +163,34 @@ class Server:
         return sorted(results, key=lambda r: r.rid)
 
 
-def run_alone(req, model=None, device=0):
-    """The same request on its own engine, stepped to completion; returns
-    (seconds, particle-steps)."""
-    t0 = time.perf_counter()
-    eng = Engine(model or req.case.model, device)
-    run = eng.add_run(req.case.n_c, req.case.dt, req.case.seed)
-    req.case.apply(run)
-    eng.commit()
-    eng.step(req.steps)
-    eng.synchronize()
-    c = run.counters()
-    eng.close()
-    return time.perf_counter() - t0, int(c["processed"])
+class OneAtATime:
+    """The same requests served one at a time: one engine, created and
+    committed beforehand with a slot pool (reserve/admit/retire, as the
+    Server's engines are), on which each request is defined, admitted,
+    stepped to its step count alone and retired before the next one starts.
+    Engine creation is outside the timing here as in Server.serve."""
+
+    def __init__(self, model, slots, max_runs, device=0):
+        self.eng = Engine(model, device)
+        self.eng.reserve(slots, max_runs)
+        self.eng.commit()
+
+    def close(self):
+        self.eng.close()
+
+    def run(self, req):
+        """One request on the idle engine; returns (seconds, particle-steps)."""
+        t0 = time.perf_counter()
+        run = self.eng.add_run(req.case.n_c, req.case.dt, req.case.seed)
+        req.case.apply(run)
+        run.set_step_target(req.steps)
+        if self.eng.admit() != 1:
+            raise RuntimeError(f"request {req.rid} does not fit the engine's pool")
+        self.eng.step(req.steps)
+        self.eng.synchronize()
+        c = run.counters()
+        self.eng.retire(run)
+        return time.perf_counter() - t0, int(c["processed"])
 
 
 def percentile(values, q):

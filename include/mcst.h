@@ -319,12 +319,12 @@ int mcst_run_get_repartition(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c
  * (MCST_ERR_ARG beyond).  Calls on one GPU share that array: calls that
  * overlap on one GPU (from several threads or engines at once) race. */
 int mcst_run_get_repartition_noalloc(mcst_engine* e, int32_t run, uint64_t* hist /* [n_c] */);
-/* Exit / division ages recorded so far (up to the probe capacity). */
 /* Export values (PostProcessing::get_properties, apps/core/public/core/post_process.hpp:54-98,173-190):
  * values[n_var+1][n]: each Idle slot's model state then Model::mass (zero
  * for inactive slots); spatial[n_var+1][n_c]: their sums per compartment
  * over Idle particles.  Either output may be NULL. */
 int mcst_run_get_properties(mcst_engine* e, int32_t run, double* values, double* spatial);
+/* Exit / division ages recorded so far (up to the probe capacity). */
 int mcst_run_get_exit_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n);
 int mcst_run_get_division_ages(mcst_engine* e, int32_t run, double* out, uint64_t max, uint64_t* n);
 
