@@ -1069,6 +1069,7 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_stage, T_max * b.nsub * DIV_STAGE * (nv + 1) * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * b.nsub * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * b.nsub * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_runs, (size_t)RC * sizeof(RunDev)))) return rc;
@@ -1245,6 +1246,7 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
       {(void**)&b.move_pos, P1 * 4, 0, 0, nullptr},
       {(void**)&b.div_slot, P1 * 2, 0, 0, nullptr},
       {(void**)&b.div_pos, P1 * 4, 0, 0, nullptr},
+      {(void**)&b.div_stage, T1 * b.nsub * DIV_STAGE * (nv + 1) * 8, 0, 0, nullptr},
       {(void**)&b.sub_moves, T1 * b.nsub * 4, 0, 0, nullptr},
       {(void**)&b.sub_divs, T1 * b.nsub * 4, 0, 0, nullptr},
   };

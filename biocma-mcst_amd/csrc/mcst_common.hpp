@@ -19,8 +19,9 @@ constexpr uint32_t TILE = BLOCK * PPT;     // 1024 slots per tile (the reference
 constexpr bool LANE_INTERLEAVE = true;     // particle pass: a lane's slots lie 64 apart (else consecutive)
 constexpr uint32_t NSUB_MAX = TILE / 128;  // sub-chunks (the pass's unit of work) per tile: TILE / (64 * M::NL)
 constexpr uint32_t WAVES = BLOCK / 64;
-constexpr uint32_t RESOLVE_GROUP_MAX = 64;  // sub-chunks per group of the resolve kernel: 32 or 64
+constexpr uint32_t RESOLVE_GROUP_MAX = 64;  // sub-chunks per group of the resolve kernel: 32 or 64 (at most 256)
 constexpr uint32_t RESOLVE_TICKETS = 64;    // sub-counters of the resolve kernel's done-count
+constexpr uint32_t DIV_STAGE = 16;          // division requests per sub-chunk divided in the particle pass
 constexpr int MAX_PARAMS = 16;
 constexpr int MAX_VAR = 16;
 constexpr int MAX_SPECIES = 8;
@@ -187,6 +188,9 @@ struct BatchDev {
   uint32_t* move_pos;       // [T_max*TILE] their compartment before the move
   uint16_t* div_slot;       // [T_max*TILE] tile-local slots of the division requests
   uint32_t* div_pos;        // [T_max*TILE] their compartment (the newborn's)
+  double* div_stage;        // [T_max*nsub*DIV_STAGE][n_var+1] the first DIV_STAGE requests of each
+                            // sub-chunk, divided in the pass: the mother's state before division
+                            // and her division age (to restore her if the request is refused)
   uint32_t* sub_moves;      // [T_max*nsub] entries of each sub-chunk's mover segment
   uint32_t* sub_divs;       // [T_max*nsub] entries of each sub-chunk's request segment
   // runs

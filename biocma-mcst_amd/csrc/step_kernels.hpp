@@ -680,8 +680,73 @@ __global__ void __launch_bounds__(BLK)
     double rn[NL][NQ];
     gather(nxt, B, rn);
 
+    // ---- ranks of the sub-chunk's movers and division requests in slot order
+    const uint64_t lt = lanemask_lt(lane);
+    const uint64_t seg = (uint64_t)sc * SUB;  // the sub-chunk's list segment
+    const uint32_t slot_t = cur.slot0 % TILE;  // tile-relative slot of the sub-chunk's first slot
+    uint32_t n_mov = 0, n_div = 0;
+    uint32_t rm[NL], rd_[NL];  // rank of slot off(j) among the flagged slots of the sub-chunk
+    {
+      uint64_t mv[NL], wd[NL];
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        mv[j] = __ballot(mover[j]);
+        wd[j] = __ballot(want[j]);
+      }
+      if constexpr (ILV) {
+        uint32_t am = 0, adv = 0;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          rm[j] = am + (uint32_t)__popcll(mv[j] & lt);
+          rd_[j] = adv + (uint32_t)__popcll(wd[j] & lt);
+          am += (uint32_t)__popcll(mv[j]);
+          adv += (uint32_t)__popcll(wd[j]);
+        }
+        n_mov = am;
+        n_div = adv;
+      } else {
+        uint32_t bm = 0, bd = 0;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          bm += (uint32_t)__popcll(mv[j] & lt);
+          bd += (uint32_t)__popcll(wd[j] & lt);
+          n_mov += (uint32_t)__popcll(mv[j]);
+          n_div += (uint32_t)__popcll(wd[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+          rm[j] = bm;
+          rd_[j] = bd;
+          bm += mover[j] ? 1u : 0u;
+          bd += want[j] ? 1u : 0u;
+        }
+      }
+    }
+    // ---- divisions in registers (rare): the first DIV_STAGE requests of the
+    // sub-chunk are divided here, where the mother's state is at hand.  Her
+    // state before division and her division age are staged, one contiguous
+    // record per request; the halved mother and her reset division age are
+    // stored below with everything else.  k_resolve decides acceptance: an
+    // accepted request writes the daughter (M::division of the staged state
+    // again: the same draws, so the same daughter), a refused one restores
+    // the mother from the stage.  Later requests of a crowded sub-chunk are
+    // divided by k_resolve from the stored state.
+    if (n_div) {
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        if (!(want[j] && rd_[j] < DIV_STAGE)) continue;
+        GLOBAL_AS double* stg = gp(b.div_stage) + ((uint64_t)sc * DIV_STAGE + rd_[j]) * (NV + 1);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) stg[v] = x[j][v];
+        stg[NV] = ad[j];
+        double xd[NV];
+        M::division(p, x[j], xd, RngKey{seed, cur.slot0 + off(j), step});
+        ad[j] = 0.0;
+      }
+    }
+
     // ---- stores: the lane's final state, status changes, and the
-    // sub-chunk's movers and division requests in slot order
+    // sub-chunk's movers and division requests
     const uint64_t gi = cur.base + i0;
     if constexpr (ILV) {
 #pragma unroll
@@ -714,58 +779,16 @@ __global__ void __launch_bounds__(BLK)
       }
       if (changed && pair_valid) st_status<NL>(gp(b.status) + gi, stn);
     }
-    const uint64_t lt = lanemask_lt(lane);
-    const uint64_t seg = (uint64_t)sc * SUB;  // the sub-chunk's list segment
-    const uint32_t slot_t = cur.slot0 % TILE;  // tile-relative slot of the sub-chunk's first slot
-    uint32_t n_mov = 0, n_div = 0;
-    {
-      uint64_t mv[NL], wd[NL];
 #pragma unroll
-      for (int j = 0; j < NL; ++j) {
-        mv[j] = __ballot(mover[j]);
-        wd[j] = __ballot(want[j]);
+    for (int j = 0; j < NL; ++j) {
+      if (mover[j]) {
+        const bool leaves = idle[j] && !dead[j];
+        gp(b.move_slot)[seg + rm[j]] = (uint16_t)((slot_t + off(j)) | (leaves ? 0u : (uint32_t)MOVER_INACTIVE));
+        gp(b.move_pos)[seg + rm[j]] = A.pos[j];
       }
-      // rank of slot off(j) among the flagged slots of the sub-chunk
-      uint32_t rm[NL], rd_[NL];
-      if constexpr (ILV) {
-        uint32_t am = 0, adv = 0;
-#pragma unroll
-        for (int j = 0; j < NL; ++j) {
-          rm[j] = am + (uint32_t)__popcll(mv[j] & lt);
-          rd_[j] = adv + (uint32_t)__popcll(wd[j] & lt);
-          am += (uint32_t)__popcll(mv[j]);
-          adv += (uint32_t)__popcll(wd[j]);
-        }
-        n_mov = am;
-        n_div = adv;
-      } else {
-        uint32_t bm = 0, bd = 0;
-#pragma unroll
-        for (int j = 0; j < NL; ++j) {
-          bm += (uint32_t)__popcll(mv[j] & lt);
-          bd += (uint32_t)__popcll(wd[j] & lt);
-          n_mov += (uint32_t)__popcll(mv[j]);
-          n_div += (uint32_t)__popcll(wd[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < NL; ++j) {
-          rm[j] = bm;
-          rd_[j] = bd;
-          bm += mover[j] ? 1u : 0u;
-          bd += want[j] ? 1u : 0u;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < NL; ++j) {
-        if (mover[j]) {
-          const bool leaves = idle[j] && !dead[j];
-          gp(b.move_slot)[seg + rm[j]] = (uint16_t)((slot_t + off(j)) | (leaves ? 0u : (uint32_t)MOVER_INACTIVE));
-          gp(b.move_pos)[seg + rm[j]] = A.pos[j];
-        }
-        if (n_div && want[j]) {  // rare
-          gp(b.div_slot)[seg + rd_[j]] = (uint16_t)(slot_t + off(j));
-          gp(b.div_pos)[seg + rd_[j]] = A.pos[j];
-        }
+      if (n_div && want[j]) {  // rare
+        gp(b.div_slot)[seg + rd_[j]] = (uint16_t)(slot_t + off(j));
+        gp(b.div_pos)[seg + rd_[j]] = A.pos[j];
       }
     }
     if (lane == 0 && sc < n_sc) {
@@ -912,7 +935,7 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
 // births, the removal decision (update_and_remove_inactive,
 // particles_container.hpp:539-557) and the removal's tile layout.
 constexpr uint32_t RES_SC = RESOLVE_GROUP_MAX;  // shared arrays; a group is GS <= RES_SC sub-chunks
-static_assert(RES_SC <= 64 && 32 % NSUB_MAX == 0, "a group is at most one wave of whole tiles");
+static_assert(RES_SC <= 256 && 32 % NSUB_MAX == 0, "a group is at most one sub-chunk per thread, of whole tiles");
 
 // Requests of the pass in sub-chunks [0, sc), summed by one wave from the
 // block counts (div_l3, div_l2, div_l1: blocks of 2^18, 2^12 and 2^6
@@ -938,6 +961,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   __shared__ uint32_t s_run[RES_SC];
   __shared__ uint32_t s_dpre[RES_SC + 1];  // division requests: prefix over the group's sub-chunks
   __shared__ uint32_t s_p0, s_pr0;         // requests before the group, before its first run
+  __shared__ uint32_t s_wm[4], s_wd[4];    // movers and requests of each wave's sub-chunks
   __shared__ uint32_t s_last;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint32_t R = b.R;
@@ -946,31 +970,48 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   const uint32_t T = tile_off[R];
   const uint32_t n_sc = T * nsub;
   // groups of 32 sub-chunks while they fit the grid (one group per block:
-  // shorter chains), else 64 (fewer, fuller groups per block); lanes
-  // GS..63 of the first wave stay idle
-  const uint32_t GS = (n_sc + 31) / 32 <= gridDim.x ? 32u : RES_SC;
+  // shorter chains); with more sub-chunks, groups twice as large as long as
+  // they still fill the grid, up to RES_SC: a mover takes a chain of
+  // dependent loads, and a larger group keeps more of them in flight per
+  // block, but one block works through all requests of its group, and where
+  // newborns divide together (the end of the run) a large group's requests
+  // become a long serial chain.  On the 50 000-compartment lattice at 1e8
+  // particles groups of 64, 128 and 256 took 225, 227 and 266 us per step on
+  // average.  Threads GS..255 hold no sub-chunk.
+  uint32_t GS = 32;
+  while (GS < RES_SC && (n_sc + 2 * GS - 1) / (2 * GS) >= gridDim.x) GS *= 2;
   const uint32_t n_groups = (n_sc + GS - 1) / GS;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
   for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
     const uint32_t sc0 = grp * GS;
-    if (tid < 64) {  // the group's sub-chunks, one lane each
-      const uint32_t sc = sc0 + lane;
+    {  // the group's sub-chunks, one thread each: block-wide prefix sums
+      const uint32_t sc = sc0 + tid;
       uint32_t cm = 0, cd = 0, r = 0;
-      if (lane < GS && sc < n_sc) {
+      if (tid < GS && sc < n_sc) {
         cm = b.sub_moves[sc];
         cd = b.sub_divs[sc];
         r = find_run(tile_off, R, sc / nsub);
       }
       const uint32_t inc = wave_incl_scan(cm, lane);
       const uint32_t incd = wave_incl_scan(cd, lane);
-      if (lane < GS) {
-        s_pre[lane] = inc - cm;
-        s_dpre[lane] = incd - cd;
-        s_run[lane] = r;
+      if (lane == 63) {
+        s_wm[wave] = inc;
+        s_wd[wave] = incd;
       }
-      if (lane == GS - 1) {
-        s_pre[GS] = inc;
-        s_dpre[GS] = incd;
+      __syncthreads();
+      uint32_t om = 0, od = 0;  // the earlier waves' totals
+      for (uint32_t w = 0; w < wave; ++w) {
+        om += s_wm[w];
+        od += s_wd[w];
+      }
+      if (tid < GS) {
+        s_pre[tid] = om + inc - cm;
+        s_dpre[tid] = od + incd - cd;
+        s_run[tid] = r;
+      }
+      if (tid == GS - 1) {
+        s_pre[GS] = om + inc;
+        s_dpre[GS] = od + incd;
       }
     }
     __syncthreads();
@@ -1000,6 +1041,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
       pos = gp(mp.nbr)[(size_t)pos * m + col];
       b.pos[g] = pos;
+      bool exits = false;
       if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {
         const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + record_terms<M>() + 1];
         if (qx < 1.0 && u01(u.z) > qx) {
@@ -1010,9 +1052,18 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
           }
           b.age_hyd[g] = 0.0;
           b.status[g] = EXIT;
-          atomicAdd(&rs->step_exits, 1ull);
+          exits = true;
         }
       }
+      // the exit counter takes one atomic per wave for the run of the wave's
+      // first active lane (the wave's movers are in slot order: nearly always
+      // one run), and one per exit of any other run; an atomic per exit
+      // queued tens of thousands of them on one address per step
+      const uint32_t r_lead = __builtin_amdgcn_readfirstlane(r);
+      const uint64_t ex_lead = __ballot(exits && r == r_lead);
+      if (exits && r != r_lead) atomicAdd(&rs->step_exits, 1ull);
+      if (ex_lead && lane == (uint32_t)__ffsll((long long)ex_lead) - 1u)
+        atomicAdd(&b.rs[r_lead].step_exits, (unsigned long long)__popcll(ex_lead));
     }
     // ---- divisions: every request of the group at once, one thread each
     const uint32_t n_req = s_dpre[GS];
@@ -1044,28 +1095,47 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         const uint32_t k = first <= sc0 ? s_p0 - s_pr0 + f : f - s_dpre[first - sc0];
         const uint32_t n = rs->n;
         const uint32_t room = rd.cap - n;
-        if (k >= (room < rd.div_cap ? room : rd.div_cap)) continue;  // refused: Overflow
-        const uint64_t q = (uint64_t)sc * sub + (f - s_dpre[j]);
+        const bool accepted = k < (room < rd.div_cap ? room : rd.div_cap);  // else refused: Overflow
+        const uint32_t rk = f - s_dpre[j];  // rank of the request in its sub-chunk
+        if (rk >= DIV_STAGE && !accepted) continue;
+        const uint64_t q = (uint64_t)sc * sub + rk;
         const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
         const uint64_t g = rd.p_off + slot;
         double p[NP];
 #pragma unroll
         for (int i = 0; i < NP; ++i) p[i] = rd.params[i];
-        double x[NV], xd[NV];
+        double x[NV], xd[NV], age;
+        if (rk < DIV_STAGE) {
+          // divided by the pass: the mother's slot holds her halved state; the
+          // stage her state before division and her division age
+          const double* stg = b.div_stage + ((uint64_t)sc * DIV_STAGE + rk) * (NV + 1);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
+          for (int v = 0; v < NV; ++v) x[v] = stg[v];
+          age = stg[NV];
+          if (!accepted) {  // the mother as she was before the pass divided her
+#pragma unroll
+            for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * b.P + g] = x[v];
+            b.age_div[g] = age;
+            continue;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
+          age = b.age_div[g];
+        }
         if (rd.probe_div) {
           const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
-          if (i < rd.probe_cap) rd.probe_div[i] = b.age_div[g];
+          if (i < rd.probe_cap) rd.probe_div[i] = age;
         }
         M::division(p, x, xd, RngKey{rd.seed, slot, rs->step});
         const uint64_t d = rd.p_off + n + k;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          b.state[(uint64_t)v * b.P + g] = x[v];
-          b.state[(uint64_t)v * b.P + d] = xd[v];
+        for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * b.P + d] = xd[v];
+        if (rk >= DIV_STAGE) {  // the mother, divided here
+#pragma unroll
+          for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * b.P + g] = x[v];
+          b.age_div[g] = 0.0;
         }
-        b.age_div[g] = 0.0;
         b.pos[d] = b.div_pos[q];
         b.status[d] = IDLE;
         b.age_hyd[d] = 0.0;
