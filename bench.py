@@ -357,7 +357,9 @@ def main():
     case.capacity = 2 * a.particles
     log(f"initialising {a.particles:.0e} particles")
     case.apply(run)
-    eng.commit()
+    t_c = time.perf_counter()
+    eng.commit()  # device allocation and the one host-to-device upload of particles, flowmaps and liquid
+    t_commit = time.perf_counter() - t_c
     _, nv, ns, _ = mcst.model_info(case.model)
     # FP64 fields a step reads and writes per particle: the model state, the
     # division age, and the hydraulic age only when a leaving flow exists
@@ -397,6 +399,15 @@ def main():
     ps = c1["processed"] - c0["processed"]
     value = ps / elapsed
     log(f"{ps:.3e} particle-steps in {elapsed:.3f} s -> {value:.3e}/s")
+    # the host-buffer view: the same steps with the upload at commit and one
+    # read-back of every particle field counted in (never the value)
+    t_r = time.perf_counter()
+    run.particles()
+    t_readback = time.perf_counter() - t_r
+    pcie = {"value": ps / (elapsed + t_commit + t_readback), "unit": "particle-steps/s",
+            "commit_s": t_commit, "readback_s": t_readback, "steps": steps}
+    log(f"with the commit upload ({t_commit:.3f} s) and a read-back of the particles ({t_readback:.3f} s): "
+        f"{pcie['value']:.3e}/s")
 
     log(f"profiling {a.profile_steps} steps with HIP events")
     p0 = run.counters()["processed"]
@@ -458,6 +469,9 @@ def main():
         # events of the timed steps: every division request is accepted while
         # the run has free slots (overflow counts refusals)
         "events": {k: int(c1[k] - c0[k]) for k in ("births", "overflow", "moves", "exits")},
+        # the timed steps' particle-steps over their time plus the commit
+        # (the one upload of the host buffers) and one read-back of the particles
+        "host_buffers_inclusive": pcie,
     }
     if ensemble is not None:
         out["ensemble"] = ensemble

@@ -191,6 +191,34 @@ def test_move_probability_is_linear(oracle_lib):
         assert abs(moved - n_i * p) < 5 * math.sqrt(n_i * p * (1 - p))
 
 
+def test_inactive_slots_keep_moving(oracle_lib):
+    """move_kernel.hpp:209-273,393-434: the move stage runs over every slot
+    below n whatever its status (Exit and Dead slots waiting for removal keep
+    moving) and counts a Move event for each.  Half the slots Exit, removal
+    never due: both halves move with probability dt*Q/V, and the counter
+    holds every move.  A restatement that moved only Idle particles would
+    leave the Exit half in place and count half the moves."""
+    n = 200000
+    r = oracle_lib.OracleRun("monod_mass", 2, 1, 0.01, 5)
+    r.add_flowmap(np.array([1e-3, 2e-3]), np.array([0, 1], np.uint32), np.array([1, 0], np.uint32),
+                  np.array([3e-3, 1e-3]))
+    r.set_concentrations(np.ones((1, 2)))
+    st = np.zeros(n, np.uint8)
+    st[1::2] = 2  # Exit
+    m = np.full(n, 1e-15)
+    r.set_particles(n, np.zeros(n, np.uint32), st, np.stack([m, m]), np.zeros(n), np.zeros(n), 1.0)
+    r.set_compaction(1e9, 10**9)
+    r.step_stages(MOVE)
+    p = r.particles()
+    moved = p["pos"] != 0
+    pm = 0.01 * 3e-3 / 1e-3
+    for sel in (st == 0, st == 2):
+        k, x = int(sel.sum()), int((moved & sel).sum())
+        assert abs(x - k * pm) < 5 * math.sqrt(k * pm * (1 - pm)), (k, x)
+    assert r.counters()["moves"] == int(moved.sum())
+    np.testing.assert_array_equal(p["status"], st)
+
+
 def test_neighbour_choice_follows_flows(oracle_lib):
     n = 4
     rows = np.array([0, 0, 0, 1, 2, 3], np.uint32)
