@@ -467,6 +467,7 @@ struct mcst_engine {
   int tuned_group = -1;
   int grid_move = 1;
   bool removal_possible = true;
+  bool any_leave = true;            // some run on the device has a leaving flow (the pass's leave variant)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint32_t graph_steps = 0;
@@ -691,7 +692,7 @@ int choose_launch(mcst_engine* e) {
   for (int blk : {256, 512, 1024}) {
     if (!priv && blk != 256) continue;
     if (forced_block && blk != forced_block) continue;
-    const void* fn = M.ops->particle_kernel(blk, priv, false, priv && windowed);
+    const void* fn = M.ops->particle_kernel(blk, priv, false, priv && windowed, e->any_leave);
     if (!fn) continue;
     const size_t lds = priv ? lds_need : 0;
     if (lds > 64 * 1024)
@@ -718,13 +719,15 @@ int choose_launch(mcst_engine* e) {
     best = ParticleLaunch{};
     b.acc_w = 0;
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, M.ops->particle_kernel(256, false, false, false), 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, M.ops->particle_kernel(256, false, false, false, e->any_leave), 256, 0));
     best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
   }
   best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, b.T_max));
   best.probe = probe;
+  best.leave = e->any_leave;
   if (best.probe && best.lds > 64 * 1024)
-    (void)hipFuncSetAttribute(M.ops->particle_kernel(256, best.priv, true, best.win),
+    (void)hipFuncSetAttribute(M.ops->particle_kernel(256, best.priv, true, best.win, true),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)best.lds);
   // one run whose accumulator fits: blocks take groups in turn, which keeps
   // the wave front of the whole grid inside a narrow window of the particle
@@ -746,8 +749,8 @@ int choose_launch(mcst_engine* e) {
   const uint64_t n_sc = (uint64_t)b.T_max * b.nsub;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>(resident, (n_sc + 31) / 32));
   if (std::getenv("MCST_VERBOSE"))
-    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u (lds cap %zu)\n",
-                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, lds_cap);
+    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u leave=%d (lds cap %zu)\n",
+                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, (int)best.leave, lds_cap);
   return MCST_OK;
 }
 
@@ -777,9 +780,10 @@ int refresh_tables(mcst_engine* e) {
   b.R = R;
   b.n_comp_total = n_comp;
   b.maps = e->d_maps;
-  e->removal_possible = e->model->can_die;
+  e->any_leave = false;
   for (uint32_t r = 0; r < R; ++r)
-    if (e->runs[r].phase == 1 && e->h_runs[r].enable_leave) e->removal_possible = true;
+    if (e->runs[r].phase == 1 && e->h_runs[r].enable_leave) e->any_leave = true;
+  e->removal_possible = e->model->can_die || e->any_leave;
   return choose_launch(e);
 }
 

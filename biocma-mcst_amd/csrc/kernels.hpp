@@ -10,14 +10,18 @@ namespace mcst {
 
 // Launch shape of the particle pass: block size (256/512/1024 threads per
 // 1024-slot tile), grid, the LDS bytes of the per-block contribution
-// accumulator (priv) or none (direct global atomics), and whether some run
-// records exit and division ages (probe: a 256-thread variant).
+// accumulator (priv) or none (direct global atomics), whether some run
+// records exit and division ages (probe: a 256-thread variant), and whether
+// some run has a leaving flow (leave: the variant that reads and writes
+// hydraulic ages and exit thresholds; without one the pass issues none of
+// those loads).
 struct ParticleLaunch {
   int block = 256;
   int grid = 1;
   bool priv = false;
   bool probe = false;
   bool win = false;  // the accumulator holds a window of compartments, not whole runs
+  bool leave = true;
   size_t lds = 0;
 };
 
@@ -29,7 +33,7 @@ struct ParticleLaunch {
 struct ModelOps {
   void (*transport)(const BatchDev& b, int grid, hipStream_t s);
   void (*particle)(const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
-  const void* (*particle_kernel)(int block, bool priv, bool probe, bool win);
+  const void* (*particle_kernel)(int block, bool priv, bool probe, bool win, bool leave);
   void (*resolve)(const BatchDev& b, int grid, hipStream_t s);
   const void* (*resolve_kernel)();  // for occupancy queries
   void (*properties)(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s);

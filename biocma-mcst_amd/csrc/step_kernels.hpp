@@ -385,12 +385,19 @@ __device__ __forceinline__ void st_status(GLOBAL_AS uint8_t* p, uint32_t v) {
 // Every load is issued for every lane (slots past the count read valid
 // memory of the tile and are masked where used), so the waits the compiler
 // places for a load's operands do not depend on the data.
-template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE, bool WIN>
+//
+// LV: some run of the batch has a leaving flow.  Without one (LV false) the
+// pass issues no load or store of hydraulic ages and gathers no exit
+// threshold: the loads a run without an outlet would issue anyway, to a
+// single address, took 3 % of the pass on the 1e7-particle network, and a
+// run-time branch around them cost more than it saved.
+template <class M, int BLK, int NL, bool ILV, bool PRIV, bool PROBE, bool WIN, bool LV>
 __global__ void __launch_bounds__(BLK)
     k_particle(BatchDev b, const RunDev* __restrict__ runs, const RunState* __restrict__ rsr,
                const uint32_t* __restrict__ tile_off) {
   constexpr int NV = M::NV, NS = M::NS, NR = record_terms<M>(), REC = rec_stride(NR);
   constexpr int NQ = NR + 2;             // record fields: M::record terms r_0..r_{NR-1}, p_move, q_exit
+  constexpr int NG = LV ? NQ : NQ - 1;   // fields gathered (q_exit only when some run leaves)
   constexpr int NP = M::NP > 0 ? M::NP : 1;
   constexpr uint32_t WPB = BLK / 64;     // waves per block = sub-chunks per group
   constexpr uint32_t SUB = 64 * NL;       // slots per sub-chunk
@@ -451,7 +458,7 @@ __global__ void __launch_bounds__(BLK)
     double ad[NL], ah[NL];
   };
   auto load_sub = [&](const SubRef& q, Sub& a) {
-    const bool leave = runs[q.r].enable_leave && (stages & STAGE_LEAVE);
+    const bool leave = LV && runs[q.r].enable_leave && (stages & STAGE_LEAVE);
     if constexpr (ILV) {
       uint32_t st = 0;
 #pragma unroll
@@ -462,7 +469,10 @@ __global__ void __launch_bounds__(BLK)
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.x[v][j] = ld_stream(gp(b.state) + (uint64_t)v * P + ij);
         a.ad[j] = ld_stream(gp(b.age_div) + ij);
-        a.ah[j] = ld_stream(gp(b.age_hyd) + (leave ? ij : q.base));
+        if constexpr (LV)
+          a.ah[j] = ld_stream(gp(b.age_hyd) + (leave ? ij : q.base));
+        else
+          a.ah[j] = 0.0;
       }
       a.st = st;
     } else {
@@ -472,7 +482,12 @@ __global__ void __launch_bounds__(BLK)
 #pragma unroll
       for (int v = 0; v < NV; ++v) ld_lane<NL>(gp(b.state) + (uint64_t)v * P + i0, a.x[v]);
       ld_lane<NL>(gp(b.age_div) + i0, a.ad);
-      ld_lane<NL>(gp(b.age_hyd) + (leave ? i0 : q.base), a.ah);
+      if constexpr (LV) {
+        ld_lane<NL>(gp(b.age_hyd) + (leave ? i0 : q.base), a.ah);
+      } else {
+#pragma unroll
+        for (int j = 0; j < NL; ++j) a.ah[j] = 0.0;
+      }
     }
   };
   auto gather = [&](const SubRef& q, const Sub& a, double (&rr)[NL][NQ]) {
@@ -483,7 +498,8 @@ __global__ void __launch_bounds__(BLK)
       const uint32_t pj = off(j) < q.cnt ? a.pos[j] : 0u;
       const GLOBAL_AS double* rp = rec + (size_t)pj * REC;
 #pragma unroll
-      for (int f = 0; f < NQ; ++f) rr[j][f] = rp[f];
+      for (int f = 0; f < NG; ++f) rr[j][f] = rp[f];
+      if constexpr (!LV) rr[j][NQ - 1] = 1.0;  // no exit test without a leaving flow
     }
   };
 
@@ -591,7 +607,7 @@ __global__ void __launch_bounds__(BLK)
 #pragma unroll
     for (int q = 0; q < NP; ++q) p[q] = rd.params[q];
     const bool do_move = rd.enable_move && (stages & STAGE_MOVE);
-    const bool do_leave = rd.enable_leave && (stages & STAGE_LEAVE);
+    const bool do_leave = LV && rd.enable_leave && (stages & STAGE_LEAVE);
     const uint32_t g_next = next_group(grp);
 
     // ---- prefetch: the wave's sub-chunk of the next group (a phantom past
@@ -1249,60 +1265,73 @@ __global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint
 }
 
 // ------------------------------------------------------------------ launchers
-template <class M, int BLK, bool PRIV, bool PROBE, bool WIN>
+template <class M, int BLK, bool PRIV, bool PROBE, bool WIN, bool LV>
 static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_particle<M, BLK, M::NL, LANE_INTERLEAVE, PRIV, PROBE, WIN>), dim3(grid), dim3(BLK), lds, s, b,
-                     b.runs, (const RunState*)b.rs, (const uint32_t*)b.tile_off);
+  hipLaunchKernelGGL((k_particle<M, BLK, M::NL, LANE_INTERLEAVE, PRIV, PROBE, WIN, LV>), dim3(grid), dim3(BLK), lds, s,
+                     b, b.runs, (const RunState*)b.rs, (const uint32_t*)b.tile_off);
 }
 
 // The variants: direct global atomics (256 threads); an LDS accumulator of
-// whole runs (256/512/1024) or of a compartment window (256/512); and the
-// 256-thread probe variants of each accumulator kind.
-template <class M>
-static const void* kernel_ptr_t(int blk, bool priv, bool probe, bool win) {
-  if (probe) {
-    if (!priv) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, true, false>;
-    return win ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, true>
-               : (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, false>;
-  }
-  if (!priv) return blk == 256 ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, false, false> : nullptr;
+// whole runs (256/512/1024) or of a compartment window (256/512); each with
+// and without the leaving-flow work; and the 256-thread probe variants of
+// each accumulator kind (with it).
+template <class M, bool LV>
+static const void* kernel_ptr_lv(int blk, bool priv, bool win) {
+  if (!priv) return blk == 256 ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, false, false, LV> : nullptr;
   if (win) {
-    if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, true>;
-    if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, true>;
+    if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, true, LV>;
+    if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, true, LV>;
     return nullptr;
   }
-  if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, false>;
-  if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, false>;
-  if (blk == 1024) return (const void*)&k_particle<M, 1024, M::NL, LANE_INTERLEAVE, true, false, false>;
+  if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, false, LV>;
+  if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, false, LV>;
+  if (blk == 1024) return (const void*)&k_particle<M, 1024, M::NL, LANE_INTERLEAVE, true, false, false, LV>;
   return nullptr;
+}
+template <class M>
+static const void* kernel_ptr_t(int blk, bool priv, bool probe, bool win, bool leave) {
+  if (probe) {
+    if (!priv) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, true, false, true>;
+    return win ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, true, true>
+               : (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, false, true>;
+  }
+  return leave ? kernel_ptr_lv<M, true>(blk, priv, win) : kernel_ptr_lv<M, false>(blk, priv, win);
 }
 
 // Runs with probes take a 256-thread variant that records exit and division
 // ages; the atomics it needs (with returned values) are kept out of the
 // production variants, where they would make every wait for a load in the
 // tile loop a wait for all outstanding memory operations.
+template <class M, bool LV>
+static void launch_lv(const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
+  if (!L.priv) {
+    launch_particle_t<M, 256, false, false, false, LV>(b, L.grid, 0, s);
+  } else if (L.win) {
+    if (L.block == 512)
+      launch_particle_t<M, 512, true, false, true, LV>(b, L.grid, L.lds, s);
+    else
+      launch_particle_t<M, 256, true, false, true, LV>(b, L.grid, L.lds, s);
+  } else if (L.block == 256) {
+    launch_particle_t<M, 256, true, false, false, LV>(b, L.grid, L.lds, s);
+  } else if (L.block == 512) {
+    launch_particle_t<M, 512, true, false, false, LV>(b, L.grid, L.lds, s);
+  } else {
+    launch_particle_t<M, 1024, true, false, false, LV>(b, L.grid, L.lds, s);
+  }
+}
 template <class M>
 static void launch_t(const BatchDev& b, const ParticleLaunch& L, hipStream_t s) {
   if (L.probe) {
     if (!L.priv)
-      launch_particle_t<M, 256, false, true, false>(b, L.grid, 0, s);
+      launch_particle_t<M, 256, false, true, false, true>(b, L.grid, 0, s);
     else if (L.win)
-      launch_particle_t<M, 256, true, true, true>(b, L.grid, L.lds, s);
+      launch_particle_t<M, 256, true, true, true, true>(b, L.grid, L.lds, s);
     else
-      launch_particle_t<M, 256, true, true, false>(b, L.grid, L.lds, s);
-  } else if (!L.priv) {
-    launch_particle_t<M, 256, false, false, false>(b, L.grid, 0, s);
-  } else if (L.win) {
-    if (L.block == 512)
-      launch_particle_t<M, 512, true, false, true>(b, L.grid, L.lds, s);
-    else
-      launch_particle_t<M, 256, true, false, true>(b, L.grid, L.lds, s);
-  } else if (L.block == 256) {
-    launch_particle_t<M, 256, true, false, false>(b, L.grid, L.lds, s);
-  } else if (L.block == 512) {
-    launch_particle_t<M, 512, true, false, false>(b, L.grid, L.lds, s);
+      launch_particle_t<M, 256, true, true, false, true>(b, L.grid, L.lds, s);
+  } else if (L.leave) {
+    launch_lv<M, true>(b, L, s);
   } else {
-    launch_particle_t<M, 1024, true, false, false>(b, L.grid, L.lds, s);
+    launch_lv<M, false>(b, L, s);
   }
 }
 
