@@ -16,7 +16,12 @@ namespace mcst {
 constexpr uint32_t BLOCK = 256;            // threads per particle block (4 wave64)
 constexpr uint32_t PPT = 4;                // particles per thread per tile
 constexpr uint32_t TILE = BLOCK * PPT;     // 1024 slots per tile (the reference's 1024 particles/team)
-constexpr bool LANE_INTERLEAVE = true;     // particle pass: a lane's slots lie 64 apart (else consecutive)
+// Particle pass layout: with 4 slots per lane a lane's slots lie 64 apart
+// (every access of a wave one contiguous 64-slot run); with 2 they are
+// consecutive (16-byte accesses per lane).  Measured on MI355X: 64 apart is
+// 3 % faster at 4 slots per lane (pair-wise 16-byte accesses, half the load
+// instructions, were slower), consecutive 4 % faster at 2.
+constexpr bool lane_interleave(int nl) { return nl >= 4; }
 constexpr uint32_t NSUB_MAX = TILE / 128;  // sub-chunks (the pass's unit of work) per tile: TILE / (64 * M::NL)
 constexpr uint32_t WAVES = BLOCK / 64;
 constexpr uint32_t RESOLVE_GROUP_MAX = 64;  // sub-chunks per group of the resolve kernel: 32 or 64 (at most 256)

@@ -1267,7 +1267,7 @@ __global__ void __launch_bounds__(256) k_properties(BatchDev b, uint32_t r, uint
 // ------------------------------------------------------------------ launchers
 template <class M, int BLK, bool PRIV, bool PROBE, bool WIN, bool LV>
 static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL((k_particle<M, BLK, M::NL, LANE_INTERLEAVE, PRIV, PROBE, WIN, LV>), dim3(grid), dim3(BLK), lds, s,
+  hipLaunchKernelGGL((k_particle<M, BLK, M::NL, lane_interleave(M::NL), PRIV, PROBE, WIN, LV>), dim3(grid), dim3(BLK), lds, s,
                      b, b.runs, (const RunState*)b.rs, (const uint32_t*)b.tile_off);
 }
 
@@ -1277,23 +1277,23 @@ static void launch_particle_t(const BatchDev& b, int grid, size_t lds, hipStream
 // each accumulator kind (with it).
 template <class M, bool LV>
 static const void* kernel_ptr_lv(int blk, bool priv, bool win) {
-  if (!priv) return blk == 256 ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, false, false, LV> : nullptr;
+  if (!priv) return blk == 256 ? (const void*)&k_particle<M, 256, M::NL, lane_interleave(M::NL), false, false, false, LV> : nullptr;
   if (win) {
-    if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, true, LV>;
-    if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, true, LV>;
+    if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, lane_interleave(M::NL), true, false, true, LV>;
+    if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, lane_interleave(M::NL), true, false, true, LV>;
     return nullptr;
   }
-  if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, false, false, LV>;
-  if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, LANE_INTERLEAVE, true, false, false, LV>;
-  if (blk == 1024) return (const void*)&k_particle<M, 1024, M::NL, LANE_INTERLEAVE, true, false, false, LV>;
+  if (blk == 256) return (const void*)&k_particle<M, 256, M::NL, lane_interleave(M::NL), true, false, false, LV>;
+  if (blk == 512) return (const void*)&k_particle<M, 512, M::NL, lane_interleave(M::NL), true, false, false, LV>;
+  if (blk == 1024) return (const void*)&k_particle<M, 1024, M::NL, lane_interleave(M::NL), true, false, false, LV>;
   return nullptr;
 }
 template <class M>
 static const void* kernel_ptr_t(int blk, bool priv, bool probe, bool win, bool leave) {
   if (probe) {
-    if (!priv) return (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, false, true, false, true>;
-    return win ? (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, true, true>
-               : (const void*)&k_particle<M, 256, M::NL, LANE_INTERLEAVE, true, true, false, true>;
+    if (!priv) return (const void*)&k_particle<M, 256, M::NL, lane_interleave(M::NL), false, true, false, true>;
+    return win ? (const void*)&k_particle<M, 256, M::NL, lane_interleave(M::NL), true, true, true, true>
+               : (const void*)&k_particle<M, 256, M::NL, lane_interleave(M::NL), true, true, false, true>;
   }
   return leave ? kernel_ptr_lv<M, true>(blk, priv, win) : kernel_ptr_lv<M, false>(blk, priv, win);
 }
