@@ -838,7 +838,7 @@ int upload_run(mcst_engine* e, uint32_t r) {
   std::vector<double> F, sink;
   feed_tables(h, F, sink);
   const uint32_t rw = (uint32_t)rec_stride(e->model->nr);
-  std::vector<double> rec((size_t)h.n_c * rw, 0.0);
+  std::vector<double> rec((size_t)2 * h.n_c * rw, 0.0);  // two tables, by step parity
   std::vector<double> zeros(nsc, 0.0);
   if ((rc = run_upload(h, &d.C[0], C0.data(), nsc))) return rc;
   if ((rc = run_upload(h, &d.C[1], C0.data(), nsc))) return rc;
@@ -1137,7 +1137,11 @@ int retire(mcst_engine* e, uint32_t r) {
 // The kernels of one step of the batch.  With `ev`, records ev[0] before
 // the transport and ev[k] after the k-th stage (transport, particle pass,
 // resolve with the bookkeeping, removal) for mcst_engine_profile.
-void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr) {
+// `transported`: the step's transport (and tile layout) already ran, inside
+// the previous step's resolve kernel.  `ahead`: run the next step's transport
+// inside this step's resolve kernel.
+void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr, bool transported = false,
+                 bool ahead = false) {
   if (e->bd.R == 0) return;  // no run admitted yet
   BatchDev b = e->bd;
   b.stages = stages;
@@ -1147,15 +1151,21 @@ void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr
     if (ev) (void)hipEventRecord(ev[k], e->stream);
   };
   mark(0);
-  ops.transport(b, e->grid_transport, e->stream);
+  if (!transported) ops.transport(b, e->grid_transport, e->stream);
   mark(1);
   ops.particle(b, e->plaunch, e->stream);
   mark(2);
-  ops.resolve(b, e->grid_move, e->stream);
+  ops.resolve(b, e->grid_move, ahead ? (uint32_t)(e->grid_transport - 1) : 0u, e->stream);
   mark(3);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   mark(4);
 }
+
+// Whether consecutive steps of a call may run each next step's transport
+// inside the resolve kernel: not when the host writes flows before every
+// step (time-dependent feeds) or may re-lay the runs between steps (growth),
+// and only for whole steps.
+bool can_transport_ahead(const mcst_engine* e, uint32_t stages);
 
 int read_state(mcst_engine* e, int32_t run, RunState& s) {
   HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1414,15 +1424,21 @@ bool has_time_feeds(const mcst_engine* e) {
   return false;
 }
 
-// One step of the batch, then the reorders that fall due.
-int step_once(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr) {
+bool can_transport_ahead(const mcst_engine* e, uint32_t stages) {
+  return stages == STAGE_ALL && !has_time_feeds(e) && !has_growth(e);
+}
+
+// One step of the batch, then the reorders that fall due (transported,
+// ahead: as launch_step).
+int step_once(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr, bool transported = false,
+              bool ahead = false) {
   const int rcg = grow_runs(e);
   if (rcg) return rcg;
   const int rcf = prepare_feeds(e);
   if (rcf) return rcf;
   std::vector<uint32_t> before(e->runs.size());
   for (size_t r = 0; r < e->runs.size(); ++r) before[r] = e->runs[r].host_step;
-  launch_step(e, stages, ev);
+  launch_step(e, stages, ev, transported, ahead);
   count_steps(e, 1);
   for (size_t r = 0; r < e->runs.size(); ++r) {
     const HostRun& h = e->runs[r];
@@ -1920,8 +1936,12 @@ int mcst_engine_step(mcst_engine* e, uint32_t n_steps) {
   CHECK_ENGINE(e);
   if (!e->committed) return fail(MCST_ERR_STATE, "engine not committed");
   HIP_TRY(hipSetDevice(e->device));
+  // every step but the last transports the next one inside its resolve
+  // kernel; the last leaves the liquid as a step ends, so what the host
+  // reads between calls is the same either way
+  const bool ahead = can_transport_ahead(e, STAGE_ALL);
   for (uint32_t k = 0; k < n_steps; ++k) {
-    const int rc = step_once(e, STAGE_ALL);
+    const int rc = step_once(e, STAGE_ALL, nullptr, ahead && k > 0, ahead && k + 1 < n_steps);
     if (rc) return rc;
   }
   HIP_TRY(hipGetLastError());
@@ -1953,7 +1973,8 @@ int mcst_engine_capture(mcst_engine* e, uint32_t n_steps) {
   e->graph_exec = nullptr;
   e->graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-  for (uint32_t k = 0; k < n_steps; ++k) launch_step(e, STAGE_ALL);
+  const bool ahead = can_transport_ahead(e, STAGE_ALL);
+  for (uint32_t k = 0; k < n_steps; ++k) launch_step(e, STAGE_ALL, nullptr, ahead && k > 0, ahead && k + 1 < n_steps);
   HIP_TRY(hipStreamEndCapture(e->stream, &e->graph));
   HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
   e->graph_steps = n_steps;

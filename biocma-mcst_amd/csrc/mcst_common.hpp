@@ -125,7 +125,9 @@ struct RunDev {
   const uint32_t* vf_sink_k;   // (| VF_GAS for a gas outlet)
   const int32_t* vx_slot;      // [n_c]
   uint32_t n_vf;
-  double* rec;              // [n_c][rec] per-compartment record read by the particle pass
+  double* rec;              // [2][n_c][rec_w] per-compartment record read by the particle pass, the
+                            // table of step k at k & 1 (the next step's transport may write its
+                            // table while this step's resolve kernel reads the other)
   double* probe_exit;       // exit ages (residence times), may be null
   double* probe_div;        // division ages, may be null
   uint32_t cap;             // slots reserved for the run

@@ -125,28 +125,30 @@ __device__ __forceinline__ void column_inflow(const uint32_t* __restrict__ col_p
   }
 }
 
+// The transport of every active run's compartments for the step `ahead`
+// steps after the run's current one (0: the step about to run; 1: the next
+// one, computed during this step's resolve kernel), by blocks `blk` of
+// `n_blk`.  It reads C and the contributions S of the last pass and writes
+// the other C buffer, m, the record table of its step's parity and clears S;
+// the resolve kernel reads none of these but the other record table.
 template <class M>
-__global__ void __launch_bounds__(256) k_transport(BatchDev b) {
-  if (blockIdx.x == gridDim.x - 1) {
-    layout_pass<256>(b);
-    return;
-  }
+__device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t n_blk, uint32_t ahead) {
   constexpr int NS = M::NS;
   const uint32_t total = b.n_comp_total;
-  const uint32_t stride = (gridDim.x - 1) * blockDim.x;
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+  const uint32_t stride = n_blk * blockDim.x;
+  for (uint32_t g = blk * blockDim.x + threadIdx.x; g < total; g += stride) {
     const uint32_t r = find_run(b.comp_off, b.R, g);
     const RunDev& rd = b.runs[r];
     const RunState* rs = b.rs + r;
-    // the run's activity is recomputed here rather than read from the
-    // layout block, which writes it concurrently
-    if (rs->step >= rd.step_target) continue;
+    // the step being prepared; the run's activity is recomputed here rather
+    // than read from the layout, which other blocks write concurrently
+    const uint32_t step = rs->step + ahead;
+    if (step >= rd.step_target) continue;
     const uint32_t j = g - b.comp_off[r];
     const uint32_t n_c = rd.n_c;
-    const uint32_t step = rs->step;
     const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
     constexpr int NR = record_terms<M>();
-    double* rec = rd.rec + (size_t)j * rd.rec_w;
+    double* rec = rd.rec + ((size_t)(step & 1u) * n_c + j) * rd.rec_w;
     rec[NR] = mp.p_move[j];
     double qx = mp.q_exit[j];
     if (rd.n_vf && rd.vx_slot[j] >= 0) qx = rd.vf_q[rd.n_vf + (uint32_t)rd.vx_slot[j]];
@@ -236,6 +238,15 @@ __global__ void __launch_bounds__(256) k_transport(BatchDev b) {
       rd.S[o] = 0.0;
     }
   }
+}
+
+template <class M>
+__global__ void __launch_bounds__(256) k_transport(BatchDev b) {
+  if (blockIdx.x == gridDim.x - 1) {
+    layout_pass<256>(b);
+    return;
+  }
+  transport_compartments<M>(b, blockIdx.x, gridDim.x - 1, 0u);
 }
 
 // ------------------------------------------------------------------ particles
@@ -429,9 +440,10 @@ __global__ void __launch_bounds__(BLK)
   struct SubRef {
     uint32_t sc, t, r, slot0, cnt;  // slot0: run-relative slot of the sub-chunk's first slot
     uint64_t base;                   // its batch-wide slot
+    const double* rec;               // the run's record table of its current step
   };
   auto sub_ref = [&](uint32_t sc) -> SubRef {
-    SubRef q{sc, 0u, 0u, 0u, 0u, 0ull};
+    SubRef q{sc, 0u, 0u, 0u, 0u, 0ull, nullptr};
     // a phantom stands at the first slot of the pass's last tile: the run of
     // that tile is live, so the loads of the phantom (particle arrays, the
     // run's compartment record) stay inside memory the run owns; the run of
@@ -445,6 +457,7 @@ __global__ void __launch_bounds__(BLK)
       q.cnt = n > q.slot0 ? (n - q.slot0 < SUB ? n - q.slot0 : SUB) : 0u;
     }
     q.base = runs[q.r].p_off + q.slot0;
+    q.rec = runs[q.r].rec + ((rsr[q.r].step & 1u) ? (size_t)runs[q.r].n_c * runs[q.r].rec_w : (size_t)0);
     return q;
   };
   // sub-chunk-relative slot of the lane's j-th slot: interleaved (64 apart,
@@ -491,7 +504,7 @@ __global__ void __launch_bounds__(BLK)
     }
   };
   auto gather = [&](const SubRef& q, const Sub& a, double (&rr)[NL][NQ]) {
-    const GLOBAL_AS double* __restrict__ rec = gp(runs[q.r].rec);
+    const GLOBAL_AS double* __restrict__ rec = gp(q.rec);
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
       // slots past the count hold no particle: their position may be anything
@@ -967,10 +980,17 @@ __device__ __forceinline__ uint32_t request_prefix(const BatchDev& b, uint32_t s
   return wave_sum(v + x2 + x1 + x0);
 }
 
+//
+// With tr_blocks > 0 the grid's first tr_blocks blocks (dispatched first:
+// their chain is the shorter) transport the liquid (and gas) for the next
+// step meanwhile: the two touch disjoint data (the
+// records of the two steps are separate tables, the concentrations are
+// double-buffered), and the block that keeps the books also lays out the next
+// step's tiles, so the next step starts with its particle pass.
 template <class M>
 __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
                                                  const MapDev* __restrict__ maps,
-                                                 const uint32_t* __restrict__ tile_off) {
+                                                 const uint32_t* __restrict__ tile_off, uint32_t tr_blocks) {
   constexpr int NV = M::NV;
   constexpr int NP = M::NP > 0 ? M::NP : 1;
   __shared__ uint32_t s_pre[RES_SC + 1];   // movers: prefix over the group's sub-chunks
@@ -994,11 +1014,15 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   // become a long serial chain.  On the 50 000-compartment lattice at 1e8
   // particles groups of 64, 128 and 256 took 225, 227 and 266 us per step on
   // average.  Threads GS..255 hold no sub-chunk.
+  const uint32_t n_res = gridDim.x - tr_blocks;  // blocks of the resolve work
+  const bool tr_block = blockIdx.x < tr_blocks;
+  const uint32_t rb = tr_block ? 0u : blockIdx.x - tr_blocks;  // index among the resolve blocks
   uint32_t GS = 32;
-  while (GS < RES_SC && (n_sc + 2 * GS - 1) / (2 * GS) >= gridDim.x) GS *= 2;
+  while (GS < RES_SC && (n_sc + 2 * GS - 1) / (2 * GS) >= n_res) GS *= 2;
   const uint32_t n_groups = (n_sc + GS - 1) / GS;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
-  for (uint32_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+  if (tr_block) transport_compartments<M>(b, blockIdx.x, tr_blocks, 1u);
+  for (uint32_t grp = rb; !tr_block && grp < n_groups; grp += n_res) {
     const uint32_t sc0 = grp * GS;
     {  // the group's sub-chunks, one thread each: block-wide prefix sums
       const uint32_t sc = sc0 + tid;
@@ -1059,7 +1083,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       b.pos[g] = pos;
       bool exits = false;
       if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {
-        const double qx = gp(rd.rec)[(size_t)pos * rd.rec_w + record_terms<M>() + 1];
+        const double qx = gp(rd.rec)[((size_t)(step & 1u) * rd.n_c + pos) * rd.rec_w + record_terms<M>() + 1];
         if (qx < 1.0 && u01(u.z) > qx) {
           const double age = b.age_hyd[g];
           if (rd.probe_exit) {
@@ -1168,19 +1192,22 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   // writes back this XCD's L2 each time: 60 us over a 2048-block grid).
   // Blocks count themselves done on RESOLVE_TICKETS sub-counters (block b on
   // b % RESOLVE_TICKETS), and the last of each on the main counter, so no counter
-  // takes more than a few dozen atomics; blocks without a group take no part.
-  const uint32_t nb = n_groups < gridDim.x ? n_groups : gridDim.x;  // blocks with work
-  if (nb == 0) {
+  // takes more than a few dozen atomics; resolve blocks without a group take no
+  // part, transport blocks do (the books advance the step count they read).
+  const uint32_t nb = n_groups < n_res ? n_groups : n_res;  // resolve blocks with work
+  const uint32_t np = nb + tr_blocks;                       // blocks that count themselves done
+  if (np == 0) {
     if (blockIdx.x == 0) bookkeep_pass(b, runs, n_sc);
     return;
   }
-  if (blockIdx.x >= nb) return;
+  if (!tr_block && rb >= nb) return;
+  const uint32_t id = tr_block ? blockIdx.x : tr_blocks + rb;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid == 0) {
-    const uint32_t j = blockIdx.x % RESOLVE_TICKETS;
-    const uint32_t in_j = (nb - j + RESOLVE_TICKETS - 1u) / RESOLVE_TICKETS;  // blocks counting on sub-counter j
-    const uint32_t n_sub = nb < RESOLVE_TICKETS ? nb : RESOLVE_TICKETS;        // sub-counters in use
+    const uint32_t j = id % RESOLVE_TICKETS;
+    const uint32_t in_j = (np - j + RESOLVE_TICKETS - 1u) / RESOLVE_TICKETS;  // blocks counting on sub-counter j
+    const uint32_t n_sub = np < RESOLVE_TICKETS ? np : RESOLVE_TICKETS;        // sub-counters in use
     uint32_t last = 0;
     if (atomicAdd(b.ticket + 1u + j, 1u) == in_j - 1u) last = atomicAdd(b.ticket, 1u) == n_sub - 1u ? 1u : 0u;
     s_last = last;
@@ -1188,6 +1215,10 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   __syncthreads();
   if (!s_last) return;
   bookkeep_pass(b, runs, n_sc);
+  if (tr_blocks) {
+    __syncthreads();
+    layout_pass<256>(b);
+  }
 }
 
 // ------------------------------------------------------------------ layout
@@ -1340,8 +1371,9 @@ void transport_t(const BatchDev& b, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_transport<M>, dim3(grid), dim3(256), 0, s, b);
 }
 template <class M>
-void resolve_t(const BatchDev& b, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve<M>, dim3(grid), dim3(256), 0, s, b, b.runs, b.maps, (const uint32_t*)b.tile_off);
+void resolve_t(const BatchDev& b, int grid, uint32_t tr_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve<M>, dim3(grid + (int)tr_blocks), dim3(256), 0, s, b, b.runs, b.maps,
+                     (const uint32_t*)b.tile_off, tr_blocks);
 }
 template <class M>
 const void* resolve_kernel_t() {
