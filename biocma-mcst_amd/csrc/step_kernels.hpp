@@ -947,18 +947,21 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
   for (uint32_t i = tid; i <= RESOLVE_TICKETS; i += NT) b.ticket[i] = 0u;
 }
 
-// After the particle pass, per group of 32 or 64 sub-chunks (one per lane of
-// the first wave): (1) the movers the pass listed take their neighbour, the
-// first CDF column at or above u, and an Idle one runs the exit test in its
-// new compartment (move_kernel.hpp:392-437, 586-648); (2) the division
-// requests are accepted in slot order over the whole run: the k-th request
-// of the step (k counted from the run's first slot) succeeds iff
-// k < min(cap - n, div_cap), the reference's division buffer
-// (particles_container.hpp:559-573) made deterministic; the mother is
-// halved (M::division) and the daughter written straight to slot n + k,
-// where the reference's merge_buffer would append her (:575-599).  k is the
-// requests before the group (the per-block counts of the pass, summed over
-// at most 64 entries per level) plus the requests before it in the group.
+// After the particle pass, per group of 32 or 64 sub-chunks (one per thread
+// of the block for the prefix sums): (1) the movers the pass listed take
+// their neighbour, the first CDF column at or above u, and an Idle one runs
+// the exit test in its new compartment (move_kernel.hpp:392-437, 586-648);
+// (2) the division requests are accepted in slot order over the whole run:
+// the k-th request of the step (k counted from the run's first slot)
+// succeeds iff k < min(cap - n, div_cap), the reference's division buffer
+// (particles_container.hpp:559-573) made deterministic; the daughter is
+// written straight to slot n + k, where the reference's merge_buffer would
+// append her (:575-599), from the mother's state the pass staged (the pass
+// stored the halved mother; a refused request restores her) or, past the
+// first DIV_STAGE requests of a sub-chunk, from the mother's slot (halved
+// here, M::division).  k is the requests before the group (the per-block
+// counts of the pass, summed over at most 64 entries per level) plus the
+// requests before it in the group.
 // Every mover and every request is independent, so the processing order is
 // free.  (3) The last block to finish keeps the books of every run: counters,
 // births, the removal decision (update_and_remove_inactive,
@@ -980,13 +983,12 @@ __device__ __forceinline__ uint32_t request_prefix(const BatchDev& b, uint32_t s
   return wave_sum(v + x2 + x1 + x0);
 }
 
-//
-// With tr_blocks > 0 the grid's first tr_blocks blocks (dispatched first:
-// their chain is the shorter) transport the liquid (and gas) for the next
-// step meanwhile: the two touch disjoint data (the
-// records of the two steps are separate tables, the concentrations are
-// double-buffered), and the block that keeps the books also lays out the next
-// step's tiles, so the next step starts with its particle pass.
+// The resolve kernel.  With tr_blocks > 0 the grid's first tr_blocks blocks
+// (dispatched first: their chain is the shorter) transport the liquid (and
+// gas) for the next step meanwhile: the two touch disjoint data (the records
+// of the two steps are separate tables, the concentrations are
+// double-buffered), and the block that keeps the books also lays out the
+// next step's tiles, so the next step starts with its particle pass.
 template <class M>
 __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
                                                  const MapDev* __restrict__ maps,
