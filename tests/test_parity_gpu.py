@@ -278,6 +278,79 @@ def test_steps_in_one_call_equal_steps_one_by_one(model):
         eng.close()
 
 
+def test_crowded_divisions_with_refusals_in_a_batch():
+    """Every particle of two runs asks to divide at once (birth mass half the
+    mass): every sub-chunk holds 256 requests, so the first 16 of each are
+    divided in the particle pass and the rest by the resolve kernel, and the
+    runs' free slots accept only part of them, refusing staged and unstaged
+    requests alike (the pass's mothers restored).  Both runs, side by side in
+    one batch and over three steps (the refused ask again), match the oracle."""
+    import mcst
+    from mcst import cases
+    from oracle.oracle import OracleRun
+
+    c = cases.small_case("monod_mass", n_c=12, n_particles=100, seed=9)
+    eng = mcst.Engine("monod_mass")
+    specs = [(3000, 4000, 21), (5000, 5600, 22)]  # (particles, capacity, seed)
+    pairs = []
+    for n, cap, seed in specs:
+        rng = np.random.default_rng(seed)
+        pos = rng.integers(0, c.n_c, n).astype(np.uint32)
+        m = rng.uniform(1.0e-15, 1.5e-15, n)
+        state = np.stack([m, 0.5 * m])
+        g = eng.add_run(c.n_c, c.dt, seed)
+        o = OracleRun(c.model, c.n_c, c.n_s, c.dt, seed)
+        for r in (g, o):
+            r.set_params(c.params)
+            for k, (V, rows, cols, flows) in enumerate(c.maps):
+                r.add_flowmap(V, rows, cols, flows)
+            r.set_steps_per_map(c.steps_per_map)
+            r.set_concentrations(c.C0)
+            r.set_particles(cap, pos, np.zeros(n, np.uint8), state, np.zeros(n), np.zeros(n), 1e3)
+            r.set_compaction(0.01, 0)
+        g.set_probes(1 << 16)
+        pairs.append((g, o, n, cap))
+    eng.commit()
+    for step in range(3):
+        eng.step(1)
+        for g, o, n, cap in pairs:
+            o.step(1)
+            assert_parity(g, o)
+    for g, o, n, cap in pairs:
+        cnt = g.counters()
+        assert cnt["births"] == cap - n and cnt["overflow"] > 0, (cnt, n, cap)
+    eng.close()
+
+
+def test_batch_with_and_without_a_leaving_flow():
+    """A batch where one run has an outlet and the other none takes the pass
+    variant with the leaving-flow work for both (the run without an outlet
+    then reads no hydraulic age of its own); a batch of runs without outlets
+    takes the variant without it.  Every run matches its oracle."""
+    import mcst
+    from mcst import cases
+    from oracle.oracle import OracleRun
+
+    for outlets in ((True, False), (False, False)):
+        eng = mcst.Engine("dual_monod")
+        pairs = []
+        for k, feeds in enumerate(outlets):
+            c = cases.small_case("dual_monod", n_c=10, n_particles=4000, seed=30 + k, feeds=feeds)
+            g = eng.add_run(c.n_c, c.dt, c.seed)
+            o = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+            for r in (g, o):
+                c.apply(r)
+            g.set_probes(1 << 16)
+            pairs.append((g, o))
+        eng.commit()
+        eng.step(7)
+        for g, o in pairs:
+            o.step(7)
+            assert_parity(g, o)
+        assert (pairs[0][0].counters()["exits"] > 0) == outlets[0]
+        eng.close()
+
+
 def test_glucose_equivalent_mass_balance_rgg5000():
     """Closed dual-substrate system: liquid glucose + pending sources +
     weight * sum(m/Yxs + X) is conserved to 1e-10 relative."""
