@@ -29,22 +29,29 @@
  *                           (monod_mass, dual_monod, quad_monod: the benchmark cases' models;
  *                           udf_minimal: apps/udf_model/minimal.cpp:23-107, and udf_starvation,
  *                           the example plugins; the latter's Dead status: mc/alias.hpp:124-130)
- *   gas-liquid transfer ... apps/libs/simulation/src/implScalar.cpp:228-248 (liquid side, fixed gas)
+ *   gas-liquid transfer ... apps/libs/simulation/src/simulation.model.cpp:131-150,
+ *                           implScalar.cpp:228-248,266-297, hydro/mass_transfer.cpp:102-157
+ *                           (two-phase step: gas transport, kLa, clearNegs), and the liquid
+ *                           side alone with a fixed gas saturation
+ *   flowmap tables ........ apps/libs/mc/src/domain.cpp:44-74 with implScalar.cpp:316-330
  *   export properties ..... apps/core/public/core/post_process.hpp:54-98
  *
  * Checks that pin each part (tests/test_oracle_kat.py, test_oracle_invariants.py,
+ * test_oracle_models.py, test_two_phase_cpu.py, test_flow_tables_cpu.py,
  * test_export_cpu.py, test_case_io_cpu.py): Random123 and numpy Philox vectors;
  * scipy for the normal quantile and for sparse transport; the reference's
- * transition-matrix, 16-compartment flow, feed and container fixtures; closed
- * forms of the model updates (Euler at constant substrate, uptake lag, the
- * oxygen-free overflow of quad_monod), of gas-liquid transfer in a 0-D box and
- * of the move/exit probabilities; exact distributions (chi-square, KS) of moves,
- * residence times and initial states; mass balance; a numpy stable sort for the
- * reorder; math.fsum for the property sums.
+ * transition-matrix, 16-compartment flow and neighbour, feed and container
+ * fixtures; closed forms of the model updates (Euler at constant substrate,
+ * the Monod mu_p lag and caps, FixedLength's linear growth, SimpleAcetate's
+ * uptake branches, dual_monod's and quad_monod's lags and splits), of the
+ * two-phase transfer in a 0-D box and of the move/exit probabilities; exact
+ * distributions (chi-square, KS) of moves, residence times, initial states and
+ * division draws; mass balance; a numpy stable sort for the reorder;
+ * math.fsum for the property sums.
  * Parity unpinned: which random number each particle receives in the
- * reference (Kokkos XorShift1024 pool, replaced by Philox by design), rcmtool's
- * neighbour order and CDF padding (here: ascending neighbours, padding repeats
- * the last), and Eigen's summation order inside C*M (here: ascending source).
+ * reference (Kokkos XorShift1024 pool, replaced by Philox by design), and
+ * Eigen's summation order inside C*M (here: the transport matrix's column
+ * entries in the caller's order).
  *
  * Deliberate, documented differences from the reference (the reference is
  * itself scheduling-dependent, so parity is defined against these rules):
@@ -52,9 +59,10 @@
  *     (slot, step, purpose), not from a thread-indexed XorShift pool;
  *   - the leave test u > exp(-dt*Q/V) uses a per-compartment threshold
  *     computed once per flowmap/step on the host;
- *   - the k-th division of a 1024-slot tile gets staging slot k (a per-tile
- *     budget replaces the atomic buffer index), newborns are appended in
- *     parent slot order;
+ *   - division requests are accepted in slot order over the whole run while
+ *     it has free slots (and up to an optional newborn cap per step), the
+ *     k-th accepted daughter taking slot n + k (parent slot order), written
+ *     before the removal compacts [0, n + births);
  *   - removal fills the i-th gap (ascending) with the i-th live particle from
  *     the tail (descending) instead of an atomic tail offset;
  *   - per-particle state, contributions and ages are FP64;
@@ -366,7 +374,7 @@ static int model_update(int model, const double* p, double dt, double* x, const 
       contrib[0] = x[5];
       return (x[0] >= x[1]) ? ORC_DIVISION : ORC_IDLE;
     }
-    case ORC_MODEL_FIXED_LENGTH: { /* fixed_length.hpp:374-394 */
+    case ORC_MODEL_FIXED_LENGTH: { /* fixed_length.hpp:123-142 */
       const double ld = lin_density(p[3]);
       const double phi_s_max = (ld * p[0]) / 0.5;
       const double s = c[0];
@@ -484,7 +492,7 @@ static void model_division(int model, const double* p, double* x, double* xd, ui
       xd[5] = 0.0;
       break;
     }
-    case ORC_MODEL_FIXED_LENGTH: { /* fixed_length.hpp:396-413 */
+    case ORC_MODEL_FIXED_LENGTH: { /* fixed_length.hpp:144-161 */
       const double h = x[0] / 2.0;
       x[0] = h;
       x[1] = p[1];
