@@ -659,7 +659,7 @@ int check_definable(const mcst_engine* e, const HostRun& h, uint32_t r) {
 
 // Launch shapes for the runs on the device: LDS contribution accumulator
 // when the largest live run's [n_s][n_c] fits, then the block size with the
-// most resident waves; a grid of resident blocks.  MCST_PARTICLE_BLOCK
+// most resident waves, up to 8 per CU; a grid of that many blocks per CU.  MCST_PARTICLE_BLOCK
 // (256/512/1024), MCST_CONTRIB_LDS (0/1) and MCST_TILE_GROUP override.
 int choose_launch(mcst_engine* e) {
   const ModelEntry& M = *e->model;
@@ -705,14 +705,21 @@ int choose_launch(mcst_engine* e) {
     hipFuncAttributes fa{};
     if (hipFuncGetAttributes(&fa, fn) == hipSuccess && fa.localSizeBytes > 0 && !forced_block)
       per_cu = per_cu / 2;  // register spills: prefer a variant that fits
-    const int waves = per_cu * blk / 64;
+    // waves per CU count up to 8, and a grid holds at most 8 waves per CU
+    // (but one block of 1024): the pass is bound by the misses a CU's vector
+    // L1 keeps in flight, and more waves only share them (on the
+    // 1e7-particle network 16 waves per CU took 0.204-0.214 ms against 0.167
+    // ms for 8); on a tie the smaller block wins (more blocks, shorter
+    // flushes)
+    const int use = std::min(per_cu, std::max(1, 512 / blk));
+    const int waves = std::min(use * blk / 64, 8);
     if (per_cu > 0 && waves > best_waves) {
       best_waves = waves;
       best.block = blk;
       best.priv = priv;
       best.win = priv && windowed;
       best.lds = lds;
-      best.grid = per_cu * prop.multiProcessorCount;
+      best.grid = use * prop.multiProcessorCount;
     }
   }
   if (best_waves < 0) {  // no resident configuration with the accumulator: go direct
@@ -721,7 +728,7 @@ int choose_launch(mcst_engine* e) {
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, M.ops->particle_kernel(256, false, false, false, e->any_leave), 256, 0));
-    best.grid = std::max(per_cu, 1) * prop.multiProcessorCount;
+    best.grid = std::min(std::max(per_cu, 1), 2) * prop.multiProcessorCount;  // 8 waves per CU
   }
   best.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best.grid, b.T_max));
   best.probe = probe;
@@ -798,6 +805,9 @@ int upload_run(mcst_engine* e, uint32_t r) {
     HIP_TRY(hipMemcpy(b.status + h.p_off, h.status.data(), h.n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b.age_hyd + h.p_off, h.ah.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b.age_div + h.p_off, h.ad.data(), h.n * sizeof(double), hipMemcpyHostToDevice));
+    // the run's age clocks start at 0: every age is its base age
+    HIP_TRY(hipMemset(b.clk_hyd + h.p_off, 0, h.n * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(b.clk_div + h.p_off, 0, h.n * sizeof(uint32_t)));
     for (uint32_t v = 0; v < nv; ++v)
       HIP_TRY(hipMemcpy(b.state + (uint64_t)v * b.P + h.p_off, h.state.data() + (size_t)v * h.n,
                         h.n * sizeof(double), hipMemcpyHostToDevice));
@@ -1064,6 +1074,8 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.state, P * nv * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.age_hyd, P * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.age_div, P * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.clk_hyd, P * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.clk_div, P * sizeof(uint32_t)))) return rc;
   uint32_t** tile_arrays[] = {&b.tile_gap, &b.tile_fill, &b.tile_gap_off, &b.tile_fill_off};
   for (uint32_t** a : tile_arrays)
     if ((rc = dev_alloc(e, (void**)a, T_max * sizeof(uint32_t)))) return rc;
@@ -1073,7 +1085,7 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.move_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.div_stage, T_max * b.nsub * DIV_STAGE * (nv + 1) * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_stage, T_max * b.nsub * DIV_STAGE * nv * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * b.nsub * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.sub_divs, T_max * b.nsub * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&e->d_runs, (size_t)RC * sizeof(RunDev)))) return rc;
@@ -1173,6 +1185,34 @@ int read_state(mcst_engine* e, int32_t run, RunState& s) {
   return MCST_OK;
 }
 
+// The hydraulic and division ages of slots [o, o + n) of a run in state s
+// (read_state, the stream idle), from the device's base ages and age-clock
+// origins (clock_age, the kernels' expression): an Idle slot ages with its
+// run's clocks, another keeps the ages it had when it stopped being Idle.
+// Either output may be null; the bytes are copied, so they need no alignment.
+int read_ages(mcst_engine* e, uint64_t o, uint64_t n, const RunState& s, double dt, void* age_hyd, void* age_div) {
+  const BatchDev& b = e->bd;
+  std::vector<uint8_t> st(n);
+  std::vector<double> ah(n), ad(n);
+  std::vector<uint32_t> ch(n), cd(n);
+  HIP_TRY(hipMemcpy(st.data(), b.status + o, n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ah.data(), b.age_hyd + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ad.data(), b.age_div + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ch.data(), b.clk_hyd + o, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(cd.data(), b.clk_div + o, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (uint64_t q = 0; q < n; ++q) {
+    if (st[q] == IDLE) {
+      ah[q] = clock_age(ah[q], s.clock_hyd, ch[q], dt);
+      ad[q] = clock_age(ad[q], s.clock_div, cd[q], dt);
+    } else {
+      ad[q] = clock_age(ad[q], ch[q], cd[q], dt);
+    }
+  }
+  if (age_hyd) std::memcpy(age_hyd, ah.data(), n * sizeof(double));
+  if (age_div) std::memcpy(age_div, ad.data(), n * sizeof(double));
+  return MCST_OK;
+}
+
 // Reorders run r's particles by compartment, stable in slot order
 // (sort.hip).  Reads the particle count, so it waits for the stream.
 int sort_run(mcst_engine* e, uint32_t r) {
@@ -1213,6 +1253,9 @@ int sort_run(mcst_engine* e, uint32_t r) {
   a.f64[nv] = b.age_hyd + o;
   a.f64[nv + 1] = b.age_div + o;
   a.n_f64 = nv + 2;
+  a.u32[0] = b.clk_hyd + o;
+  a.u32[1] = b.clk_div + o;
+  a.n_u32 = 2;
   a.keys = (uint32_t*)m;
   a.iota = a.keys + cap;
   a.perm = a.iota + cap;
@@ -1247,6 +1290,8 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
       {(void**)&b.status, P1, P0, 0, nullptr},
       {(void**)&b.age_hyd, P1 * 8, P0 * 8, 0, nullptr},
       {(void**)&b.age_div, P1 * 8, P0 * 8, 0, nullptr},
+      {(void**)&b.clk_hyd, P1 * 4, P0 * 4, 0, nullptr},
+      {(void**)&b.clk_div, P1 * 4, P0 * 4, 0, nullptr},
       {(void**)&b.state, P1 * nv * 8, P0 * 8, nv, nullptr},
       {(void**)&b.div_l1, lv.n1 * 4, 0, 0, nullptr},
       {(void**)&b.div_l2, lv.n2 * 4, 0, 0, nullptr},
@@ -1260,7 +1305,7 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
       {(void**)&b.move_pos, P1 * 4, 0, 0, nullptr},
       {(void**)&b.div_slot, P1 * 2, 0, 0, nullptr},
       {(void**)&b.div_pos, P1 * 4, 0, 0, nullptr},
-      {(void**)&b.div_stage, T1 * b.nsub * DIV_STAGE * (nv + 1) * 8, 0, 0, nullptr},
+      {(void**)&b.div_stage, T1 * b.nsub * DIV_STAGE * nv * 8, 0, 0, nullptr},
       {(void**)&b.sub_moves, T1 * b.nsub * 4, 0, 0, nullptr},
       {(void**)&b.sub_divs, T1 * b.nsub * 4, 0, 0, nullptr},
   };
@@ -1330,6 +1375,8 @@ int regrow_run(mcst_engine* e, uint32_t r, uint64_t n, uint64_t new_cap) {
     HIP_TRY(hipMemcpyAsync(b.status + o1, b.status + o0, n, hipMemcpyDeviceToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(b.age_hyd + o1, b.age_hyd + o0, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(b.age_div + o1, b.age_div + o0, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(b.clk_hyd + o1, b.clk_hyd + o0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(b.clk_div + o1, b.clk_div + o0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, e->stream));
     for (uint32_t v = 0; v < (uint32_t)e->model->nv; ++v)
       HIP_TRY(hipMemcpyAsync(b.state + (uint64_t)v * b.P + o1, b.state + (uint64_t)v * b.P + o0, n * sizeof(double),
                              hipMemcpyDeviceToDevice, e->stream));
@@ -2155,8 +2202,10 @@ int mcst_run_get_particles(mcst_engine* e, int32_t run, uint32_t* pos, uint8_t* 
   if (n == 0) return MCST_OK;
   if (pos) HIP_TRY(hipMemcpy(pos, b.pos + h.p_off, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (status) HIP_TRY(hipMemcpy(status, b.status + h.p_off, n, hipMemcpyDeviceToHost));
-  if (age_hyd) HIP_TRY(hipMemcpy(age_hyd, b.age_hyd + h.p_off, n * sizeof(double), hipMemcpyDeviceToHost));
-  if (age_div) HIP_TRY(hipMemcpy(age_div, b.age_div + h.p_off, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (age_hyd || age_div) {
+    rc = read_ages(e, h.p_off, n, s, h.dt, age_hyd, age_div);
+    if (rc) return rc;
+  }
   if (state)
     for (int v = 0; v < e->model->nv; ++v)
       HIP_TRY(hipMemcpy(state + (size_t)v * n, b.state + (uint64_t)v * b.P + h.p_off, n * sizeof(double),
@@ -2380,8 +2429,8 @@ int mcst_run_checkpoint(mcst_engine* e, int32_t run, void* buf, uint64_t size, u
   std::memset(p, 0, ((n + 7) / 8) * 8 - n);
   p += ((n + 7) / 8) * 8 - n;
   for (uint32_t v = 0; v < nv; ++v) HIP_TRY(down(b.state + (uint64_t)v * b.P + o, 8 * n));
-  HIP_TRY(down(b.age_hyd + o, 8 * n));
-  HIP_TRY(down(b.age_div + o, 8 * n));
+  if ((rc = read_ages(e, o, n, s, d.dt, (double*)p, (double*)(p + 8 * n)))) return rc;
+  p += 16 * n;
   HIP_TRY(down(d.C[s.step & 1u], 8 * nsc));
   HIP_TRY(down(d.m, 8 * nsc));
   HIP_TRY(down(d.S, 8 * nsc));

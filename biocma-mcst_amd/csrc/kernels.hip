@@ -258,6 +258,8 @@ __global__ void __launch_bounds__(BLOCK) k_gap_fill(BatchDev b, uint32_t n_var) 
       b.status[dst] = IDLE;
       b.age_hyd[dst] = b.age_hyd[src];
       b.age_div[dst] = b.age_div[src];
+      b.clk_hyd[dst] = b.clk_hyd[src];
+      b.clk_div[dst] = b.clk_div[src];
       for (uint32_t v = 0; v < n_var; ++v) b.state[(uint64_t)v * b.P + dst] = b.state[(uint64_t)v * b.P + src];
     }
   }

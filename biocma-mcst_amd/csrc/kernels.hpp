@@ -57,6 +57,8 @@ struct SortArgs {
   uint8_t* status = nullptr;
   double* f64[MAX_VAR + 2] = {};
   uint32_t n_f64 = 0;
+  uint32_t* u32[2] = {};
+  uint32_t n_u32 = 0;
   uint32_t *keys = nullptr, *iota = nullptr, *perm = nullptr;
   void* buf = nullptr;
   void* temp = nullptr;

@@ -158,6 +158,11 @@ struct RunState {
   uint32_t n_tiles;         // tiles of the run in the current layout
   uint32_t gap_total;
   uint32_t active;          // 1 while step < step_target
+  // age clocks: the steps so far in which the run's Idle particles aged
+  // (division age: steps with the model stage; hydraulic age: steps with the
+  // leave stage while the run has a leaving flow)
+  uint32_t clock_div;
+  uint32_t clock_hyd;
   uint32_t pad;
   unsigned long long step_exits, step_deaths, step_moves, step_requests, step_processed;
   unsigned long long counters[C_COUNT];
@@ -170,8 +175,15 @@ struct BatchDev {
   uint32_t* pos;
   uint8_t* status;
   double* state;            // [n_var][P]
+  // ages: an Idle slot's age is age_x + (clock_x - clk_x) * dt, with clock_x
+  // its run's age clock, so a step writes them only when they restart;
+  // a slot that is not Idle has hydraulic age age_hyd and division age
+  // age_div + (clk_hyd - clk_div) * dt (clk_hyd holds the division clock at
+  // which it stopped being Idle)
   double* age_hyd;
   double* age_div;
+  uint32_t* clk_hyd;
+  uint32_t* clk_div;
   uint64_t P;               // total slots
   // division requests of the pass per block of 2^6, 2^12 and 2^18
   // sub-chunks (global sub-chunk index of the layout); with sub_divs they
@@ -195,9 +207,9 @@ struct BatchDev {
   uint32_t* move_pos;       // [T_max*TILE] their compartment before the move
   uint16_t* div_slot;       // [T_max*TILE] tile-local slots of the division requests
   uint32_t* div_pos;        // [T_max*TILE] their compartment (the newborn's)
-  double* div_stage;        // [T_max*nsub*DIV_STAGE][n_var+1] the first DIV_STAGE requests of each
+  double* div_stage;        // [T_max*nsub*DIV_STAGE][n_var] the first DIV_STAGE requests of each
                             // sub-chunk, divided in the pass: the mother's state before division
-                            // and her division age (to restore her if the request is refused)
+                            // (to restore her if the request is refused)
   uint32_t* sub_moves;      // [T_max*nsub] entries of each sub-chunk's mover segment
   uint32_t* sub_divs;       // [T_max*nsub] entries of each sub-chunk's request segment
   // runs
@@ -214,5 +226,11 @@ struct BatchDev {
   uint32_t acc_w;           // particle pass: compartments per LDS contribution window (PRIV)
   uint32_t stages;
 };
+
+// Age of a slot from its base age, its origin on the age clock and the
+// clock (the expression the host's conversion uses too).
+__host__ __device__ inline double clock_age(double base, uint32_t clock, uint32_t origin, double dt) {
+  return base + (double)(clock - origin) * dt;
+}
 
 }  // namespace mcst

@@ -268,6 +268,11 @@ hipError_t sort_run_particles(const SortArgs& a, hipStream_t s) {
                        (const double*)a.f64[f], (double*)a.buf);
     if ((err = hipMemcpyAsync(a.f64[f], a.buf, n * sizeof(double), hipMemcpyDeviceToDevice, s)) != hipSuccess) return err;
   }
+  for (uint32_t f = 0; f < a.n_u32; ++f) {
+    hipLaunchKernelGGL(k_gather<uint32_t>, dim3(grid_for(n)), dim3(256), 0, s, n, (const uint32_t*)a.perm,
+                       (const uint32_t*)a.u32[f], (uint32_t*)a.buf);
+    if ((err = hipMemcpyAsync(a.u32[f], a.buf, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s)) != hipSuccess) return err;
+  }
   return hipGetLastError();
 }
 

@@ -468,10 +468,8 @@ __global__ void __launch_bounds__(BLK)
     uint32_t st;       // NL status bytes
     uint32_t pos[NL];
     double x[NV][NL];
-    double ad[NL], ah[NL];
   };
   auto load_sub = [&](const SubRef& q, Sub& a) {
-    const bool leave = LV && runs[q.r].enable_leave && (stages & STAGE_LEAVE);
     if constexpr (ILV) {
       uint32_t st = 0;
 #pragma unroll
@@ -481,11 +479,6 @@ __global__ void __launch_bounds__(BLK)
         a.pos[j] = ld_stream(gp(b.pos) + ij);
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.x[v][j] = ld_stream(gp(b.state) + (uint64_t)v * P + ij);
-        a.ad[j] = ld_stream(gp(b.age_div) + ij);
-        if constexpr (LV)
-          a.ah[j] = ld_stream(gp(b.age_hyd) + (leave ? ij : q.base));
-        else
-          a.ah[j] = 0.0;
       }
       a.st = st;
     } else {
@@ -494,13 +487,6 @@ __global__ void __launch_bounds__(BLK)
       ld_lane<NL>(gp(b.pos) + i0, a.pos);
 #pragma unroll
       for (int v = 0; v < NV; ++v) ld_lane<NL>(gp(b.state) + (uint64_t)v * P + i0, a.x[v]);
-      ld_lane<NL>(gp(b.age_div) + i0, a.ad);
-      if constexpr (LV) {
-        ld_lane<NL>(gp(b.age_hyd) + (leave ? i0 : q.base), a.ah);
-      } else {
-#pragma unroll
-        for (int j = 0; j < NL; ++j) a.ah[j] = 0.0;
-      }
     }
   };
   auto gather = [&](const SubRef& q, const Sub& a, double (&rr)[NL][NQ]) {
@@ -539,7 +525,7 @@ __global__ void __launch_bounds__(BLK)
   asm volatile("" : "+v"(A.st));
 #pragma unroll
   for (int j = 0; j < NL; ++j) {
-    asm volatile("" : "+v"(A.ad[j]), "+v"(A.ah[j]), "+v"(A.pos[j]));
+    asm volatile("" : "+v"(A.pos[j]));
 #pragma unroll
     for (int v = 0; v < NV; ++v) asm volatile("" : "+v"(A.x[v][j]));
 #pragma unroll
@@ -610,6 +596,11 @@ __global__ void __launch_bounds__(BLK)
       cnt_run = r;
     }
     const uint32_t step = rsr[r].step;
+    // the run's age clocks: the division clock of this step (after its tick
+    // when the model runs), the hydraulic clock before this step's tick and
+    // after it
+    const uint32_t clk_d = rsr[r].clock_div + (do_model ? 1u : 0u);
+    const uint32_t clk_h0 = rsr[r].clock_hyd;
     const double dt = rd.dt;
     const double w = rd.weight;
     const uint32_t n_c = rd.n_c;
@@ -621,6 +612,7 @@ __global__ void __launch_bounds__(BLK)
     for (int q = 0; q < NP; ++q) p[q] = rd.params[q];
     const bool do_move = rd.enable_move && (stages & STAGE_MOVE);
     const bool do_leave = LV && rd.enable_leave && (stages & STAGE_LEAVE);
+    const uint32_t clk_h = clk_h0 + (do_leave ? 1u : 0u);
     const uint32_t g_next = next_group(grp);
 
     // ---- prefetch: the wave's sub-chunk of the next group (a phantom past
@@ -635,7 +627,7 @@ __global__ void __launch_bounds__(BLK)
     const uint32_t i0 = NL * lane;  // contiguous layout: sub-chunk-relative slot of j = 0
     const bool pair_valid = i0 < cur.cnt;
     bool valid[NL], idle[NL], want[NL], mover[NL], exited[NL], dead[NL];
-    double x[NL][NV], ad[NL], ah[NL];
+    double x[NL][NV];
     double contrib[NL][NS];
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
@@ -645,17 +637,12 @@ __global__ void __launch_bounds__(BLK)
       want[j] = mover[j] = exited[j] = dead[j] = false;
 #pragma unroll
       for (int v = 0; v < NV; ++v) x[j][v] = A.x[v][j];
-      ad[j] = A.ad[j];
-      ah[j] = A.ah[j];
 #pragma unroll
       for (int q = 0; q < NS; ++q) contrib[j][q] = 0.0;
       if (!valid[j]) continue;
-      if (idle[j]) {
+      if (idle[j]) {  // its division age ticks with the run's clock
         int code = IDLE;
-        if (do_model) {
-          ad[j] += dt;
-          code = M::update(p, dt, x[j], rr[j], contrib[j]);
-        }
+        if (do_model) code = M::update(p, dt, x[j], rr[j], contrib[j]);
         if constexpr (M::CAN_DIE) dead[j] = code == DEAD;  // stops contributing and leaving
         want[j] = code == DIVISION;
       }
@@ -666,17 +653,17 @@ __global__ void __launch_bounds__(BLK)
       // neighbour and, for an Idle one, runs its exit test there
       const U4 u = draw4(seed, cur.slot0 + off(j), step, TAG_TRANSPORT, 0u);
       mover[j] = do_move && rr[j][NR] > u01(u.x);
-      if (do_leave && idle[j] && !dead[j]) {
-        ah[j] += dt;
+      if (do_leave && idle[j] && !dead[j]) {  // its hydraulic age ticks with the run's clock
         const double qx = rr[j][NR + 1];
         if (!mover[j] && qx < 1.0 && u01(u.z) > qx) {  // u > exp(-dt*Q/V)  <=>  dt*Q > -ln(u)*V
           if constexpr (PROBE) {
             if (rd.probe_exit) {
+              const uint64_t g = cur.base + off(j);
+              const double age = clock_age(gp(b.age_hyd)[g], clk_h, gp(b.clk_hyd)[g], dt);
               const unsigned long long e = atomicAdd(&b.rs[r].probe_exit_n, 1ull);
-              if (e < rd.probe_cap) rd.probe_exit[e] = ah[j];
+              if (e < rd.probe_cap) rd.probe_exit[e] = age;
             }
           }
-          ah[j] = 0.0;
           exited[j] = true;
         }
       }
@@ -753,24 +740,22 @@ __global__ void __launch_bounds__(BLK)
     }
     // ---- divisions in registers (rare): the first DIV_STAGE requests of the
     // sub-chunk are divided here, where the mother's state is at hand.  Her
-    // state before division and her division age are staged, one contiguous
-    // record per request; the halved mother and her reset division age are
-    // stored below with everything else.  k_resolve decides acceptance: an
-    // accepted request writes the daughter (M::division of the staged state
-    // again: the same draws, so the same daughter), a refused one restores
-    // the mother from the stage.  Later requests of a crowded sub-chunk are
-    // divided by k_resolve from the stored state.
+    // state before division is staged, one contiguous record per request; the
+    // halved mother is stored below with everything else.  k_resolve decides
+    // acceptance: an accepted request writes the daughter (M::division of the
+    // staged state again: the same draws, so the same daughter) and restarts
+    // the mother's division age, a refused one restores the mother from the
+    // stage.  Later requests of a crowded sub-chunk are divided by k_resolve
+    // from the stored state.
     if (n_div) {
 #pragma unroll
       for (int j = 0; j < NL; ++j) {
         if (!(want[j] && rd_[j] < DIV_STAGE)) continue;
-        GLOBAL_AS double* stg = gp(b.div_stage) + ((uint64_t)sc * DIV_STAGE + rd_[j]) * (NV + 1);
+        GLOBAL_AS double* stg = gp(b.div_stage) + ((uint64_t)sc * DIV_STAGE + rd_[j]) * NV;
 #pragma unroll
         for (int v = 0; v < NV; ++v) stg[v] = x[j][v];
-        stg[NV] = ad[j];
         double xd[NV];
         M::division(p, x[j], xd, RngKey{seed, cur.slot0 + off(j), step});
-        ad[j] = 0.0;
       }
     }
 
@@ -784,8 +769,6 @@ __global__ void __launch_bounds__(BLK)
         const uint64_t gj = cur.base + off(j);
 #pragma unroll
         for (int v = 0; v < NV; ++v) st_stream(gp(b.state) + (uint64_t)v * P + gj, x[j][v]);
-        st_stream(gp(b.age_div) + gj, ad[j]);
-        if (do_leave) st_stream(gp(b.age_hyd) + gj, ah[j]);
         if (exited[j] || dead[j]) gp(b.status)[gj] = dead[j] ? (uint8_t)DEAD : (uint8_t)EXIT;
       }
     } else {
@@ -796,8 +779,6 @@ __global__ void __launch_bounds__(BLK)
         for (int j = 0; j < NL; ++j) xv[j] = x[j][v];
         if (pair_valid) st_lane<NL>(gp(b.state) + (uint64_t)v * P + gi, xv);
       }
-      if (pair_valid) st_lane<NL>(gp(b.age_div) + gi, ad);
-      if (do_leave && pair_valid) st_lane<NL>(gp(b.age_hyd) + gi, ah);
       uint32_t stn = A.st;
       bool changed = false;
 #pragma unroll
@@ -807,6 +788,18 @@ __global__ void __launch_bounds__(BLK)
         changed = changed || exited[j] || dead[j];
       }
       if (changed && pair_valid) st_status<NL>(gp(b.status) + gi, stn);
+    }
+    // a slot that stops being Idle keeps its ages from now on: the hydraulic
+    // age (0 after an exit; a dead cell's without this step's tick) and, in
+    // clk_hyd, the division clock it stopped at (rare)
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      if (!(exited[j] || dead[j])) continue;
+      const uint64_t g = cur.base + off(j);
+      GLOBAL_AS double* a_h = gp(b.age_hyd) + g;
+      GLOBAL_AS uint32_t* c_h = gp(b.clk_hyd) + g;
+      *a_h = dead[j] ? clock_age(*a_h, clk_h0, *c_h, dt) : 0.0;
+      *c_h = clk_d;
     }
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
@@ -912,6 +905,9 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
           inactive = 0;
         }
         const uint32_t step = rs->step + 1u;
+        // the age clocks tick with the stages the step ran
+        if (b.stages & STAGE_MODEL) rs->clock_div += 1u;
+        if (rd.enable_leave && (b.stages & STAGE_LEAVE)) rs->clock_hyd += 1u;
         rs->n_end = n_end;
         rs->n_keep = n_keep;
         rs->compact = compact;
@@ -1087,12 +1083,14 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {
         const double qx = gp(rd.rec)[((size_t)(step & 1u) * rd.n_c + pos) * rd.rec_w + record_terms<M>() + 1];
         if (qx < 1.0 && u01(u.z) > qx) {
-          const double age = b.age_hyd[g];
-          if (rd.probe_exit) {
+          if (rd.probe_exit) {  // its hydraulic age after this step's tick
+            const double age = clock_age(b.age_hyd[g], rs->clock_hyd + 1u, b.clk_hyd[g], rd.dt);
             const unsigned long long k = atomicAdd(&rs->probe_exit_n, 1ull);
             if (k < rd.probe_cap) rd.probe_exit[k] = age;
           }
+          // no longer Idle: hydraulic age 0, the division clock it stopped at
           b.age_hyd[g] = 0.0;
+          b.clk_hyd[g] = rs->clock_div + ((b.stages & STAGE_MODEL) ? 1u : 0u);
           b.status[g] = EXIT;
           exits = true;
         }
@@ -1146,26 +1144,26 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         double p[NP];
 #pragma unroll
         for (int i = 0; i < NP; ++i) p[i] = rd.params[i];
-        double x[NV], xd[NV], age;
+        double x[NV], xd[NV];
         if (rk < DIV_STAGE) {
-          // divided by the pass: the mother's slot holds her halved state; the
-          // stage her state before division and her division age
-          const double* stg = b.div_stage + ((uint64_t)sc * DIV_STAGE + rk) * (NV + 1);
+          // divided by the pass: the mother's slot holds her halved state, the
+          // stage her state before division
+          const double* stg = b.div_stage + ((uint64_t)sc * DIV_STAGE + rk) * NV;
 #pragma unroll
           for (int v = 0; v < NV; ++v) x[v] = stg[v];
-          age = stg[NV];
           if (!accepted) {  // the mother as she was before the pass divided her
 #pragma unroll
             for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * b.P + g] = x[v];
-            b.age_div[g] = age;
             continue;
           }
         } else {
 #pragma unroll
           for (int v = 0; v < NV; ++v) x[v] = b.state[(uint64_t)v * b.P + g];
-          age = b.age_div[g];
         }
-        if (rd.probe_div) {
+        // requests come from the model stage: the division clock has ticked
+        const uint32_t clk_d = rs->clock_div + 1u;
+        if (rd.probe_div) {  // the mother's division age
+          const double age = clock_age(b.age_div[g], clk_d, b.clk_div[g], rd.dt);
           const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
           if (i < rd.probe_cap) rd.probe_div[i] = age;
         }
@@ -1176,12 +1174,16 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         if (rk >= DIV_STAGE) {  // the mother, divided here
 #pragma unroll
           for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * b.P + g] = x[v];
-          b.age_div[g] = 0.0;
         }
+        // the mother's division age restarts; the daughter's ages start at 0
+        b.age_div[g] = 0.0;
+        b.clk_div[g] = clk_d;
         b.pos[d] = b.div_pos[q];
         b.status[d] = IDLE;
         b.age_hyd[d] = 0.0;
         b.age_div[d] = 0.0;
+        b.clk_hyd[d] = rs->clock_hyd + ((rd.enable_leave && leave_stage) ? 1u : 0u);
+        b.clk_div[d] = clk_d;
       }
     }
     __syncthreads();
