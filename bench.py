@@ -313,6 +313,8 @@ def main():
     ap.add_argument("--particles", type=int, default=10_000_000)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--graph", type=int, default=0, help="replay a captured graph of this many steps")
+    ap.add_argument("--sort", type=int, default=0, metavar="K",
+                    help="reorder the 1e7-particle run by compartment on the device every K steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--served-requests", type=int, default=40)
@@ -357,6 +359,8 @@ def main():
     case.capacity = 2 * a.particles
     log(f"initialising {a.particles:.0e} particles")
     case.apply(run)
+    if a.sort:
+        run.set_sort_interval(a.sort)
     t_c = time.perf_counter()
     eng.commit()  # device allocation and the one host-to-device upload of particles, flowmaps and liquid
     t_commit = time.perf_counter() - t_c

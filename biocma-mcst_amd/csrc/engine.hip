@@ -464,6 +464,16 @@ struct mcst_engine {
   void* sort_mem = nullptr;         // scratch of the particle reorder, sized for sort_n slots
   uint64_t sort_n = 0;
   size_t sort_temp = 0;
+  // a second set of the pool's particle arrays (alt_P slots, in allocs): the
+  // reorder of an engine's only live run gathers into it and swaps it in
+  uint32_t* alt_pos = nullptr;
+  uint8_t* alt_status = nullptr;
+  double* alt_state = nullptr;
+  double* alt_age_hyd = nullptr;
+  double* alt_age_div = nullptr;
+  uint32_t* alt_clk_hyd = nullptr;
+  uint32_t* alt_clk_div = nullptr;
+  uint64_t alt_P = 0;
   int tuned_group = -1;
   int grid_move = 1;
   bool removal_possible = true;
@@ -524,6 +534,11 @@ void drop_graph(mcst_engine* e) {
 
 void release(mcst_engine* e) {
   drop_graph(e);
+  e->alt_pos = nullptr;  // freed with allocs
+  e->alt_status = nullptr;
+  e->alt_state = e->alt_age_hyd = e->alt_age_div = nullptr;
+  e->alt_clk_hyd = e->alt_clk_div = nullptr;
+  e->alt_P = 0;
   if (e->sort_mem) (void)hipFree(e->sort_mem);
   e->sort_mem = nullptr;
   e->sort_n = 0;
@@ -1189,15 +1204,23 @@ int read_state(mcst_engine* e, int32_t run, RunState& s) {
 // (read_state, the stream idle), from the device's base ages and age-clock
 // origins (clock_age, the kernels' expression): an Idle slot ages with its
 // run's clocks, another keeps the ages it had when it stopped being Idle.
-// Either output may be null; the bytes are copied, so they need no alignment.
+// The base ages are read into the outputs and converted there; an output
+// that is null or not 8-byte aligned goes through a host buffer.
 int read_ages(mcst_engine* e, uint64_t o, uint64_t n, const RunState& s, double dt, void* age_hyd, void* age_div) {
   const BatchDev& b = e->bd;
   std::vector<uint8_t> st(n);
-  std::vector<double> ah(n), ad(n);
   std::vector<uint32_t> ch(n), cd(n);
+  std::vector<double> th, td;
+  auto place = [&](void* out, std::vector<double>& tmp) -> double* {
+    if (out && (uintptr_t)out % alignof(double) == 0) return (double*)out;
+    tmp.resize(n);
+    return tmp.data();
+  };
+  double* ah = place(age_hyd, th);
+  double* ad = place(age_div, td);
   HIP_TRY(hipMemcpy(st.data(), b.status + o, n, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(ah.data(), b.age_hyd + o, n * sizeof(double), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(ad.data(), b.age_div + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ah, b.age_hyd + o, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ad, b.age_div + o, n * sizeof(double), hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(ch.data(), b.clk_hyd + o, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(cd.data(), b.clk_div + o, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   for (uint64_t q = 0; q < n; ++q) {
@@ -1208,13 +1231,52 @@ int read_ages(mcst_engine* e, uint64_t o, uint64_t n, const RunState& s, double 
       ad[q] = clock_age(ad[q], ch[q], cd[q], dt);
     }
   }
-  if (age_hyd) std::memcpy(age_hyd, ah.data(), n * sizeof(double));
-  if (age_div) std::memcpy(age_div, ad.data(), n * sizeof(double));
+  if (age_hyd && ah != age_hyd) std::memcpy(age_hyd, ah, n * sizeof(double));
+  if (age_div && ad != age_div) std::memcpy(age_div, ad, n * sizeof(double));
   return MCST_OK;
 }
 
+// Frees the second set of particle arrays (the stream idle).
+void release_alt(mcst_engine* e) {
+  void* arrays[] = {e->alt_pos, e->alt_status, e->alt_state, e->alt_age_hyd,
+                    e->alt_age_div, e->alt_clk_hyd, e->alt_clk_div};
+  for (void* p : arrays) {
+    if (!p) continue;
+    (void)hipFree(p);
+    e->allocs.erase(std::remove(e->allocs.begin(), e->allocs.end(), p), e->allocs.end());
+  }
+  e->alt_pos = nullptr;
+  e->alt_status = nullptr;
+  e->alt_state = e->alt_age_hyd = e->alt_age_div = nullptr;
+  e->alt_clk_hyd = e->alt_clk_div = nullptr;
+  e->alt_P = 0;
+}
+
+// The second set of particle arrays, sized like the pool's; false (and no
+// set) when the device cannot hold it.
+bool ensure_alt(mcst_engine* e) {
+  const uint64_t P = e->bd.P, nv = (uint64_t)e->model->nv;
+  if (e->alt_P == P) return true;
+  release_alt(e);
+  if (dev_alloc(e, (void**)&e->alt_pos, P * sizeof(uint32_t)) || dev_alloc(e, (void**)&e->alt_status, P) ||
+      dev_alloc(e, (void**)&e->alt_state, P * nv * sizeof(double)) ||
+      dev_alloc(e, (void**)&e->alt_age_hyd, P * sizeof(double)) ||
+      dev_alloc(e, (void**)&e->alt_age_div, P * sizeof(double)) ||
+      dev_alloc(e, (void**)&e->alt_clk_hyd, P * sizeof(uint32_t)) ||
+      dev_alloc(e, (void**)&e->alt_clk_div, P * sizeof(uint32_t))) {
+    release_alt(e);
+    return false;
+  }
+  e->alt_P = P;
+  return true;
+}
+
 // Reorders run r's particles by compartment, stable in slot order
-// (sort.hip).  Reads the particle count, so it waits for the stream.
+// (sort.hip).  Reads the particle count, so it waits for the stream.  The
+// engine's only live run is gathered once into the second set of arrays,
+// which then becomes the pool's (a captured graph, which holds the old
+// pointers, is dropped); a run of a batch is gathered into scratch and
+// copied back.
 int sort_run(mcst_engine* e, uint32_t r) {
   RunState st;
   int rc = read_state(e, (int32_t)r, st);
@@ -1240,7 +1302,7 @@ int sort_run(mcst_engine* e, uint32_t r) {
     e->sort_n = cap;
     e->sort_temp = temp;
   }
-  const BatchDev& b = e->bd;
+  BatchDev& b = e->bd;
   const uint64_t o = h.p_off, cap = e->sort_n;
   char* m = (char*)e->sort_mem;
   SortArgs a;
@@ -1262,7 +1324,30 @@ int sort_run(mcst_engine* e, uint32_t r) {
   a.buf = (void*)(((uintptr_t)(a.perm + cap) + 255) & ~(uintptr_t)255);
   a.temp = (char*)a.buf + cap * sizeof(double);
   a.temp_bytes = e->sort_temp;
+  uint32_t live = 0;
+  for (const auto& x : e->runs) live += x.phase == 1 ? 1u : 0u;
+  const bool swap = live == 1 && ensure_alt(e);
+  if (swap) {
+    a.to_dst = true;
+    a.dst_pos = e->alt_pos + o;
+    a.dst_status = e->alt_status + o;
+    for (uint32_t v = 0; v < nv; ++v) a.dst_f64[v] = e->alt_state + (uint64_t)v * b.P + o;
+    a.dst_f64[nv] = e->alt_age_hyd + o;
+    a.dst_f64[nv + 1] = e->alt_age_div + o;
+    a.dst_u32[0] = e->alt_clk_hyd + o;
+    a.dst_u32[1] = e->alt_clk_div + o;
+  }
   HIP_TRY(sort_run_particles(a, e->stream));
+  if (swap) {  // later launches read the sorted set; the old one is the next reorder's target
+    std::swap(b.pos, e->alt_pos);
+    std::swap(b.status, e->alt_status);
+    std::swap(b.state, e->alt_state);
+    std::swap(b.age_hyd, e->alt_age_hyd);
+    std::swap(b.age_div, e->alt_age_div);
+    std::swap(b.clk_hyd, e->alt_clk_hyd);
+    std::swap(b.clk_div, e->alt_clk_div);
+    drop_graph(e);
+  }
   return MCST_OK;
 }
 
@@ -1273,6 +1358,8 @@ int sort_run(mcst_engine* e, uint32_t r) {
 // leaves the engine as it was.
 int grow_pool(mcst_engine* e, uint64_t min_tiles) {
   BatchDev& b = e->bd;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  release_alt(e);  // sized for the old pool
   const uint64_t T0 = b.T_max, P0 = b.P;
   const uint64_t T1 = std::max<uint64_t>(min_tiles, 2 * T0), P1 = T1 * TILE;
   if (P1 >= 0xFFFFFFFFull) return fail(MCST_ERR_MEMORY, "more than 2^32-1 particle slots in one engine");

@@ -59,6 +59,13 @@ struct SortArgs {
   uint32_t n_f64 = 0;
   uint32_t* u32[2] = {};
   uint32_t n_u32 = 0;
+  // to_dst: every field is gathered once, straight into these arrays (the
+  // sorted order at index 0..n), and the source arrays are left as they were
+  bool to_dst = false;
+  uint32_t* dst_pos = nullptr;
+  uint8_t* dst_status = nullptr;
+  double* dst_f64[MAX_VAR + 2] = {};
+  uint32_t* dst_u32[2] = {};
   uint32_t *keys = nullptr, *iota = nullptr, *perm = nullptr;
   void* buf = nullptr;
   void* temp = nullptr;

@@ -18,8 +18,10 @@
 // slot order: a wave finds the lanes that share its digit with 8 ballots, its
 // lowest such lane counts them into LDS, and one thread per digit turns the
 // waves' counts into offsets.  Then every field is gathered through the
-// permutation into scratch and copied back (a run owns a slot range of arrays
-// the whole batch shares, so the sorted copy cannot replace the original).
+// permutation: for the only live run of an engine, in one pass straight into
+// a second set of the batch arrays, which then replaces the first (the
+// engine swaps the pointers); otherwise into scratch and copied back (a run
+// owns a slot range of arrays the whole batch shares).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -199,6 +201,43 @@ __global__ void __launch_bounds__(256) k_gather(uint64_t n, const uint32_t* __re
     dst[i] = src[perm[i]];
 }
 
+// Every field of the sorted order at once: one read of the permutation per
+// slot, the fields' loads issued four at a time before their stores.
+struct FieldSet {
+  const double* src_f64[MAX_VAR + 2];
+  double* dst_f64[MAX_VAR + 2];
+  const uint32_t* src_u32[2];
+  uint32_t* dst_u32[2];
+  const uint8_t* src_status;
+  uint8_t* dst_status;
+  uint32_t* dst_pos;
+  uint32_t n_f64, n_u32;
+};
+
+__global__ void __launch_bounds__(256) k_gather_fields(uint64_t n, const uint32_t* __restrict__ sorted_keys,
+                                                       const uint32_t* __restrict__ perm, FieldSet f) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = perm[i];
+    const uint8_t st = f.src_status[s];
+    uint32_t u[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) u[k] = k < (int)f.n_u32 ? f.src_u32[k][s] : 0u;
+    for (uint32_t k0 = 0; k0 < f.n_f64; k0 += 4) {
+      double v[4];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) v[j] = k0 + j < f.n_f64 ? f.src_f64[k0 + j][s] : 0.0;
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j)
+        if (k0 + j < f.n_f64) f.dst_f64[k0 + j][i] = v[j];
+    }
+    f.dst_pos[i] = sorted_keys[i];
+    f.dst_status[i] = st;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (k < (int)f.n_u32) f.dst_u32[k][i] = u[k];
+  }
+}
+
 int grid_for(uint64_t n) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 16384)); }
 
 struct SortLayout {
@@ -258,6 +297,26 @@ hipError_t sort_run_particles(const SortArgs& a, hipStream_t s) {
   }
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
+  if (a.to_dst) {  // one pass into the destination arrays
+    if (a.n_f64 > MAX_VAR + 2 || a.n_u32 > 2) return hipErrorInvalidValue;
+    FieldSet f{};
+    for (uint32_t k = 0; k < a.n_f64; ++k) {
+      f.src_f64[k] = a.f64[k];
+      f.dst_f64[k] = a.dst_f64[k];
+    }
+    for (uint32_t k = 0; k < a.n_u32; ++k) {
+      f.src_u32[k] = a.u32[k];
+      f.dst_u32[k] = a.dst_u32[k];
+    }
+    f.src_status = a.status;
+    f.dst_status = a.dst_status;
+    f.dst_pos = a.dst_pos;
+    f.n_f64 = a.n_f64;
+    f.n_u32 = a.n_u32;
+    hipLaunchKernelGGL(k_gather_fields, dim3(grid_for(n)), dim3(256), 0, s, n, (const uint32_t*)a.keys,
+                       (const uint32_t*)a.perm, f);
+    return hipGetLastError();
+  }
   // positions: the sorted keys; every other field through the permutation
   hipLaunchKernelGGL(k_gather_pos, dim3(grid_for(n)), dim3(256), 0, s, n, (const uint32_t*)a.keys, a.pos);
   hipLaunchKernelGGL(k_gather<uint8_t>, dim3(grid_for(n)), dim3(256), 0, s, n, (const uint32_t*)a.perm,

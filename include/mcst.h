@@ -300,7 +300,10 @@ int mcst_engine_get_steps(mcst_engine* e, uint32_t* steps, uint32_t n_runs);
 int mcst_run_get_counters(mcst_engine* e, int32_t run, uint64_t* counters /* MCST_C_COUNT */);
 int mcst_run_get_n(mcst_engine* e, int32_t run, uint64_t* n);
 int mcst_run_get_weight(mcst_engine* e, int32_t run, double* weight);
-/* The first n slots (n from mcst_run_get_n); any pointer may be NULL. */
+/* The first n slots (n from mcst_run_get_n); any pointer may be NULL.
+ * Ages are computed on the host from the device's base ages and age clocks:
+ * an age of k steps reads k*dt (one rounding) where the reference's running
+ * sum of dt differs in the last bits. */
 int mcst_run_get_particles(mcst_engine* e, int32_t run, uint32_t* pos, uint8_t* status, double* state,
                            double* age_hyd, double* age_div);
 int mcst_run_get_concentrations(mcst_engine* e, int32_t run, double* C);

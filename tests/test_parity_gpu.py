@@ -516,6 +516,55 @@ def test_reorder_radix_passes(n_c):
     eng.close()
 
 
+def test_reorder_of_a_run_in_a_batch():
+    """The reorder of a run that shares the engine's arrays with another
+    gathers through scratch and copies back; once it is the only run left, it
+    gathers into a second set of arrays that replaces the first.  Both runs
+    match their oracle runs at every check, the unsorted one included."""
+    import mcst
+    from mcst import cases
+    from oracle.oracle import OracleRun
+
+    eng = mcst.Engine("monod_mass")
+    eng.reserve(slots=80_000, runs=2)
+    pairs = []
+    for n_c, seed in ((12, 51), (40, 52)):
+        c = cases.small_case("monod_mass", n_c=n_c, n_particles=8000, seed=seed)
+        c.capacity = 30000
+        g = eng.add_run(c.n_c, c.dt, c.seed)
+        o = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+        for r in (g, o):
+            c.apply(r)
+            r.set_compaction(0.01, 0)
+        pairs.append((g, o))
+    (g0, o0), (g1, o1) = pairs
+    for r in (g0, o0):
+        r.set_sort_interval(4)
+    eng.commit()
+    for _ in range(3):
+        eng.step(5)
+        for g, o in pairs:
+            o.step(5)
+            assert_parity(g, o, probes=False)
+    g0.sort_particles()
+    o0.sort_particles()
+    for g, o in pairs:
+        assert_parity(g, o, probes=False)
+    eng.retire(g1)  # the sorted run is now the engine's only one
+    for _ in range(2):
+        eng.step(4)
+        o0.step(4)
+        assert_parity(g0, o0, probes=False)
+    g0.sort_particles()
+    o0.sort_particles()
+    assert_parity(g0, o0, probes=False)
+    assert np.all(np.diff(g0.particles()["pos"].astype(np.int64)) >= 0)
+    eng.step(3)
+    o0.step(3)
+    assert_parity(g0, o0, probes=False)
+    eng.close()
+
+
 def test_lattice50k_batch_of_two_window_runs():
     """Two lattice runs in one engine (their accumulators do not fit LDS, so
     the pass keeps a following window per block and switches windows between
