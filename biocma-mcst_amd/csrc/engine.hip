@@ -474,6 +474,11 @@ struct mcst_engine {
   uint32_t* alt_clk_hyd = nullptr;
   uint32_t* alt_clk_div = nullptr;
   uint64_t alt_P = 0;
+  // contribution rows of the particle pass (BatchDev::part), part_bytes
+  // allocated; part_aux: the resolve kernel's blocks that sum them
+  double* part_mem = nullptr;
+  size_t part_bytes = 0;
+  uint32_t part_aux = 0;
   int tuned_group = -1;
   int grid_move = 1;
   bool removal_possible = true;
@@ -534,6 +539,9 @@ void drop_graph(mcst_engine* e) {
 
 void release(mcst_engine* e) {
   drop_graph(e);
+  e->part_mem = nullptr;  // freed with allocs
+  e->part_bytes = 0;
+  e->part_aux = 0;
   e->alt_pos = nullptr;  // freed with allocs
   e->alt_status = nullptr;
   e->alt_state = e->alt_age_hyd = e->alt_age_div = nullptr;
@@ -757,6 +765,46 @@ int choose_launch(mcst_engine* e) {
   // block, so it meets few runs and few compartments
   b.tile_group = e->tuned_group >= 0 ? (uint32_t)e->tuned_group : (live <= 1 && !windowed ? 1u : 0u);
   if (const char* v = std::getenv("MCST_TILE_GROUP")) b.tile_group = (uint32_t)std::max(0, std::atoi(v));
+  // contribution rows (MCST_CONTRIB_ROWS=1; one live run whose whole
+  // accumulator is in LDS): each block of the pass stores its accumulator as
+  // a row and the resolve kernel sums the rows in a fixed order, instead of
+  // each block adding its accumulator to S with FP64 atomics.  On the
+  // 1e7-particle network the pass takes 0.156 ms instead of 0.167 but the
+  // resolve kernel 0.031 instead of 0.020 (the rows are written and read
+  // back: 41 MB each way), the same step, so the atomics stay the default.
+  b.part = nullptr;
+  b.part_rows = b.part_w = b.part_run = 0;
+  e->part_aux = 0;
+  bool rows = false;
+  if (const char* v = std::getenv("MCST_CONTRIB_ROWS")) rows = live == 1 && best.priv && !best.win && std::atoi(v) != 0;
+  if (rows) {
+    uint32_t r1 = 0;
+    for (uint32_t r = 0; r < e->runs.size(); ++r)
+      if (e->runs[r].phase == 1) r1 = r;
+    const HostRun& h1 = e->runs[r1];
+    const uint64_t w = (uint64_t)h1.n_s * h1.n_c;
+    const size_t bytes = (size_t)best.grid * w * sizeof(double);
+    if (bytes > e->part_bytes) {  // a larger row buffer
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      drop_graph(e);
+      if (e->part_mem) {
+        (void)hipFree(e->part_mem);
+        e->allocs.erase(std::remove(e->allocs.begin(), e->allocs.end(), (void*)e->part_mem), e->allocs.end());
+        e->part_mem = nullptr;
+        e->part_bytes = 0;
+      }
+      if (dev_alloc(e, (void**)&e->part_mem, bytes) == MCST_OK) e->part_bytes = bytes;
+    }
+    if (e->part_mem && bytes <= e->part_bytes) {
+      const uint32_t ns = (uint32_t)std::max(1, M.ns);
+      const uint32_t cpb = ns >= 16 ? 1u : 16u / ns;  // part_cpb<NS>() of the kernels
+      b.part = e->part_mem;
+      b.part_rows = (uint32_t)best.grid;
+      b.part_w = (uint32_t)w;
+      b.part_run = r1;
+      e->part_aux = (h1.n_c + cpb - 1) / cpb;
+    }
+  }
   e->plaunch = best;
   e->grid_particle = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)prop.multiProcessorCount * 4, b.T_max));
   // one thread per compartment, plus the block that lays out the tiles
@@ -771,8 +819,9 @@ int choose_launch(mcst_engine* e) {
   const uint64_t n_sc = (uint64_t)b.T_max * b.nsub;
   e->grid_move = (int)std::max<uint64_t>(1, std::min<uint64_t>(resident, (n_sc + 31) / 32));
   if (std::getenv("MCST_VERBOSE"))
-    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u leave=%d (lds cap %zu)\n",
-                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, (int)best.leave, lds_cap);
+    std::fprintf(stderr, "[mcst] particle pass: block=%d grid=%d lds=%zu priv=%d group=%u window=%u leave=%d rows=%d (lds cap %zu)\n",
+                 best.block, best.grid, best.lds, (int)best.priv, b.tile_group, b.acc_w, (int)best.leave,
+                 (int)(b.part != nullptr), lds_cap);
   return MCST_OK;
 }
 
@@ -1182,7 +1231,11 @@ void launch_step(mcst_engine* e, uint32_t stages, const hipEvent_t* ev = nullptr
   mark(1);
   ops.particle(b, e->plaunch, e->stream);
   mark(2);
-  ops.resolve(b, e->grid_move, ahead ? (uint32_t)(e->grid_transport - 1) : 0u, e->stream);
+  // aux blocks of the resolve kernel: with contribution rows, the blocks
+  // that sum them (and transport the next step with the sums when ahead);
+  // else, when ahead, the next step's transport
+  const uint32_t aux = b.part ? e->part_aux : ahead ? (uint32_t)(e->grid_transport - 1) : 0u;
+  ops.resolve(b, e->grid_move, aux, ahead, e->stream);
   mark(3);
   if (e->removal_possible) launch_removal(b, nv, e->grid_particle, e->stream);
   mark(4);

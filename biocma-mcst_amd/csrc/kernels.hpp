@@ -34,8 +34,10 @@ struct ModelOps {
   void (*transport)(const BatchDev& b, int grid, hipStream_t s);
   void (*particle)(const BatchDev& b, const ParticleLaunch& L, hipStream_t s);
   const void* (*particle_kernel)(int block, bool priv, bool probe, bool win, bool leave);
-  // resolve on `grid` blocks; with tr_blocks > 0, tr_blocks more transport the next step
-  void (*resolve)(const BatchDev& b, int grid, uint32_t tr_blocks, hipStream_t s);
+  // resolve: grid blocks plus aux blocks first in the grid, which transport
+  // the next step (ahead) and, with contribution rows, sum the rows first
+  // (for the next step's transport, or into S)
+  void (*resolve)(const BatchDev& b, int grid, uint32_t aux, bool ahead, hipStream_t s);
   const void* (*resolve_kernel)();  // for occupancy queries
   void (*properties)(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s);
 };

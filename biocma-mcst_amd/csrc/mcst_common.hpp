@@ -225,7 +225,23 @@ struct BatchDev {
   uint32_t tile_group;      // particle pass: consecutive tiles per block turn (0: one range per block)
   uint32_t acc_w;           // particle pass: compartments per LDS contribution window (PRIV)
   uint32_t stages;
+  // contribution rows (one live run whose whole [n_s][n_c] accumulator fits
+  // LDS): block k of the particle pass stores its accumulator at
+  // part[k * part_w] (part_rows blocks, part_w = n_s * n_c) and the resolve
+  // kernel sums the rows; null: the blocks add their accumulators to S with
+  // atomics
+  double* part;
+  uint32_t part_rows;
+  uint32_t part_w;
+  uint32_t part_run;        // the run the rows belong to
 };
+
+// Compartments per block of the row sum: (NS species x PART_CPB values) per
+// thread and row.
+template <int NS>
+constexpr uint32_t part_cpb() {
+  return NS >= 16 ? 1u : 16u / (uint32_t)NS;
+}
 
 // Age of a slot from its base age, its origin on the age clock and the
 // clock (the expression the host's conversion uses too).

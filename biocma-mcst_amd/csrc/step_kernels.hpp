@@ -125,25 +125,24 @@ __device__ __forceinline__ void column_inflow(const uint32_t* __restrict__ col_p
   }
 }
 
-// The transport of every active run's compartments for the step `ahead`
-// steps after the run's current one (0: the step about to run; 1: the next
-// one, computed during this step's resolve kernel), by blocks `blk` of
-// `n_blk`.  It reads C and the contributions S of the last pass and writes
-// the other C buffer, m, the record table of its step's parity and clears S;
-// the resolve kernel reads none of these but the other record table.
+// The transport of batch compartment g for the step `ahead` steps after its
+// run's current one (0: the step about to run; 1: the next one, computed
+// during this step's resolve kernel).  It reads C and the contributions of
+// the last pass, from S (s_in null; S is then cleared) or from s_in (the
+// compartment's NS summed contribution rows), and writes the other C buffer,
+// m and the record table of its step's parity; the resolve kernel reads none
+// of these but the other record table.
 template <class M>
-__device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t n_blk, uint32_t ahead) {
+__device__ void transport_one(const BatchDev& b, uint32_t g, uint32_t ahead, const double* s_in) {
   constexpr int NS = M::NS;
-  const uint32_t total = b.n_comp_total;
-  const uint32_t stride = n_blk * blockDim.x;
-  for (uint32_t g = blk * blockDim.x + threadIdx.x; g < total; g += stride) {
+  {
     const uint32_t r = find_run(b.comp_off, b.R, g);
     const RunDev& rd = b.runs[r];
     const RunState* rs = b.rs + r;
     // the step being prepared; the run's activity is recomputed here rather
     // than read from the layout, which other blocks write concurrently
     const uint32_t step = rs->step + ahead;
-    if (step >= rd.step_target) continue;
+    if (step >= rd.step_target) return;
     const uint32_t j = g - b.comp_off[r];
     const uint32_t n_c = rd.n_c;
     const MapDev& mp = b.maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
@@ -159,6 +158,9 @@ __device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t
 #pragma unroll
     for (int s = 0; s < NS; ++s) cn[s] = c[(size_t)s * n_c + j];
     if (b.stages & STAGE_TRANSPORT) {
+      double sv[NS];  // the contributions of the last pass
+#pragma unroll
+      for (int s = 0; s < NS; ++s) sv[s] = s_in ? s_in[s] : rd.S[(size_t)s * n_c + j];
       double acc[NS];
       column_inflow<NS>(mp.col_ptr, mp.src, mp.val, c, n_c, j, acc);
       double sink = rd.sink[j];
@@ -207,7 +209,7 @@ __device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t
           const double mgo = rd.mg[o] + rd.dt * dg;
           rd.mg[o] = mgo;
           Cgn[o] = mgo * inv_vg;
-          const double dl = ((acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]))) + mtr;
+          const double dl = ((acc[s] - cn[s] * sink) + (sv[s] + (rd.F[o] + fv[s]))) + mtr;
           const double mm = rd.m[o] + rd.dt * dl;
           rd.m[o] = mm;
           double cl = mm * inv_v;
@@ -218,7 +220,7 @@ __device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           const size_t o = (size_t)s * n_c + j;
-          double dmdt = (acc[s] - cn[s] * sink) + (rd.S[o] + (rd.F[o] + fv[s]));
+          double dmdt = (acc[s] - cn[s] * sink) + (sv[s] + (rd.F[o] + fv[s]));
           // gas-liquid transfer from a fixed gas phase: kla*(C* - C)*V
           if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];
           const double mm = rd.m[o] + rd.dt * dmdt;
@@ -235,7 +237,73 @@ __device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t
     for (int s = 0; s < NS; ++s) {
       const size_t o = (size_t)s * n_c + j;
       Cn[o] = cn[s];
-      rd.S[o] = 0.0;
+      if (!s_in) rd.S[o] = 0.0;
+    }
+  }
+}
+
+// The transport of every active run's compartments (transport_one from S),
+// by blocks `blk` of `n_blk`.
+template <class M>
+__device__ void transport_compartments(const BatchDev& b, uint32_t blk, uint32_t n_blk, uint32_t ahead) {
+  const uint32_t total = b.n_comp_total;
+  const uint32_t stride = n_blk * blockDim.x;
+  for (uint32_t g = blk * blockDim.x + threadIdx.x; g < total; g += stride) transport_one<M>(b, g, ahead, nullptr);
+}
+
+// With contribution rows: block `blk` of the resolve kernel's aux blocks sums
+// the rows of compartments [blk*CPB, (blk+1)*CPB) of the run, every species,
+// in a fixed order (each lane its rows in turn, then the wave's four lane
+// groups by a butterfly, then the waves in turn), then transports those compartments for the
+// next step with the sums (ahead) or stores the sums in S (the last step of a
+// call or of the run, so that S holds the step's contributions whenever the
+// host can read it).
+template <class M>
+__device__ void sum_rows(const BatchDev& b, uint32_t blk, bool ahead) {
+  constexpr int NS = M::NS;
+  constexpr uint32_t CPB = part_cpb<NS>();
+  constexpr uint32_t NE = (uint32_t)NS * CPB;  // 16: entries of the block, one per lane of a group of 16
+  constexpr uint32_t NWV = 256 / 64;
+  static_assert(NE <= 16, "at most 16 entries per block");
+  __shared__ double s_wsum[NWV][16];
+  __shared__ double s_sum[16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t r = b.part_run;
+  const RunDev& rd = b.runs[r];
+  const uint32_t n_c = rd.n_c;
+  const uint32_t c0 = blk * CPB;
+  // lane group g (16 lanes) takes rows g, g + 16, ...; lane e of the group
+  // the entry e = species s, compartment c0 + k: a wave's load covers four
+  // rows, 64 contiguous bytes of each per species
+  const uint32_t e = tid & 15u, grp16 = tid >> 4;
+  const uint32_t sp = e / CPB, k = e % CPB;
+  const bool on = e < NE && c0 + k < n_c;
+  const size_t off = on ? (size_t)sp * n_c + c0 + k : 0;
+  double acc = 0.0;
+#pragma unroll 8
+  for (uint32_t row = grp16; row < b.part_rows; row += 16u)
+    if (on) acc += b.part[(size_t)row * b.part_w + off];
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  if (lane < 16) s_wsum[wave][lane] = acc;
+  __syncthreads();
+  if (tid < 16) {
+    double t = 0.0;
+#pragma unroll
+    for (uint32_t w = 0; w < NWV; ++w) t += s_wsum[w][tid];
+    s_sum[tid] = t;
+  }
+  __syncthreads();
+  if (tid < CPB && c0 + tid < n_c) {
+    double sv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) sv[s] = s_sum[s * CPB + tid];
+    const uint32_t step = b.rs[r].step;  // the books advance it after every aux block is done
+    if (ahead && step + 1u < rd.step_target) {
+      transport_one<M>(b, b.comp_off[r] + c0 + tid, 1u, sv);
+    } else if (step < rd.step_target) {  // the run took this step, its last of the call or its last
+#pragma unroll
+      for (int s = 0; s < NS; ++s) rd.S[(size_t)s * n_c + c0 + tid] = sv[s];
     }
   }
 }
@@ -427,7 +495,15 @@ __global__ void __launch_bounds__(BLK)
   const uint32_t chunk = (n_groups + gridDim.x - 1) / gridDim.x;
   const uint32_t G = b.tile_group ? b.tile_group : (chunk ? chunk : 1u);
   uint32_t grp = blockIdx.x * G;
-  if (grp >= n_groups) return;  // the whole block
+  // contribution rows: this block's row of the pass (rows of all blocks are
+  // summed by the resolve kernel)
+  GLOBAL_AS double* const part_row =
+      (PRIV && !WIN && b.part && blockIdx.x < b.part_rows) ? gp(b.part) + (size_t)blockIdx.x * b.part_w : nullptr;
+  if (grp >= n_groups) {  // the whole block: its row holds nothing
+    if (part_row)
+      for (uint32_t i = threadIdx.x; i < b.part_w; i += BLK) part_row[i] = 0.0;
+    return;
+  }
   auto next_group = [&](uint32_t g) -> uint32_t {
     const uint32_t g0 = g - g % G;
     return (g + 1 < g0 + G) ? g + 1 : g0 + gridDim.x * G;
@@ -844,7 +920,11 @@ __global__ void __launch_bounds__(BLK)
   }
   if (PRIV) {
     __syncthreads();
-    flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
+    if (part_row) {  // the accumulator of the one run, every entry
+      for (uint32_t i = tid; i < b.part_w; i += BLK) part_row[i] = s_acc[i];
+    } else {
+      flush_window<BLK>(s_acc, acc_S, acc_ns, acc_win, acc_nc, acc_w0, tid);
+    }
   }
   if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
 }
@@ -979,16 +1059,19 @@ __device__ __forceinline__ uint32_t request_prefix(const BatchDev& b, uint32_t s
   return wave_sum(v + x2 + x1 + x0);
 }
 
-// The resolve kernel.  With tr_blocks > 0 the grid's first tr_blocks blocks
-// (dispatched first: their chain is the shorter) transport the liquid (and
-// gas) for the next step meanwhile: the two touch disjoint data (the records
-// of the two steps are separate tables, the concentrations are
-// double-buffered), and the block that keeps the books also lays out the
-// next step's tiles, so the next step starts with its particle pass.
+// The resolve kernel.  The grid's first tr_blocks blocks (dispatched first:
+// their chain is the shorter) are aux blocks.  With `ahead` they transport
+// the liquid (and gas) for the next step meanwhile: the two touch disjoint
+// data (the records of the two steps are separate tables, the concentrations
+// are double-buffered), and the block that keeps the books also lays out the
+// next step's tiles, so the next step starts with its particle pass.  With
+// contribution rows (b.part) they first sum the pass's rows, for that
+// transport or, without `ahead`, into S.
 template <class M>
 __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
                                                  const MapDev* __restrict__ maps,
-                                                 const uint32_t* __restrict__ tile_off, uint32_t tr_blocks) {
+                                                 const uint32_t* __restrict__ tile_off, uint32_t tr_blocks,
+                                                 uint32_t ahead) {
   constexpr int NV = M::NV;
   constexpr int NP = M::NP > 0 ? M::NP : 1;
   __shared__ uint32_t s_pre[RES_SC + 1];   // movers: prefix over the group's sub-chunks
@@ -1019,7 +1102,12 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   while (GS < RES_SC && (n_sc + 2 * GS - 1) / (2 * GS) >= n_res) GS *= 2;
   const uint32_t n_groups = (n_sc + GS - 1) / GS;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
-  if (tr_block) transport_compartments<M>(b, blockIdx.x, tr_blocks, 1u);
+  if (tr_block) {
+    if (b.part)
+      sum_rows<M>(b, blockIdx.x, ahead != 0);
+    else
+      transport_compartments<M>(b, blockIdx.x, tr_blocks, 1u);
+  }
   for (uint32_t grp = rb; !tr_block && grp < n_groups; grp += n_res) {
     const uint32_t sc0 = grp * GS;
     {  // the group's sub-chunks, one thread each: block-wide prefix sums
@@ -1219,7 +1307,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   __syncthreads();
   if (!s_last) return;
   bookkeep_pass(b, runs, n_sc);
-  if (tr_blocks) {
+  if (ahead) {
     __syncthreads();
     layout_pass<256>(b);
   }
@@ -1375,9 +1463,9 @@ void transport_t(const BatchDev& b, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_transport<M>, dim3(grid), dim3(256), 0, s, b);
 }
 template <class M>
-void resolve_t(const BatchDev& b, int grid, uint32_t tr_blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve<M>, dim3(grid + (int)tr_blocks), dim3(256), 0, s, b, b.runs, b.maps,
-                     (const uint32_t*)b.tile_off, tr_blocks);
+void resolve_t(const BatchDev& b, int grid, uint32_t aux, bool ahead, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve<M>, dim3(grid + (int)aux), dim3(256), 0, s, b, b.runs, b.maps,
+                     (const uint32_t*)b.tile_off, aux, ahead ? 1u : 0u);
 }
 template <class M>
 const void* resolve_kernel_t() {

@@ -565,6 +565,35 @@ def test_reorder_of_a_run_in_a_batch():
     eng.close()
 
 
+def test_contribution_rows_when_a_run_ends_inside_a_call(monkeypatch):
+    """With contribution rows (MCST_CONTRIB_ROWS=1) the next step's transport
+    takes the summed rows and S is written only on a step after which the
+    host can read it: the last of a call, or the run's last, here in the
+    middle of a call of 20 steps.  The contributions read afterwards are the
+    last pass's, as the oracle's."""
+    from mcst import cases
+
+    import mcst
+    from oracle.oracle import OracleRun
+
+    monkeypatch.setenv("MCST_CONTRIB_ROWS", "1")
+    c = cases.small_case("monod_mass", n_c=12, n_particles=6000, seed=61)
+    eng = mcst.Engine(c.model)
+    g = eng.add_run(c.n_c, c.dt, c.seed)
+    o = OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    for r in (g, o):
+        c.apply(r)
+        r.set_compaction(0.01, 0)
+    g.set_probes(1 << 20)
+    g.set_step_target(13)
+    eng.commit()
+    eng.step(20)
+    o.step(13)
+    assert g.counters()["steps"] == 13
+    assert_parity(g, o)
+    eng.close()
+
+
 def test_lattice50k_batch_of_two_window_runs():
     """Two lattice runs in one engine (their accumulators do not fit LDS, so
     the pass keeps a following window per block and switches windows between
@@ -593,10 +622,13 @@ def test_lattice50k_batch_of_two_window_runs():
 
 
 @pytest.mark.parametrize("env", [{"MCST_PARTICLE_BLOCK": "512"}, {"MCST_PARTICLE_BLOCK": "1024"},
-                                 {"MCST_CONTRIB_LDS": "0"}, {"MCST_TILE_GROUP": "0"}, {"MCST_TILE_GROUP": "4"}])
+                                 {"MCST_CONTRIB_LDS": "0"}, {"MCST_TILE_GROUP": "0"}, {"MCST_TILE_GROUP": "4"},
+                                 {"MCST_CONTRIB_ROWS": "1"}])
 def test_forced_launch_variants(env, monkeypatch):
     """Every particle-pass variant the autotuner or the overrides can select
-    (block size, global-atomic contributions, tile grouping) gives the same
+    (block size, global-atomic contributions, tile grouping, block
+    accumulators stored as rows and summed instead of added to S with
+    atomics) gives the same
     events as the oracle: on the small network (whole-run LDS accumulator)
     and on the lattice (compartment window)."""
     from mcst import cases
