@@ -934,21 +934,25 @@ __global__ void __launch_bounds__(BLK)
 // of update_and_remove_inactive (particles_container.hpp:539-557, threshold
 // from the pass's count) over [0, n_end) where the newborns already sit, and
 // the removal's tile layout; then the request block counts are cleared for
-// the next pass (n_sc: the sub-chunks of this step's layout).
-__device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs, uint32_t n_sc) {
+// the next pass (n_sc: the sub-chunks of this step's layout).  With `layout`
+// it also lays out the next step's tiles (layout_pass), from the counts it
+// has just computed.
+__device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs, uint32_t n_sc, bool layout) {
   constexpr uint32_t NT = 256, NWV = NT / 64;
-  __shared__ uint32_t s_w[NWV];
-  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_w[NWV], s_wl[NWV];
+  __shared__ uint32_t s_carry, s_carry_l;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  if (tid == 0) s_carry = 0;
+  if (tid == 0) s_carry = s_carry_l = 0;
   __syncthreads();
   for (uint32_t r0 = 0; r0 < b.R; r0 += NT) {
     const uint32_t r = r0 + tid;
-    uint32_t tiles = 0;  // tiles of [0, n_end) when the run compacts
+    uint32_t tiles = 0;   // tiles of [0, n_end) when the run compacts
+    uint32_t tiles_l = 0; // tiles of the next step's layout
     if (r < b.R) {
       RunState* rs = b.rs + r;
+      const RunDev& rd = runs[r];
+      uint32_t step_now, n_now;  // after this step's books
       if (rs->active) {
-        const RunDev& rd = runs[r];
         const uint32_t n = rs->n;
         // read and reset with atomics: the resolve kernel's blocks added to
         // the exit counter with atomics up to a moment ago
@@ -998,25 +1002,55 @@ __device__ void bookkeep_pass(const BatchDev& b, const RunDev* __restrict__ runs
         cnt[C_STEPS] = step;
         cnt[C_N] = n_keep;
         cnt[C_INACTIVE] = (uint32_t)inactive;
+        step_now = step;
+        n_now = n_keep;
       } else {
         rs->compact = 0;
+        step_now = rs->step;
+        n_now = rs->n;
+      }
+      if (layout) {  // layout_pass's table, from the values at hand
+        const uint32_t act = step_now < rd.step_target ? 1u : 0u;
+        tiles_l = act ? (n_now + TILE - 1u) / TILE : 0u;
+        rs->active = act;
+        rs->n_tiles = tiles_l;
       }
     }
     const uint32_t inc = wave_incl_scan(tiles, lane);
-    if (lane == 63) s_w[wave] = inc;
-    __syncthreads();
-    uint32_t wpre = 0, tot = 0;
-    for (uint32_t w = 0; w < NWV; ++w) {
-      if (w < wave) wpre += s_w[w];
-      tot += s_w[w];
+    const uint32_t inc_l = layout ? wave_incl_scan(tiles_l, lane) : 0u;
+    if (lane == 63) {
+      s_w[wave] = inc;
+      s_wl[wave] = inc_l;
     }
-    const uint32_t carry = s_carry;
-    if (r < b.R) b.rm_tile_off[r] = carry + wpre + inc - tiles;
     __syncthreads();
-    if (tid == 0) s_carry = carry + tot;
+    uint32_t wpre = 0, tot = 0, wpre_l = 0, tot_l = 0;
+    for (uint32_t w = 0; w < NWV; ++w) {
+      if (w < wave) {
+        wpre += s_w[w];
+        wpre_l += s_wl[w];
+      }
+      tot += s_w[w];
+      tot_l += s_wl[w];
+    }
+    const uint32_t carry = s_carry, carry_l = s_carry_l;
+    if (r < b.R) {
+      b.rm_tile_off[r] = carry + wpre + inc - tiles;
+      if (layout) {
+        b.tile_off[r] = carry_l + wpre_l + inc_l - tiles_l;
+        b.rs[r].tile_begin = carry_l + wpre_l + inc_l - tiles_l;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_carry = carry + tot;
+      s_carry_l = carry_l + tot_l;
+    }
     __syncthreads();
   }
-  if (tid == 0) b.rm_tile_off[b.R] = s_carry;
+  if (tid == 0) {
+    b.rm_tile_off[b.R] = s_carry;
+    if (layout) b.tile_off[b.R] = s_carry_l;
+  }
   for (uint32_t i = tid; i < (n_sc + 63u) >> 6; i += NT) b.div_l1[i] = 0u;
   for (uint32_t i = tid; i < (n_sc + 4095u) >> 12; i += NT) b.div_l2[i] = 0u;
   for (uint32_t i = tid; i <= (n_sc >> 18); i += NT) b.div_l3[i] = 0u;
@@ -1110,6 +1144,16 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   }
   for (uint32_t grp = rb; !tr_block && grp < n_groups; grp += n_res) {
     const uint32_t sc0 = grp * GS;
+    // the run of the group's first sub-chunk, its step and flowmap: nearly
+    // every mover of a group belongs to it, and these loads then overlap the
+    // group's counts and scan instead of heading each mover's chain
+    const uint32_t r0 = uniform(find_run(tile_off, R, sc0 / nsub));
+    const uint32_t step0 = b.rs[r0].step;
+    const RunDev& rd0 = runs[r0];
+    const MapDev& mp0 = maps[rd0.map_base + (step0 / rd0.steps_per_map) % rd0.n_maps];
+    const double* const cdf0 = mp0.cdf;
+    const uint32_t* const nbr0 = mp0.nbr;
+    const uint32_t m0 = mp0.m;
     {  // the group's sub-chunks, one thread each: block-wide prefix sums
       const uint32_t sc = sc0 + tid;
       uint32_t cm = 0, cd = 0, r = 0;
@@ -1160,12 +1204,19 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const uint32_t slot = (t - tile_off[r]) * TILE + (ms & (TILE - 1u));
       uint32_t pos = b.move_pos[q];
       const uint64_t g = rd.p_off + slot;
-      const uint32_t step = rs->step;
-      const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
+      uint32_t step = step0, m = m0;
+      const double* cdf = cdf0;
+      const uint32_t* nbr = nbr0;
+      if (r != r0) {  // a mover of a later run of the group
+        step = rs->step;
+        const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
+        cdf = mp.cdf;
+        nbr = mp.nbr;
+        m = mp.m;
+      }
       const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
-      const uint32_t m = mp.m;
-      const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
-      pos = gp(mp.nbr)[(size_t)pos * m + col];
+      const uint32_t col = cdf_column(gp(cdf) + (size_t)pos * m, m, u01(u.y));
+      pos = gp(nbr)[(size_t)pos * m + col];
       b.pos[g] = pos;
       bool exits = false;
       if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {
@@ -1289,7 +1340,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   const uint32_t nb = n_groups < n_res ? n_groups : n_res;  // resolve blocks with work
   const uint32_t np = nb + tr_blocks;                       // blocks that count themselves done
   if (np == 0) {
-    if (blockIdx.x == 0) bookkeep_pass(b, runs, n_sc);
+    if (blockIdx.x == 0) bookkeep_pass(b, runs, n_sc, ahead != 0);
     return;
   }
   if (!tr_block && rb >= nb) return;
@@ -1306,11 +1357,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   }
   __syncthreads();
   if (!s_last) return;
-  bookkeep_pass(b, runs, n_sc);
-  if (ahead) {
-    __syncthreads();
-    layout_pass<256>(b);
-  }
+  bookkeep_pass(b, runs, n_sc, ahead != 0);
 }
 
 // ------------------------------------------------------------------ layout
