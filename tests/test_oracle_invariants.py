@@ -507,3 +507,72 @@ def test_reorder_is_a_stable_sort_by_compartment(oracle_lib):
     for k in pa:
         np.testing.assert_array_equal(pa[k], pb[k])
     assert a.counters() == b.counters()
+
+
+def test_ages_count_the_steps_that_age_them(oracle_lib):
+    """The reference adds dt to an Idle particle's division age in every step
+    that runs the model (model_kernel.hpp:193) and to its hydraulic age in
+    every step that runs the leave stage of a run with a leaving flow
+    (move_kernel.hpp:347-359), and resets them at division and exit
+    (particles_container.hpp:559-573, move_kernel.hpp:633-636).  Without
+    divisions and exits, every Idle particle's ages are then exactly k
+    steps' worth; a step without the model (leave) stage leaves the division
+    (hydraulic) age where it was; a run without an outlet keeps hydraulic
+    age 0; and with divisions and exits every age and every recorded age is
+    a whole number of steps."""
+    from mcst import STAGE_ALL, STAGES
+    from mcst import cases
+
+    def ages_in_steps(p, dt):
+        k = np.asarray(p) / dt
+        assert np.all(np.abs(k - np.rint(k)) < 1e-9 * np.maximum(1.0, k))
+        return np.rint(k)
+
+    # slow growth, an outlet: no division within 40 steps, some exits
+    c = cases.small_case("monod_mass", n_c=12, n_particles=4000, seed=7, fast=False)
+    c.capacity = 20000
+    r = oracle_lib.OracleRun(c.model, c.n_c, c.n_s, c.dt, c.seed)
+    c.apply(r)
+    r.set_compaction(1.0, 1 << 40)  # no removal: exited slots stay
+    r.step(30)
+    for _ in range(4):  # no model stage: the division age stands still
+        r.step_stages(STAGE_ALL & ~STAGES["model"])
+    for _ in range(6):  # no leave stage: the hydraulic age stands still
+        r.step_stages(STAGE_ALL & ~STAGES["leave"])
+    cnt = r.counters()
+    assert cnt["births"] == 0 and cnt["exits"] > 0
+    p = r.particles()
+    idle = p["status"] == 0
+    assert np.all(ages_in_steps(p["age_div"][idle], c.dt) == 36)
+    assert np.all(ages_in_steps(p["age_hyd"][idle], c.dt) == 34)
+    np.testing.assert_allclose(p["age_div"][idle], 36 * c.dt, rtol=1e-13)
+    gone = p["status"] == 2  # exited: hydraulic age reset, division age frozen
+    assert gone.any() and np.all(p["age_hyd"][gone] == 0.0)
+    assert np.all(ages_in_steps(p["age_div"][gone], c.dt) <= 36)
+
+    # no outlet: the hydraulic age stays 0
+    c2 = cases.small_case("monod_mass", n_c=12, n_particles=2000, seed=8, fast=False, feeds=False)
+    r2 = oracle_lib.OracleRun(c2.model, c2.n_c, c2.n_s, c2.dt, c2.seed)
+    c2.apply(r2)
+    r2.step(12)
+    p2 = r2.particles()
+    assert np.all(p2["age_hyd"] == 0.0)
+    assert np.all(ages_in_steps(p2["age_div"], c2.dt) == 12)
+
+    # divisions and exits: whole numbers of steps everywhere
+    c3 = cases.small_case("monod_mass", n_c=12, n_particles=4000, seed=9)
+    c3.capacity = 60000
+    r3 = oracle_lib.OracleRun(c3.model, c3.n_c, c3.n_s, c3.dt, c3.seed)
+    c3.apply(r3)
+    r3.set_probes(1 << 20)
+    r3.step(25)
+    cnt3 = r3.counters()
+    assert cnt3["births"] > 0 and cnt3["exits"] > 0
+    p3 = r3.particles()
+    for f in ("age_div", "age_hyd"):
+        k = ages_in_steps(p3[f], c3.dt)
+        assert np.all((k >= 0) & (k <= 25))
+    for rec in (r3.exit_ages(), r3.division_ages()):
+        k = ages_in_steps(rec, c3.dt)
+        assert len(k) and np.all((k >= 1) & (k <= 25))
+
