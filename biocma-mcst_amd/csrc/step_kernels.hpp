@@ -581,6 +581,20 @@ __global__ void __launch_bounds__(BLK)
   // per-wave counters of the run cnt_run, flushed when the wave changes runs
   TileCounts cnt_acc;
   uint32_t cnt_run = 0xFFFFFFFFu;
+  // the wave's division requests summed per block of each level of the
+  // request counts (div_l1/2/3: blocks of 2^6, 2^12, 2^18 sub-chunks),
+  // added with one atomic when the wave leaves the block: its sub-chunks
+  // come in increasing order.  An atomic per sub-chunk with requests put
+  // most of a step's atomics on the few top-level entries when a tenth of
+  // the cells divide together (the lattice tank: 1e6 requests per step)
+  uint32_t dv_i[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, dv_n[3] = {0u, 0u, 0u};
+  auto flush_requests = [&](int lvl) {
+    if (dv_n[lvl] && lane == 0) {
+      uint32_t* const base = lvl == 0 ? b.div_l1 : lvl == 1 ? b.div_l2 : b.div_l3;
+      atomicAdd(base + dv_i[lvl], dv_n[lvl]);
+    }
+    dv_n[lvl] = 0u;
+  };
   // the LDS accumulator holds compartments [acc_w0, acc_w0 + acc_win) of run
   // acc_run: the whole run when it fits (acc_win = n_c), else a window
   // centred on the block's first slot of the run (particles are stored in
@@ -892,10 +906,16 @@ __global__ void __launch_bounds__(BLK)
     if (lane == 0 && sc < n_sc) {
       gp(b.sub_moves)[sc] = n_mov;
       gp(b.sub_divs)[sc] = n_div;
-      if (n_div) {  // rare: the block counts the resolve kernel's prefixes start from
-        atomicAdd(b.div_l1 + (sc >> 6), n_div);
-        atomicAdd(b.div_l2 + (sc >> 12), n_div);
-        atomicAdd(b.div_l3 + (sc >> 18), n_div);
+    }
+    if (n_div) {  // the block counts the resolve kernel's prefixes start from
+#pragma unroll
+      for (int lvl = 0; lvl < 3; ++lvl) {
+        const uint32_t i = sc >> (6 * (lvl + 1));
+        if (i != dv_i[lvl]) {
+          flush_requests(lvl);
+          dv_i[lvl] = i;
+        }
+        dv_n[lvl] += n_div;
       }
     }
     // ---- counters of the sub-chunk (wave-uniform)
@@ -927,6 +947,8 @@ __global__ void __launch_bounds__(BLK)
     }
   }
   if (cnt_run != 0xFFFFFFFFu && lane == 0) flush_counts(b.rs + cnt_run, cnt_acc);
+#pragma unroll
+  for (int lvl = 0; lvl < 3; ++lvl) flush_requests(lvl);
 }
 
 // The books of every run after its step, kept by the last block of
