@@ -1166,16 +1166,6 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   }
   for (uint32_t grp = rb; !tr_block && grp < n_groups; grp += n_res) {
     const uint32_t sc0 = grp * GS;
-    // the run of the group's first sub-chunk, its step and flowmap: nearly
-    // every mover of a group belongs to it, and these loads then overlap the
-    // group's counts and scan instead of heading each mover's chain
-    const uint32_t r0 = uniform(find_run(tile_off, R, sc0 / nsub));
-    const uint32_t step0 = b.rs[r0].step;
-    const RunDev& rd0 = runs[r0];
-    const MapDev& mp0 = maps[rd0.map_base + (step0 / rd0.steps_per_map) % rd0.n_maps];
-    const double* const cdf0 = mp0.cdf;
-    const uint32_t* const nbr0 = mp0.nbr;
-    const uint32_t m0 = mp0.m;
     {  // the group's sub-chunks, one thread each: block-wide prefix sums
       const uint32_t sc = sc0 + tid;
       uint32_t cm = 0, cd = 0, r = 0;
@@ -1226,19 +1216,12 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const uint32_t slot = (t - tile_off[r]) * TILE + (ms & (TILE - 1u));
       uint32_t pos = b.move_pos[q];
       const uint64_t g = rd.p_off + slot;
-      uint32_t step = step0, m = m0;
-      const double* cdf = cdf0;
-      const uint32_t* nbr = nbr0;
-      if (r != r0) {  // a mover of a later run of the group
-        step = rs->step;
-        const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
-        cdf = mp.cdf;
-        nbr = mp.nbr;
-        m = mp.m;
-      }
+      const uint32_t step = rs->step;
+      const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
       const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
-      const uint32_t col = cdf_column(gp(cdf) + (size_t)pos * m, m, u01(u.y));
-      pos = gp(nbr)[(size_t)pos * m + col];
+      const uint32_t m = mp.m;
+      const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
+      pos = gp(mp.nbr)[(size_t)pos * m + col];
       b.pos[g] = pos;
       bool exits = false;
       if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {
