@@ -103,7 +103,7 @@ __device__ __forceinline__ void column_inflow(const uint32_t* __restrict__ col_p
 #pragma unroll
   for (int s = 0; s < NS; ++s) acc[s] = 0.0;
   const uint32_t e0 = col_ptr[j], e1 = col_ptr[j + 1];
-  constexpr uint32_t EG = 8;
+  constexpr uint32_t EG = NS >= 4 ? 4u : 8u;  // entries in flight: fewer for many species (registers)
   for (uint32_t e = e0; e < e1; e += EG) {
     uint32_t i[EG];
     double v[EG], cv[EG][NS];
@@ -158,9 +158,6 @@ __device__ void transport_one(const BatchDev& b, uint32_t g, uint32_t ahead, con
 #pragma unroll
     for (int s = 0; s < NS; ++s) cn[s] = c[(size_t)s * n_c + j];
     if (b.stages & STAGE_TRANSPORT) {
-      double sv[NS];  // the contributions of the last pass
-#pragma unroll
-      for (int s = 0; s < NS; ++s) sv[s] = s_in ? s_in[s] : rd.S[(size_t)s * n_c + j];
       double acc[NS];
       column_inflow<NS>(mp.col_ptr, mp.src, mp.val, c, n_c, j, acc);
       double sink = rd.sink[j];
@@ -209,7 +206,8 @@ __device__ void transport_one(const BatchDev& b, uint32_t g, uint32_t ahead, con
           const double mgo = rd.mg[o] + rd.dt * dg;
           rd.mg[o] = mgo;
           Cgn[o] = mgo * inv_vg;
-          const double dl = ((acc[s] - cn[s] * sink) + (sv[s] + (rd.F[o] + fv[s]))) + mtr;
+          const double sv = s_in ? s_in[s] : rd.S[o];  // the contributions of the last pass
+          const double dl = ((acc[s] - cn[s] * sink) + (sv + (rd.F[o] + fv[s]))) + mtr;
           const double mm = rd.m[o] + rd.dt * dl;
           rd.m[o] = mm;
           double cl = mm * inv_v;
@@ -220,7 +218,8 @@ __device__ void transport_one(const BatchDev& b, uint32_t g, uint32_t ahead, con
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           const size_t o = (size_t)s * n_c + j;
-          double dmdt = (acc[s] - cn[s] * sink) + (sv[s] + (rd.F[o] + fv[s]));
+          const double sv = s_in ? s_in[s] : rd.S[o];  // the contributions of the last pass
+          double dmdt = (acc[s] - cn[s] * sink) + (sv + (rd.F[o] + fv[s]));
           // gas-liquid transfer from a fixed gas phase: kla*(C* - C)*V
           if (rd.kla) dmdt = dmdt + (rd.kla[o] * (rd.cstar[o] - cn[s])) * mp.vol[j];
           const double mm = rd.m[o] + rd.dt * dmdt;
@@ -1121,9 +1120,10 @@ __device__ __forceinline__ uint32_t request_prefix(const BatchDev& b, uint32_t s
 // data (the records of the two steps are separate tables, the concentrations
 // are double-buffered), and the block that keeps the books also lays out the
 // next step's tiles, so the next step starts with its particle pass.  With
-// contribution rows (b.part) they first sum the pass's rows, for that
-// transport or, without `ahead`, into S.
-template <class M>
+// contribution rows (ROWS: the variant launched when b.part is set; the
+// other one keeps no code of it, which would cost it registers) they first
+// sum the pass's rows, for that transport or, without `ahead`, into S.
+template <class M, bool ROWS>
 __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __restrict__ runs,
                                                  const MapDev* __restrict__ maps,
                                                  const uint32_t* __restrict__ tile_off, uint32_t tr_blocks,
@@ -1159,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
   const uint32_t n_groups = (n_sc + GS - 1) / GS;
   const bool leave_stage = (b.stages & STAGE_LEAVE) != 0;
   if (tr_block) {
-    if (b.part)
+    if constexpr (ROWS)
       sum_rows<M>(b, blockIdx.x, ahead != 0);
     else
       transport_compartments<M>(b, blockIdx.x, tr_blocks, 1u);
@@ -1516,12 +1516,16 @@ void transport_t(const BatchDev& b, int grid, hipStream_t s) {
 }
 template <class M>
 void resolve_t(const BatchDev& b, int grid, uint32_t aux, bool ahead, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve<M>, dim3(grid + (int)aux), dim3(256), 0, s, b, b.runs, b.maps,
-                     (const uint32_t*)b.tile_off, aux, ahead ? 1u : 0u);
+  if (b.part)
+    hipLaunchKernelGGL((k_resolve<M, true>), dim3(grid + (int)aux), dim3(256), 0, s, b, b.runs, b.maps,
+                       (const uint32_t*)b.tile_off, aux, ahead ? 1u : 0u);
+  else
+    hipLaunchKernelGGL((k_resolve<M, false>), dim3(grid + (int)aux), dim3(256), 0, s, b, b.runs, b.maps,
+                       (const uint32_t*)b.tile_off, aux, ahead ? 1u : 0u);
 }
 template <class M>
 const void* resolve_kernel_t() {
-  return (const void*)&k_resolve<M>;
+  return (const void*)&k_resolve<M, false>;
 }
 template <class M>
 void properties_t(const BatchDev& b, uint32_t run, uint64_t n, double* values, double* spatial, hipStream_t s) {
