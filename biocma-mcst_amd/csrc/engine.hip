@@ -704,7 +704,14 @@ int choose_launch(mcst_engine* e) {
   // window of compartments (particles outside it add to S directly)
   const bool windowed = lds_need > lds_cap;
   const size_t ns_b = sizeof(double) * std::max<uint32_t>(1, (uint32_t)M.ns);
-  lds_need = std::min(lds_need, (lds_cap / ns_b) * ns_b);
+  // a window is sized so that two blocks fit a CU's LDS (with 1 KB for the
+  // kernel's own shared variables): on the 50 000-compartment lattice two
+  // 512-thread blocks per CU with windows of 2496 compartments took 5.36 ms
+  // per step against 5.59 ms for one block with 3072
+  size_t win_cap = lds_cap;
+  if (windowed && prop.maxSharedMemoryPerMultiProcessor > 2 * 1024)
+    win_cap = std::min(lds_cap, (size_t)prop.maxSharedMemoryPerMultiProcessor / 2 - 1024);
+  lds_need = std::min(lds_need, ((windowed ? win_cap : lds_cap) / ns_b) * ns_b);
   bool priv = lds_need > 0;
   if (const char* v = std::getenv("MCST_CONTRIB_LDS")) priv = priv && std::atoi(v) != 0;
   b.acc_w = priv ? (uint32_t)(lds_need / ns_b) : 0u;
@@ -733,9 +740,12 @@ int choose_launch(mcst_engine* e) {
     // L1 keeps in flight, and more waves only share them (on the
     // 1e7-particle network 16 waves per CU took 0.204-0.214 ms against 0.167
     // ms for 8); on a tie the smaller block wins (more blocks, shorter
-    // flushes)
-    const int use = std::min(per_cu, std::max(1, 512 / blk));
-    const int waves = std::min(use * blk / 64, 8);
+    // flushes).  With a compartment window, up to 16 (two 512-thread
+    // blocks): its waves also wait on window flushes and global atomics of
+    // the particles outside the window
+    const int wcap = windowed ? 16 : 8;
+    const int use = std::min(per_cu, std::max(1, wcap * 64 / blk));
+    const int waves = std::min(use * blk / 64, wcap);
     if (per_cu > 0 && waves > best_waves) {
       best_waves = waves;
       best.block = blk;
