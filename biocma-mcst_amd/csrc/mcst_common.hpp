@@ -33,7 +33,15 @@ constexpr int MAX_SPECIES = 8;
 
 // Particle status byte (apps/libs/mc/public/mc/alias.hpp:124-130).
 enum Status : uint8_t { IDLE = 0, DIVISION = 1, EXIT = 2, DEAD = 3 };
-constexpr uint16_t MOVER_INACTIVE = 0x8000u;  // mover-list flag: the slot moves but does not leave
+// A mover-list entry (u32): the slot within its tile (bits 0-9), MOVER_INACTIVE
+// for a slot that moves but does not leave (not Idle: no exit test), and the
+// compartment before the move (bits 11-31), or MOVER_POS_FAR for a compartment
+// id that does not fit (the slot's position is then read: only the mover's own
+// thread writes it, after the read).
+constexpr uint32_t MOVER_INACTIVE = 1u << 10;
+constexpr uint32_t MOVER_POS_SHIFT = 11;
+constexpr uint32_t MOVER_POS_FAR = 0xFFFFFFFFu >> MOVER_POS_SHIFT;
+static_assert(TILE <= MOVER_INACTIVE, "a tile-local slot fits below the flag");
 
 // Stage mask of one step (values shared with the C ABI).
 enum Stage : uint32_t {
@@ -202,9 +210,9 @@ struct BatchDev {
   uint32_t* fill_src;       // [P] scratch list of filler slots during removal
   uint32_t* tile_off;       // [R+1] first global tile of each run in the current layout
   // lists of the particle pass, one segment of TILE/nsub entries per sub-chunk
-  uint16_t* move_slot;      // [T_max*TILE] tile-local slots of the particles leaving their compartment,
-                            // | MOVER_INACTIVE for a slot that is not Idle (no exit test)
-  uint32_t* move_pos;       // [T_max*TILE] their compartment before the move
+  uint32_t* move_rec;       // [T_max*TILE] particles leaving their compartment, one entry each
+                            // (slot, flag and compartment in one word, so that the few movers of a
+                            // sub-chunk share one line)
   uint16_t* div_slot;       // [T_max*TILE] tile-local slots of the division requests
   uint32_t* div_pos;        // [T_max*TILE] their compartment (the newborn's)
   double* div_stage;        // [T_max*nsub*DIV_STAGE][n_var] the first DIV_STAGE requests of each

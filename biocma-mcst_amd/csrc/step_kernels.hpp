@@ -894,8 +894,8 @@ __global__ void __launch_bounds__(BLK)
     for (int j = 0; j < NL; ++j) {
       if (mover[j]) {
         const bool leaves = idle[j] && !dead[j];
-        gp(b.move_slot)[seg + rm[j]] = (uint16_t)((slot_t + off(j)) | (leaves ? 0u : (uint32_t)MOVER_INACTIVE));
-        gp(b.move_pos)[seg + rm[j]] = A.pos[j];
+        const uint32_t pv = A.pos[j] < MOVER_POS_FAR ? A.pos[j] : MOVER_POS_FAR;
+        gp(b.move_rec)[seg + rm[j]] = (slot_t + off(j)) | (leaves ? 0u : MOVER_INACTIVE) | (pv << MOVER_POS_SHIFT);
       }
       if (n_div && want[j]) {  // rare
         gp(b.div_slot)[seg + rd_[j]] = (uint16_t)(slot_t + off(j));
@@ -1212,10 +1212,11 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const RunDev& rd = runs[r];
       RunState* rs = b.rs + r;
       const uint64_t q = (uint64_t)sc * sub + (e - s_pre[j]);
-      const uint32_t ms = b.move_slot[q];
+      const uint32_t ms = b.move_rec[q];
       const uint32_t slot = (t - tile_off[r]) * TILE + (ms & (TILE - 1u));
-      uint32_t pos = b.move_pos[q];
       const uint64_t g = rd.p_off + slot;
+      uint32_t pos = ms >> MOVER_POS_SHIFT;
+      if (pos == MOVER_POS_FAR) pos = b.pos[g];  // a compartment id of 21 bits or more
       const uint32_t step = rs->step;
       const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
       const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
