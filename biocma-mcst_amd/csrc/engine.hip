@@ -1156,7 +1156,7 @@ int commit(mcst_engine* e) {
   if ((rc = dev_alloc(e, (void**)&b.fill_src, P * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.tile_off, (RC + 1) * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.move_rec, T_max * TILE * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(e, (void**)&b.div_slot, T_max * TILE * sizeof(uint16_t)))) return rc;
+  if ((rc = dev_alloc(e, (void**)&b.div_rec, T_max * TILE * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_pos, T_max * TILE * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.div_stage, T_max * b.nsub * DIV_STAGE * nv * sizeof(double)))) return rc;
   if ((rc = dev_alloc(e, (void**)&b.sub_moves, T_max * b.nsub * sizeof(uint32_t)))) return rc;
@@ -1451,7 +1451,7 @@ int grow_pool(mcst_engine* e, uint64_t min_tiles) {
       {(void**)&b.tile_fill_off, T1 * 4, 0, 0, nullptr},
       {(void**)&b.fill_src, P1 * 4, 0, 0, nullptr},
       {(void**)&b.move_rec, P1 * 4, 0, 0, nullptr},
-      {(void**)&b.div_slot, P1 * 2, 0, 0, nullptr},
+      {(void**)&b.div_rec, P1 * 4, 0, 0, nullptr},
       {(void**)&b.div_pos, P1 * 4, 0, 0, nullptr},
       {(void**)&b.div_stage, T1 * b.nsub * DIV_STAGE * nv * 8, 0, 0, nullptr},
       {(void**)&b.sub_moves, T1 * b.nsub * 4, 0, 0, nullptr},

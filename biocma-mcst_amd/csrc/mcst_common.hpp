@@ -33,14 +33,16 @@ constexpr int MAX_SPECIES = 8;
 
 // Particle status byte (apps/libs/mc/public/mc/alias.hpp:124-130).
 enum Status : uint8_t { IDLE = 0, DIVISION = 1, EXIT = 2, DEAD = 3 };
-// A mover-list entry (u32): the slot within its tile (bits 0-9), MOVER_INACTIVE
-// for a slot that moves but does not leave (not Idle: no exit test), and the
-// compartment before the move (bits 11-31), or MOVER_POS_FAR for a compartment
-// id that does not fit (the slot's position is then read: only the mover's own
-// thread writes it, after the read).
+// An entry of the pass's lists (u32): the slot within its tile (bits 0-9),
+// MOVER_INACTIVE in a mover entry for a slot that moves but does not leave
+// (not Idle: no exit test), and a compartment (bits 11-31): a mover's before
+// the move, a division request's (the newborn's); LIST_POS_FAR for an id that
+// does not fit.  A mover's compartment is then read from the slot's position
+// (only the mover's own thread writes it, after the read), a request's from
+// div_pos.
 constexpr uint32_t MOVER_INACTIVE = 1u << 10;
-constexpr uint32_t MOVER_POS_SHIFT = 11;
-constexpr uint32_t MOVER_POS_FAR = 0xFFFFFFFFu >> MOVER_POS_SHIFT;
+constexpr uint32_t LIST_POS_SHIFT = 11;
+constexpr uint32_t LIST_POS_FAR = 0xFFFFFFFFu >> LIST_POS_SHIFT;
 static_assert(TILE <= MOVER_INACTIVE, "a tile-local slot fits below the flag");
 
 // Stage mask of one step (values shared with the C ABI).
@@ -213,8 +215,8 @@ struct BatchDev {
   uint32_t* move_rec;       // [T_max*TILE] particles leaving their compartment, one entry each
                             // (slot, flag and compartment in one word, so that the few movers of a
                             // sub-chunk share one line)
-  uint16_t* div_slot;       // [T_max*TILE] tile-local slots of the division requests
-  uint32_t* div_pos;        // [T_max*TILE] their compartment (the newborn's)
+  uint32_t* div_rec;        // [T_max*TILE] division requests, one entry each
+  uint32_t* div_pos;        // [T_max*TILE] their compartment (read for ids past the entry's field only)
   double* div_stage;        // [T_max*nsub*DIV_STAGE][n_var] the first DIV_STAGE requests of each
                             // sub-chunk, divided in the pass: the mother's state before division
                             // (to restore her if the request is refused)

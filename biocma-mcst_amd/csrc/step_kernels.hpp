@@ -894,11 +894,12 @@ __global__ void __launch_bounds__(BLK)
     for (int j = 0; j < NL; ++j) {
       if (mover[j]) {
         const bool leaves = idle[j] && !dead[j];
-        const uint32_t pv = A.pos[j] < MOVER_POS_FAR ? A.pos[j] : MOVER_POS_FAR;
-        gp(b.move_rec)[seg + rm[j]] = (slot_t + off(j)) | (leaves ? 0u : MOVER_INACTIVE) | (pv << MOVER_POS_SHIFT);
+        const uint32_t pv = A.pos[j] < LIST_POS_FAR ? A.pos[j] : LIST_POS_FAR;
+        gp(b.move_rec)[seg + rm[j]] = (slot_t + off(j)) | (leaves ? 0u : MOVER_INACTIVE) | (pv << LIST_POS_SHIFT);
       }
       if (n_div && want[j]) {  // rare
-        gp(b.div_slot)[seg + rd_[j]] = (uint16_t)(slot_t + off(j));
+        const uint32_t pv = A.pos[j] < LIST_POS_FAR ? A.pos[j] : LIST_POS_FAR;
+        gp(b.div_rec)[seg + rd_[j]] = (slot_t + off(j)) | (pv << LIST_POS_SHIFT);
         gp(b.div_pos)[seg + rd_[j]] = A.pos[j];
       }
     }
@@ -1215,8 +1216,8 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const uint32_t ms = b.move_rec[q];
       const uint32_t slot = (t - tile_off[r]) * TILE + (ms & (TILE - 1u));
       const uint64_t g = rd.p_off + slot;
-      uint32_t pos = ms >> MOVER_POS_SHIFT;
-      if (pos == MOVER_POS_FAR) pos = b.pos[g];  // a compartment id of 21 bits or more
+      uint32_t pos = ms >> LIST_POS_SHIFT;
+      if (pos == LIST_POS_FAR) pos = b.pos[g];  // a compartment id of 21 bits or more
       const uint32_t step = rs->step;
       const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
       const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
@@ -1284,7 +1285,8 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         const uint32_t rk = f - s_dpre[j];  // rank of the request in its sub-chunk
         if (rk >= DIV_STAGE && !accepted) continue;
         const uint64_t q = (uint64_t)sc * sub + rk;
-        const uint32_t slot = (t - tile_off[r]) * TILE + b.div_slot[q];
+        const uint32_t dw = b.div_rec[q];
+        const uint32_t slot = (t - tile_off[r]) * TILE + (dw & (TILE - 1u));
         const uint64_t g = rd.p_off + slot;
         double p[NP];
 #pragma unroll
@@ -1323,7 +1325,9 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         // the mother's division age restarts; the daughter's ages start at 0
         b.age_div[g] = 0.0;
         b.clk_div[g] = clk_d;
-        b.pos[d] = b.div_pos[q];
+        uint32_t dpos = dw >> LIST_POS_SHIFT;
+        if (dpos == LIST_POS_FAR) dpos = b.div_pos[q];  // a compartment id of 21 bits or more
+        b.pos[d] = dpos;
         b.status[d] = IDLE;
         b.age_hyd[d] = 0.0;
         b.age_div[d] = 0.0;

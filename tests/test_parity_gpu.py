@@ -517,10 +517,12 @@ def test_reorder_radix_passes(n_c):
 
 
 def test_compartment_ids_past_the_mover_entry():
-    """A mover's list entry holds its compartment before the move in 21 bits;
-    a larger id is marked there and read back from the slot's position.  On a
-    ring of 2^22 compartments about half the particles sit at ids of 2^21 - 1
-    and above; over 3 steps (about 8 500 moves) the run matches the oracle."""
+    """A list entry of the pass holds a compartment in 21 bits; a larger id is
+    marked there and read back from the slot's position (a mover) or from a
+    side list (a division request, the newborn's compartment).  On a ring of
+    2^22 compartments about half the particles sit at ids of 2^21 - 1 and
+    above; over 3 steps (about 8 500 moves, 570 births) the run matches the
+    oracle."""
     from mcst import cases
 
     c = cases.small_case("monod_mass", n_c=1 << 22, n_particles=20000, seed=43)
@@ -530,7 +532,7 @@ def test_compartment_ids_past_the_mover_entry():
     eng.step(3)
     o.step(3)
     assert_parity(g, o)
-    assert g.counters()["moves"] > 5000
+    assert g.counters()["moves"] > 5000 and g.counters()["births"] > 300
     assert (g.particles()["pos"] >= far).mean() > 0.4
     eng.close()
 
