@@ -1288,9 +1288,9 @@ int read_ages(mcst_engine* e, uint64_t o, uint64_t n, const RunState& s, double 
   for (uint64_t q = 0; q < n; ++q) {
     if (st[q] == IDLE) {
       ah[q] = clock_age(ah[q], s.clock_hyd, ch[q], dt);
-      ad[q] = clock_age(ad[q], s.clock_div, cd[q], dt);
+      ad[q] = div_age(ad[q], s.clock_div, cd[q], dt);
     } else {
-      ad[q] = clock_age(ad[q], ch[q], cd[q], dt);
+      ad[q] = div_age(ad[q], ch[q], cd[q], dt);
     }
   }
   if (age_hyd && ah != age_hyd) std::memcpy(age_hyd, ah, n * sizeof(double));

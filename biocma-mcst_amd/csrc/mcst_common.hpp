@@ -189,7 +189,8 @@ struct BatchDev {
   // its run's age clock, so a step writes them only when they restart;
   // a slot that is not Idle has hydraulic age age_hyd and division age
   // age_div + (clk_hyd - clk_div) * dt (clk_hyd holds the division clock at
-  // which it stopped being Idle)
+  // which it stopped being Idle).  clk_div is an origin word (div_age): a
+  // restart at 0 sets CLK_FRESH there and leaves age_div unread.
   double* age_hyd;
   double* age_div;
   uint32_t* clk_hyd;
@@ -257,6 +258,15 @@ constexpr uint32_t part_cpb() {
 // clock (the expression the host's conversion uses too).
 __host__ __device__ inline double clock_age(double base, uint32_t clock, uint32_t origin, double dt) {
   return base + (double)(clock - origin) * dt;
+}
+
+// A division-age origin word: the origin on the division clock in its low 31
+// bits (the age counts (clock - origin) mod 2^31 steps), and CLK_FRESH when
+// the age restarted there from 0, so that a restart is one store (age_div is
+// then not read).  Equal to clock_age for ages of fewer than 2^31 steps.
+constexpr uint32_t CLK_FRESH = 1u << 31;
+__host__ __device__ inline double div_age(double base, uint32_t clock, uint32_t word, double dt) {
+  return ((word & CLK_FRESH) ? 0.0 : base) + (double)((clock - word) & ~CLK_FRESH) * dt;
 }
 
 }  // namespace mcst

@@ -1310,7 +1310,8 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
         // requests come from the model stage: the division clock has ticked
         const uint32_t clk_d = rs->clock_div + 1u;
         if (rd.probe_div) {  // the mother's division age
-          const double age = clock_age(b.age_div[g], clk_d, b.clk_div[g], rd.dt);
+          const uint32_t w = b.clk_div[g];
+          const double age = div_age((w & CLK_FRESH) ? 0.0 : b.age_div[g], clk_d, w, rd.dt);
           const unsigned long long i = atomicAdd(&rs->probe_div_n, 1ull);
           if (i < rd.probe_cap) rd.probe_div[i] = age;
         }
@@ -1322,17 +1323,16 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
 #pragma unroll
           for (int v = 0; v < NV; ++v) b.state[(uint64_t)v * b.P + g] = x[v];
         }
-        // the mother's division age restarts; the daughter's ages start at 0
-        b.age_div[g] = 0.0;
-        b.clk_div[g] = clk_d;
+        // the mother's division age restarts (one store: an origin word with
+        // CLK_FRESH); the daughter's ages start at 0
+        b.clk_div[g] = clk_d | CLK_FRESH;
         uint32_t dpos = dw >> LIST_POS_SHIFT;
         if (dpos == LIST_POS_FAR) dpos = b.div_pos[q];  // a compartment id of 21 bits or more
         b.pos[d] = dpos;
         b.status[d] = IDLE;
         b.age_hyd[d] = 0.0;
-        b.age_div[d] = 0.0;
         b.clk_hyd[d] = rs->clock_hyd + ((rd.enable_leave && leave_stage) ? 1u : 0u);
-        b.clk_div[d] = clk_d;
+        b.clk_div[d] = clk_d | CLK_FRESH;
       }
     }
     __syncthreads();
