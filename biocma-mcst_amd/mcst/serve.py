@@ -44,11 +44,13 @@ class Result:
 
 
 def footprint(case, n_var):
-    """Device bytes a run holds while resident: its slot range (state, ages,
-    position, status and the pass's lists), its flowmaps and liquid arrays."""
+    """Device bytes a run holds while resident: its slot range (state, the
+    two base ages and their clock origins, position, status, the removal's
+    filler list and the pass's mover and division lists), its flowmaps and
+    liquid arrays."""
     cap = case.capacity or 2 * case.n_particles
     slots = -(-cap // TILE) * TILE
-    per_slot = 8 * n_var + 8 + 8 + 4 + 1 + 4 + 12
+    per_slot = 8 * n_var + 2 * 8 + 2 * 4 + 4 + 1 + 4 + 3 * 4
     maps = 0
     for V, rows, cols, flows in case.maps:
         deg = max(1, int(max(1, (len(flows) + case.n_c - 1) // case.n_c)))
