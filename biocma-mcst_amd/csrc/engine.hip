@@ -897,6 +897,18 @@ int upload_run(mcst_engine* e, uint32_t r) {
     if ((rc = run_upload(h, (double**)&md.q_exit, mp.q_exit.data(), mp.q_exit.size()))) return rc;
     if ((rc = run_upload(h, (double**)&md.cdf, mp.cdf.data(), mp.cdf.size()))) return rc;
     if ((rc = run_upload(h, (uint32_t**)&md.nbr, mp.nbr.data(), mp.nbr.size()))) return rc;
+    if (mp.m <= MOVE_ROW_M) {  // one 64-byte record per compartment for the movers
+      std::vector<uint32_t> rows((size_t)h.n_c * 16, 0u);
+      for (uint32_t i = 0; i < h.n_c; ++i) {
+        uint32_t* w = rows.data() + (size_t)i * 16;
+        for (uint32_t c = 0; c < 5; ++c) {
+          const double v = c + 1 < mp.m ? mp.cdf[(size_t)i * mp.m + c] : 2.0;
+          std::memcpy(w + 2 * c, &v, sizeof v);
+        }
+        for (uint32_t k = 0; k < MOVE_ROW_M; ++k) w[10 + k] = mp.nbr[(size_t)i * mp.m + (k < mp.m ? k : mp.m - 1)];
+      }
+      if ((rc = run_upload(h, (uint32_t**)&md.move_row, rows.data(), rows.size()))) return rc;
+    }
     if ((rc = run_upload(h, (uint32_t**)&md.col_ptr, mp.col_ptr.data(), mp.col_ptr.size()))) return rc;
     if ((rc = run_upload(h, (uint32_t**)&md.src, mp.src.data(), mp.src.size()))) return rc;
     if ((rc = run_upload(h, (double**)&md.val, mp.val.data(), mp.val.size()))) return rc;

@@ -91,9 +91,15 @@ struct MapDev {
   const double* g_val;
   const double* g_inv_v;    // [n_c]   1 / gas volume
   const double* g_kla;      // [n_s][n_c] kLa of the flowmap
+  // rows of at most MOVE_ROW_M neighbours (m <= MOVE_ROW_M), else null: one
+  // 64-byte record per compartment, cdf columns 0..4 (2.0 from column m-1
+  // on) then neighbours 0..5 as u32 (the last repeated), so that a mover
+  // reads one line where cdf and nbr take two
+  const uint32_t* move_row;
   uint32_t m;
   uint32_t pad;
 };
+constexpr uint32_t MOVE_ROW_M = 6;
 
 // vf_src_s / vf_sink_k entries that belong to the gas phase
 constexpr uint32_t VF_GAS = 0x80000000u;

@@ -1222,8 +1222,20 @@ __global__ void __launch_bounds__(256) k_resolve(BatchDev b, const RunDev* __res
       const MapDev& mp = maps[rd.map_base + (step / rd.steps_per_map) % rd.n_maps];
       const U4 u = draw4(rd.seed, slot, step, TAG_TRANSPORT, 0u);
       const uint32_t m = mp.m;
-      const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
-      pos = gp(mp.nbr)[(size_t)pos * m + col];
+      if (mp.move_row) {  // the compartment's 64-byte record: 5 cdf columns, 6 neighbours
+        const GLOBAL_AS u32x4* rw = (const GLOBAL_AS u32x4*)(gp(mp.move_row) + (size_t)pos * 16);
+        const u32x4 q0 = rw[0], q1 = rw[1], q2 = rw[2], q3 = rw[3];
+        const double uy = u01(u.y);
+        const uint32_t col = (uy > __hiloint2double((int)q0.y, (int)q0.x) ? 1u : 0u) +
+                             (uy > __hiloint2double((int)q0.w, (int)q0.z) ? 1u : 0u) +
+                             (uy > __hiloint2double((int)q1.y, (int)q1.x) ? 1u : 0u) +
+                             (uy > __hiloint2double((int)q1.w, (int)q1.z) ? 1u : 0u) +
+                             (uy > __hiloint2double((int)q2.y, (int)q2.x) ? 1u : 0u);
+        pos = col == 0 ? q2.z : col == 1 ? q2.w : col == 2 ? q3.x : col == 3 ? q3.y : col == 4 ? q3.z : q3.w;
+      } else {
+        const uint32_t col = cdf_column(gp(mp.cdf) + (size_t)pos * m, m, u01(u.y));
+        pos = gp(mp.nbr)[(size_t)pos * m + col];
+      }
       b.pos[g] = pos;
       bool exits = false;
       if (rd.enable_leave && leave_stage && !(ms & MOVER_INACTIVE)) {

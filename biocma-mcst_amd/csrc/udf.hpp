@@ -15,7 +15,7 @@
 
 namespace mcst {
 
-constexpr uint32_t UDF_ABI_VERSION = 8;
+constexpr uint32_t UDF_ABI_VERSION = 9;
 
 struct UdfModel {
   uint32_t abi_version;
